@@ -1,0 +1,209 @@
+/*
+ * ufe_dyn.h -- C-ABI of the MI355X ice-dynamics library (libufe_dyn.so).
+ *
+ * The library replaces the reference's ice-dynamics path
+ * (src/ice_dynamics_module.f90, src/general_ice_model_data_module.f90 and the
+ * mesh operators of src/mesh_derivatives_module.f90 and
+ * src/mesh_ArakawaC_module.f90) with hand-written CDNA4 HIP kernels.  Every
+ * entry point takes plain pointers and sizes and returns an int status.
+ *
+ * Layout at the boundary is the reference's own: host arrays are Fortran
+ * column-major, index arrays hold 1-based indices with 0 in unused slots,
+ * and 2-D widths are nC_mem or nC_mem+1 (src/data_types_module.f90:15-345).
+ * The library converts once, at mesh upload, into its device layouts.
+ *
+ * Threading: a context is bound to one HIP device and one stream and is not
+ * thread-safe; callers serialise (in the reference every routine is a
+ * collective over MPI ranks; here rank 0 calls and the others wait).
+ */
+#ifndef UFE_DYN_H
+#define UFE_DYN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (replace the reference's MPI_ABORT / warnings) -------- */
+enum {
+  UFE_OK = 0,
+  UFE_SOR_NOT_CONVERGED = 1,   /* src/ice_dynamics_module.f90:685-689 (warning) */
+  UFE_SSA_RESET = 2,           /* :679-684, one reset happened (informational) */
+  UFE_SSA_UNSTABLE = 3,        /* :533-540, second reset: reference aborts */
+  UFE_COLOURING_IDENTIFY = 4,  /* src/mesh_five_colour_module.f90:452: identify() aborts */
+  UFE_COLOURING_FAILED = 5,    /* src/mesh_five_colour_module.f90:168-171, 270-273, 329-341 */
+  UFE_BAD_INPUT = 6,
+  UFE_HIP_ERROR = 7,
+  UFE_NOT_SUPPORTED = 8
+};
+
+enum { UFE_FP64 = 0, UFE_MIXED = 1 };
+
+/* Mesh in reference layout.  Primary arrays are inputs everywhere; the
+ * secondary arrays are written by ufe_mesh_build and read by ufe_mesh_upload.
+ * Mirrors type_mesh, src/data_types_module.f90:216-345. */
+typedef struct ufe_mesh_desc {
+  int32_t nV, nTri, nC_mem, nAc;      /* nAc = (sum of nC) / 2 */
+  double xmin, xmax, ymin, ymax;
+  /* primary */
+  const double *V;                    /* (nV,2)        */
+  const int32_t *nC;                  /* (nV)          */
+  const int32_t *C;                   /* (nV,nC_mem)   */
+  const int32_t *niTri;               /* (nV)          */
+  const int32_t *iTri;                /* (nV,nC_mem)   */
+  const int32_t *edge_index;          /* (nV)          */
+  const int32_t *Tri;                 /* (nTri,3)      */
+  const double *A;                    /* (nV)   Voronoi cell areas */
+  const double *Cw;                   /* (nV,nC_mem) connection widths */
+  /* secondary */
+  double *Nx, *Ny, *Nxx, *Nxy, *Nyy;  /* (nV,nC_mem+1) */
+  int32_t *iAci;                      /* (nV,nC_mem)   */
+  int32_t *Aci;                       /* (nAc,4)       */
+  double *VAc;                        /* (nAc,2)       */
+  double *Nx_Ac, *Ny_Ac, *No_Ac;      /* (nAc,4)       */
+  double *Np_Ac;                      /* (nAc)         */
+  int32_t *edge_index_Ac;             /* (nAc)         */
+  double *VAaAc;                      /* (nVAaAc,2)    */
+  int32_t *nCAaAc;                    /* (nVAaAc)      */
+  int32_t *CAaAc;                     /* (nVAaAc,nC_mem) */
+  double *Nx_AaAc, *Ny_AaAc, *Nxx_AaAc, *Nxy_AaAc, *Nyy_AaAc; /* (nVAaAc,nC_mem+1) */
+  int32_t *colour;                    /* (nVAaAc)      */
+  int32_t *colour_vi;                 /* (nVAaAc,5)    */
+  int32_t colour_nV[5];
+} ufe_mesh_desc;
+
+/* Configuration subset, src/configuration_module.f90:124-126, 169-184. */
+typedef struct ufe_params {
+  int32_t nZ;
+  double zeta[32];
+  double m_enh_sia, m_enh_ssa;
+  double SSA_RN_tol;
+  int32_t SSA_max_outer_loops;
+  double SSA_max_residual_UV;
+  double SSA_SOR_omega;
+  int32_t SSA_max_inner_loops;
+  int32_t use_analytical_GL_flux;   /* not supported: returns UFE_NOT_SUPPORTED */
+  int32_t ssa_forced_zero;          /* EISMINT/Halfar/Bueler: SSA velocities zero (:431-449) */
+  int32_t zero_Hi_at_edges;         /* ice-thickness edge condition (:189-229) */
+  int32_t use_tau_c_override;       /* tau_c := field "tau_c_override_AaAc" after basal_yield_stress */
+  int32_t precision;                /* UFE_FP64 (reference arithmetic) or UFE_MIXED */
+  int32_t sweep_batch;              /* SOR sweeps queued per host convergence read; 0 = adaptive */
+} ufe_params;
+
+/* Statistics of one solve_SSA call. */
+typedef struct ufe_ssa_stats {
+  int32_t picard_iterations;
+  int32_t picard_converged;
+  int32_t resets;
+  int32_t total_sweeps;
+  double last_RN;
+  double last_max_residual;
+  int32_t sweeps[64];               /* SOR sweeps of the first 64 Picard iterations */
+  int32_t status;
+  double sor_seconds;               /* device time spent in SOR sweeps */
+  double total_seconds;
+} ufe_ssa_stats;
+
+/* Statistics of one linearised solve to a relative residual (bench path). */
+typedef struct ufe_linsolve_stats {
+  int32_t sweeps;                   /* SOR sweeps performed (FP32 sweeps in mixed mode) */
+  int32_t refinements;              /* FP64 residual evaluations */
+  double rel_residual;              /* ||b - A w||_2 / ||b||_2 in FP64 at exit */
+  double seconds;                   /* device time */
+  int32_t status;
+} ufe_linsolve_stats;
+
+typedef struct ufe_ctx ufe_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+int ufe_ctx_create(int device, ufe_ctx **out);
+int ufe_ctx_destroy(ufe_ctx *ctx);
+int ufe_set_params(ufe_ctx *ctx, const ufe_params *p);
+const char *ufe_last_error(ufe_ctx *ctx);
+int ufe_version(void);
+
+/* ---- mesh set-up -------------------------------------------------------- */
+/* Secondary mesh data from the primary arrays, in the order of
+ * src/restart_module.f90:89-102 / src/mesh_creation_module.f90:1725-1737:
+ * make_Ac_mesh (src/mesh_ArakawaC_module.f90:24-285), make_combined_AaAc_mesh
+ * (:286-540), get_neighbour_functions (src/mesh_derivatives_module.f90:14-75)
+ * and calculate_five_colouring_AaAc (src/mesh_five_colour_module.f90:18-137).
+ * Neighbour functions are computed on the device of ctx. */
+int ufe_mesh_build(ufe_ctx *ctx, ufe_mesh_desc *m);
+/* Upload a complete mesh (primary + secondary) and derive device layouts. */
+int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m);
+/* Sizes of the uploaded mesh: nV, nAc, nVAaAc, nC_mem. */
+int ufe_mesh_sizes(ufe_ctx *ctx, int32_t *out4);
+
+/* ---- fields ------------------------------------------------------------- */
+/* Ice-model fields (type_ice_model, src/data_types_module.f90:15-214) live on
+ * the device.  They are addressed by id; ids, names and grids can be listed.
+ * grid codes: 0 Aa(nV) 1 Aa x nZ 2 Aa x nC_mem 3 Ac(nAc) 4 Ac x nZ 5 AaAc(nVAaAc)
+ * dtype codes: 0 float64 1 int32 */
+int ufe_field_count(void);
+const char *ufe_field_name(int id);
+int ufe_field_id(const char *name);
+int ufe_field_info(int id, int32_t *grid, int32_t *dtype);
+int ufe_field_set(ufe_ctx *ctx, int id, const void *host);   /* reference layout */
+int ufe_field_get(ufe_ctx *ctx, int id, void *host);
+int ufe_field_set_device(ufe_ctx *ctx, int id, const void *dev);
+int ufe_field_get_device(ufe_ctx *ctx, int id, void *dev);
+
+/* ---- the hot path (operate on the device-resident fields) --------------- */
+/* update_general_ice_model_data, src/general_ice_model_data_module.f90:23-96
+ * (without ice_physical_properties: A_flow fields are inputs). */
+int ufe_update_general_ice_model_data(ufe_ctx *ctx);
+/* solve_SIA, src/ice_dynamics_module.f90:240-314 */
+int ufe_solve_SIA(ufe_ctx *ctx);
+/* solve_SSA, src/ice_dynamics_module.f90:408-557 (with basal_yield_stress
+ * :780-844, SSA_effective_viscosity :695-726, SSA_sliding_term :727-779 and
+ * solve_SSA_linearised :558-694) */
+int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st);
+/* calculate_ice_thickness_change, src/ice_dynamics_module.f90:31-237.
+ * SMB_year, BMB and mask_noice are the fields of the same names. */
+int ufe_calculate_ice_thickness_change(ufe_ctx *ctx, double dt);
+
+/* ---- linearised SSA at fixed viscosity (bench / mixed precision) -------- */
+/* Recompute eta, N, S and the SOR coefficients from the current velocities
+ * (one Picard step without the solve). */
+int ufe_ssa_prepare(ufe_ctx *ctx);
+/* Solve the linearised system to ||b - A w|| / ||b|| <= rel_tol (FP64 norm),
+ * in the precision of params.precision, starting from the current
+ * velocities. */
+int ufe_ssa_linear_solve(ufe_ctx *ctx, double rel_tol, int32_t max_sweeps,
+                         ufe_linsolve_stats *st);
+/* FP64 relative residual of the current velocities. */
+int ufe_ssa_residual(ufe_ctx *ctx, double *rel_residual);
+
+/* ---- operators (device pointers, original index order) ----------------- */
+/* src/mesh_derivatives_module.f90:315-432, 485-597 */
+int ufe_op_derivatives_Aa(ufe_ctx *ctx, const double *d, double *ddx, double *ddy);
+int ufe_op_curvatures_Aa(ufe_ctx *ctx, const double *d, double *ddxx, double *ddxy, double *ddyy);
+int ufe_op_derivatives_Aa_3D(ufe_ctx *ctx, const double *d, double *ddx, double *ddy, int nz);
+int ufe_op_neumann_Aa(ufe_ctx *ctx, double *d);
+int ufe_op_neumann_Aa_3D(ufe_ctx *ctx, double *d, int nz);
+/* src/mesh_ArakawaC_module.f90:542-844 */
+int ufe_op_derivatives_Ac(ufe_ctx *ctx, const double *d, double *ddx, double *ddy, double *ddp,
+                          double *ddo);
+int ufe_op_derivatives_AaAc(ufe_ctx *ctx, const double *d, double *ddx, double *ddy);
+int ufe_op_curvatures_AaAc(ufe_ctx *ctx, const double *d, double *ddxx, double *ddxy,
+                           double *ddyy);
+int ufe_op_neumann_AaAc(ufe_ctx *ctx, double *d);
+int ufe_op_map_Aa_to_Ac(ufe_ctx *ctx, const double *d_Aa, double *d_Ac);
+int ufe_op_map_Aa_to_Ac_3D(ufe_ctx *ctx, const double *d_Aa, double *d_Ac, int nz);
+int ufe_op_map_Ac_to_Aa(ufe_ctx *ctx, const double *d_Ac, double *d_Aa);
+int ufe_op_map_Ac_to_Aa_3D(ufe_ctx *ctx, const double *d_Ac, double *d_Aa, int nz);
+int ufe_op_rotate_xy_to_po(ufe_ctx *ctx, const double *x, const double *y, double *p, double *o);
+
+/* ---- device helpers for callers that manage their own buffers ----------- */
+int ufe_device_alloc(ufe_ctx *ctx, size_t bytes, void **dev);
+int ufe_device_free(ufe_ctx *ctx, void *dev);
+int ufe_memcpy_h2d(ufe_ctx *ctx, void *dev, const void *host, size_t bytes);
+int ufe_memcpy_d2h(ufe_ctx *ctx, void *host, const void *dev, size_t bytes);
+int ufe_synchronize(ufe_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,240 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Loaded by tests/, the smoke check and bench.py's CPU-baseline leg; the
+product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+from ufemism_amd import fields as F
+from ufemism_amd.config import DynParams
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libufe_oracle.so")
+
+def _count(fn):
+    lib = ct.CDLL(LIB_PATH)
+    f = getattr(lib, fn)
+    f.restype = ct.c_char_p
+    n = 0
+    while f(n) is not None:
+        n += 1
+    return n
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+N_MESH_D = _count("om_mesh_field_name_d")
+N_MESH_I = _count("om_mesh_field_name_i")
+N_ICE_D = _count("om_ice_field_name_d")
+N_ICE_I = _count("om_ice_field_name_i")
+
+
+class _Dims(ct.Structure):
+    _fields_ = [("nV", ct.c_int), ("nTri", ct.c_int), ("nC_mem", ct.c_int), ("nAc", ct.c_int),
+                ("nVAaAc", ct.c_int), ("xmin", ct.c_double), ("xmax", ct.c_double),
+                ("ymin", ct.c_double), ("ymax", ct.c_double), ("colour_nV", ct.c_int * 5)]
+
+
+class _Mesh(ct.Structure):
+    _fields_ = [("d", _Dims), ("f", ct.c_void_p * N_MESH_D), ("i", ct.c_void_p * N_MESH_I)]
+
+
+class _Ice(ct.Structure):
+    _fields_ = [("f", ct.c_void_p * N_ICE_D), ("i", ct.c_void_p * N_ICE_I)]
+
+
+class _Params(ct.Structure):
+    _fields_ = [("nZ", ct.c_int), ("zeta", ct.c_double * 32), ("m_enh_sia", ct.c_double),
+                ("m_enh_ssa", ct.c_double), ("SSA_RN_tol", ct.c_double),
+                ("SSA_max_outer_loops", ct.c_int), ("SSA_max_residual_UV", ct.c_double),
+                ("SSA_SOR_omega", ct.c_double), ("SSA_max_inner_loops", ct.c_int),
+                ("use_analytical_GL_flux", ct.c_int), ("ssa_forced_zero", ct.c_int),
+                ("zero_Hi_at_edges", ct.c_int), ("use_tau_c_override", ct.c_int)]
+
+
+class SSAStats(ct.Structure):
+    _fields_ = [("picard_iterations", ct.c_int), ("picard_converged", ct.c_int),
+                ("resets", ct.c_int), ("total_sweeps", ct.c_int), ("last_RN", ct.c_double),
+                ("last_max_residual", ct.c_double), ("sweeps", ct.c_int * 64),
+                ("status", ct.c_int)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+        _lib = ct.CDLL(LIB_PATH)
+        for fn in ("om_mesh_field_name_d", "om_mesh_field_name_i", "om_ice_field_name_d",
+                   "om_ice_field_name_i"):
+            getattr(_lib, fn).restype = ct.c_char_p
+        _lib.om_vertical_average.restype = ct.c_double
+    return _lib
+
+
+def _names(fn, n):
+    lib = load()
+    out = []
+    for k in range(n):
+        s = getattr(lib, fn)(k)
+        out.append(s.decode())
+    assert getattr(lib, fn)(n) is None
+    return out
+
+
+def params_struct(p: DynParams) -> _Params:
+    s = _Params()
+    s.nZ = p.nZ
+    for k, z in enumerate(p.zeta):
+        s.zeta[k] = float(z)
+    s.m_enh_sia, s.m_enh_ssa = p.m_enh_sia, p.m_enh_ssa
+    s.SSA_RN_tol, s.SSA_max_outer_loops = p.SSA_RN_tol, p.SSA_max_outer_loops
+    s.SSA_max_residual_UV, s.SSA_SOR_omega = p.SSA_max_residual_UV, p.SSA_SOR_omega
+    s.SSA_max_inner_loops = p.SSA_max_inner_loops
+    s.use_analytical_GL_flux = int(p.use_analytical_GL_flux)
+    s.ssa_forced_zero = int(p.ssa_forced_zero)
+    s.zero_Hi_at_edges = int(p.zero_Hi_at_edges)
+    s.use_tau_c_override = int(p.use_tau_c_override)
+    return s
+
+
+class OracleMesh:
+    """A primary mesh plus the secondary arrays the oracle builds from it."""
+
+    def __init__(self, mesh, nZ=15):
+        self.lib = load()
+        self.mesh = mesh
+        self.nZ = nZ
+        nV, nC_mem = mesh.nV, mesh.nC_mem
+        self.nAc = int(mesh.nC[:].sum() // 2)
+        self.nVAaAc = nV + self.nAc
+        self.arr = {
+            "V": mesh.V, "A": mesh.A, "Cw": mesh.Cw, "nC": mesh.nC, "C": mesh.C,
+            "niTri": mesh.niTri, "iTri": mesh.iTri, "edge_index": mesh.edge_index,
+            "Tri": mesh.Tri,
+        }
+        self.arr.update(F.alloc(F.MESH_SECONDARY, nV, self.nAc, nC_mem, nZ))
+        s = _Mesh()
+        s.d.nV, s.d.nTri, s.d.nC_mem = nV, mesh.nTri, nC_mem
+        s.d.nAc, s.d.nVAaAc = self.nAc, self.nVAaAc
+        s.d.xmin, s.d.xmax, s.d.ymin, s.d.ymax = mesh.xmin, mesh.xmax, mesh.ymin, mesh.ymax
+        for k, name in enumerate(_names("om_mesh_field_name_d", N_MESH_D)):
+            a = self.arr[name]
+            assert a.dtype == np.float64 and a.flags.f_contiguous, name
+            s.f[k] = a.ctypes.data
+        for k, name in enumerate(_names("om_mesh_field_name_i", N_MESH_I)):
+            a = self.arr[name]
+            assert a.dtype == np.int32 and a.flags.f_contiguous, name
+            s.i[k] = a.ctypes.data
+        self.s = s
+        self.colour_status = None
+
+    @property
+    def colour_nV(self):
+        return list(self.s.d.colour_nV)
+
+    def build(self, colour=True):
+        r = self.lib.om_make_Ac_mesh(ct.byref(self.s))
+        if r != 0:
+            raise RuntimeError(f"om_make_Ac_mesh failed ({r})")
+        self.lib.om_get_neighbour_functions(ct.byref(self.s))
+        if colour:
+            self.colour_status = self.lib.om_calculate_five_colouring_AaAc(ct.byref(self.s))
+        return self
+
+    def __getitem__(self, k):
+        return self.arr[k]
+
+
+class OracleIce:
+    """Host arrays of one ice state, bound to an OracleMesh."""
+
+    def __init__(self, om: OracleMesh):
+        self.om = om
+        self.lib = om.lib
+        self.arr = F.alloc(F.ICE_FIELDS, om.mesh.nV, om.nAc, om.mesh.nC_mem, om.nZ)
+        s = _Ice()
+        for k, name in enumerate(_names("om_ice_field_name_d", N_ICE_D)):
+            s.f[k] = self.arr[name].ctypes.data
+        for k, name in enumerate(_names("om_ice_field_name_i", N_ICE_I)):
+            s.i[k] = self.arr[name].ctypes.data
+        self.s = s
+
+    def __getitem__(self, k):
+        return self.arr[k]
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            self.arr[k][...] = v
+
+    def _m(self):
+        return ct.byref(self.om.s)
+
+    def update_general_ice_model_data(self):
+        self.lib.om_update_general_ice_model_data(self._m(), ct.byref(self.s))
+
+    def solve_SIA(self, p: DynParams):
+        ps = params_struct(p)
+        self.lib.om_solve_SIA(self._m(), ct.byref(self.s), ct.byref(ps))
+
+    def solve_SSA(self, p: DynParams) -> SSAStats:
+        ps = params_struct(p)
+        st = SSAStats()
+        self.lib.om_solve_SSA(self._m(), ct.byref(self.s), ct.byref(ps), ct.byref(st))
+        return st
+
+    def calculate_ice_thickness_change(self, SMB, BMB, dt, mask_noice, p: DynParams):
+        ps = params_struct(p)
+        SMB = np.ascontiguousarray(SMB, dtype=np.float64)
+        BMB = np.ascontiguousarray(BMB, dtype=np.float64)
+        mask = np.ascontiguousarray(mask_noice, dtype=np.int32)
+        self.lib.om_calculate_ice_thickness_change(
+            self._m(), ct.byref(self.s), SMB.ctypes.data_as(ct.c_void_p),
+            BMB.ctypes.data_as(ct.c_void_p), ct.c_double(dt),
+            mask.ctypes.data_as(ct.c_void_p), ct.byref(ps))
+
+
+def op(name, om: OracleMesh, *arrays, nz=None):
+    """Call one operator routine on contiguous float64 arrays (in place)."""
+    lib = load()
+    args = [ct.byref(om.s)] + [a.ctypes.data_as(ct.c_void_p) for a in arrays]
+    if nz is not None:
+        args.append(ct.c_int(nz))
+    getattr(lib, "om_" + name)(*args)
+
+
+def set_threads(n):
+    return load().om_set_threads(int(n))
+
+
+def run_step(wl, threads=None):
+    """The same single dynamics step as ufemism_amd.dynamics.Step, on the oracle."""
+    from ufemism_amd.dynamics import OUTPUTS, initial_fields
+    if threads:
+        set_threads(threads)
+    om = OracleMesh(wl.mesh).build()
+    if om.colour_status != 0:
+        raise RuntimeError(f"oracle colouring status {om.colour_status}")
+    ice = OracleIce(om)
+    for k, v in initial_fields(wl, om["VAaAc"], wl.params.nZ).items():
+        if k in ice.arr:
+            ice[k][...] = v
+    ice.update_general_ice_model_data()
+    ice.solve_SIA(wl.params)
+    st = ice.solve_SSA(wl.params)
+    ice.calculate_ice_thickness_change(wl.SMB, wl.BMB, wl.dt,
+                                       np.zeros(wl.mesh.nV, np.int32), wl.params)
+    out = {k: ice[k].copy() for k in OUTPUTS}
+    out["sweeps"] = list(st.sweeps)[: st.picard_iterations]
+    out["picard"] = st.picard_iterations
+    out["stats"] = st
+    out["mesh"] = om
+    return out

@@ -1,0 +1,281 @@
+"""ctypes binding of libufe_dyn.so (the C-ABI declared in include/ufe_dyn.h).
+
+The library is the product: every call here goes to the HIP implementation.
+If the shared object is missing, importing works but ``load()`` raises, so
+no caller can silently fall back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+from . import fields as F
+from .config import DynParams
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libufe_dyn.so")
+
+UFE_OK, UFE_SOR_NOT_CONVERGED, UFE_SSA_RESET, UFE_SSA_UNSTABLE = 0, 1, 2, 3
+UFE_COLOURING_IDENTIFY, UFE_COLOURING_FAILED, UFE_BAD_INPUT = 4, 5, 6
+UFE_HIP_ERROR, UFE_NOT_SUPPORTED = 7, 8
+UFE_FP64, UFE_MIXED = 0, 1
+
+STATUS_NAMES = {0: "OK", 1: "SOR_NOT_CONVERGED", 2: "SSA_RESET", 3: "SSA_UNSTABLE",
+                4: "COLOURING_IDENTIFY", 5: "COLOURING_FAILED", 6: "BAD_INPUT",
+                7: "HIP_ERROR", 8: "NOT_SUPPORTED"}
+
+# every symbol include/ufe_dyn.h declares
+EXPORTS = [
+    "ufe_ctx_create", "ufe_ctx_destroy", "ufe_set_params", "ufe_last_error", "ufe_version",
+    "ufe_mesh_build", "ufe_mesh_upload", "ufe_mesh_sizes",
+    "ufe_field_count", "ufe_field_name", "ufe_field_id", "ufe_field_info", "ufe_field_set",
+    "ufe_field_get", "ufe_field_set_device", "ufe_field_get_device",
+    "ufe_update_general_ice_model_data", "ufe_solve_SIA", "ufe_solve_SSA",
+    "ufe_calculate_ice_thickness_change", "ufe_ssa_prepare", "ufe_ssa_linear_solve",
+    "ufe_ssa_residual",
+    "ufe_op_derivatives_Aa", "ufe_op_curvatures_Aa", "ufe_op_derivatives_Aa_3D",
+    "ufe_op_neumann_Aa", "ufe_op_neumann_Aa_3D", "ufe_op_derivatives_Ac",
+    "ufe_op_derivatives_AaAc", "ufe_op_curvatures_AaAc", "ufe_op_neumann_AaAc",
+    "ufe_op_map_Aa_to_Ac", "ufe_op_map_Aa_to_Ac_3D", "ufe_op_map_Ac_to_Aa",
+    "ufe_op_map_Ac_to_Aa_3D", "ufe_op_rotate_xy_to_po",
+    "ufe_device_alloc", "ufe_device_free", "ufe_memcpy_h2d", "ufe_memcpy_d2h", "ufe_synchronize",
+]
+
+_P = ct.c_void_p
+
+
+class MeshDesc(ct.Structure):
+    _fields_ = [("nV", ct.c_int32), ("nTri", ct.c_int32), ("nC_mem", ct.c_int32), ("nAc", ct.c_int32),
+                ("xmin", ct.c_double), ("xmax", ct.c_double), ("ymin", ct.c_double), ("ymax", ct.c_double),
+                ("V", _P), ("nC", _P), ("C", _P), ("niTri", _P), ("iTri", _P), ("edge_index", _P),
+                ("Tri", _P), ("A", _P), ("Cw", _P),
+                ("Nx", _P), ("Ny", _P), ("Nxx", _P), ("Nxy", _P), ("Nyy", _P),
+                ("iAci", _P), ("Aci", _P), ("VAc", _P), ("Nx_Ac", _P), ("Ny_Ac", _P), ("No_Ac", _P),
+                ("Np_Ac", _P), ("edge_index_Ac", _P), ("VAaAc", _P), ("nCAaAc", _P), ("CAaAc", _P),
+                ("Nx_AaAc", _P), ("Ny_AaAc", _P), ("Nxx_AaAc", _P), ("Nxy_AaAc", _P), ("Nyy_AaAc", _P),
+                ("colour", _P), ("colour_vi", _P), ("colour_nV", ct.c_int32 * 5)]
+
+
+class Params(ct.Structure):
+    _fields_ = [("nZ", ct.c_int32), ("zeta", ct.c_double * 32), ("m_enh_sia", ct.c_double),
+                ("m_enh_ssa", ct.c_double), ("SSA_RN_tol", ct.c_double),
+                ("SSA_max_outer_loops", ct.c_int32), ("SSA_max_residual_UV", ct.c_double),
+                ("SSA_SOR_omega", ct.c_double), ("SSA_max_inner_loops", ct.c_int32),
+                ("use_analytical_GL_flux", ct.c_int32), ("ssa_forced_zero", ct.c_int32),
+                ("zero_Hi_at_edges", ct.c_int32), ("use_tau_c_override", ct.c_int32),
+                ("precision", ct.c_int32), ("sweep_batch", ct.c_int32)]
+
+
+class SSAStats(ct.Structure):
+    _fields_ = [("picard_iterations", ct.c_int32), ("picard_converged", ct.c_int32),
+                ("resets", ct.c_int32), ("total_sweeps", ct.c_int32), ("last_RN", ct.c_double),
+                ("last_max_residual", ct.c_double), ("sweeps", ct.c_int32 * 64),
+                ("status", ct.c_int32), ("sor_seconds", ct.c_double), ("total_seconds", ct.c_double)]
+
+
+class LinSolveStats(ct.Structure):
+    _fields_ = [("sweeps", ct.c_int32), ("refinements", ct.c_int32), ("rel_residual", ct.c_double),
+                ("seconds", ct.c_double), ("status", ct.c_int32)]
+
+
+_lib = None
+
+
+def load():
+    """Load libufe_dyn.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libufe_dyn.so not found at {LIB_PATH}; run "
+                               "`python -m ufemism_amd.build` (or __graft_entry__.build())")
+        lib = ct.CDLL(LIB_PATH)
+        lib.ufe_last_error.restype = ct.c_char_p
+        lib.ufe_field_name.restype = ct.c_char_p
+        lib.ufe_field_id.argtypes = [ct.c_char_p]
+        lib.ufe_calculate_ice_thickness_change.argtypes = [_P, ct.c_double]
+        lib.ufe_ssa_linear_solve.argtypes = [_P, ct.c_double, ct.c_int32, _P]
+        for name in EXPORTS:
+            getattr(lib, name)  # raises AttributeError if a symbol is missing
+        _lib = lib
+    return _lib
+
+
+def field_table():
+    lib = load()
+    out = {}
+    for i in range(lib.ufe_field_count()):
+        g, d = ct.c_int32(), ct.c_int32()
+        lib.ufe_field_info(i, ct.byref(g), ct.byref(d))
+        out[lib.ufe_field_name(i).decode()] = (i, g.value, d.value)
+    return out
+
+
+class UFEError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _ptr(a):
+    return None if a is None else ct.c_void_p(a.ctypes.data)
+
+
+def params_struct(p: DynParams, precision=UFE_FP64, sweep_batch=0) -> Params:
+    s = Params()
+    s.nZ = p.nZ
+    for k, z in enumerate(p.zeta):
+        s.zeta[k] = float(z)
+    s.m_enh_sia, s.m_enh_ssa = p.m_enh_sia, p.m_enh_ssa
+    s.SSA_RN_tol, s.SSA_max_outer_loops = p.SSA_RN_tol, p.SSA_max_outer_loops
+    s.SSA_max_residual_UV, s.SSA_SOR_omega = p.SSA_max_residual_UV, p.SSA_SOR_omega
+    s.SSA_max_inner_loops = p.SSA_max_inner_loops
+    s.use_analytical_GL_flux = int(p.use_analytical_GL_flux)
+    s.ssa_forced_zero = int(p.ssa_forced_zero)
+    s.zero_Hi_at_edges = int(p.zero_Hi_at_edges)
+    s.use_tau_c_override = int(p.use_tau_c_override)
+    s.precision = int(precision)
+    s.sweep_batch = int(sweep_batch)
+    return s
+
+
+class MeshArrays:
+    """Primary mesh plus secondary arrays in reference layout (host)."""
+
+    def __init__(self, mesh, nZ=15):
+        self.mesh = mesh
+        self.nV, self.nC_mem = mesh.nV, mesh.nC_mem
+        self.nAc = int(mesh.nC.sum() // 2)
+        self.nVAaAc = self.nV + self.nAc
+        self.arr = F.alloc(F.MESH_SECONDARY, self.nV, self.nAc, self.nC_mem, nZ)
+        self.colour_nV = [0] * 5
+
+    def __getitem__(self, k):
+        if k in self.arr:
+            return self.arr[k]
+        return getattr(self.mesh, k)
+
+    def desc(self) -> MeshDesc:
+        m, d = self.mesh, MeshDesc()
+        d.nV, d.nTri, d.nC_mem, d.nAc = m.nV, m.nTri, m.nC_mem, self.nAc
+        d.xmin, d.xmax, d.ymin, d.ymax = m.xmin, m.xmax, m.ymin, m.ymax
+        for name in ("V", "nC", "C", "niTri", "iTri", "edge_index", "Tri", "A", "Cw"):
+            a = getattr(m, name)
+            assert a.flags.f_contiguous, name
+            setattr(d, name, a.ctypes.data)
+        for name in F.MESH_SECONDARY:
+            setattr(d, name, self.arr[name].ctypes.data)
+        for k in range(5):
+            d.colour_nV[k] = self.colour_nV[k]
+        self._keep = d
+        return d
+
+
+class Context:
+    """One device context: uploaded mesh plus device-resident ice fields."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ct.c_void_p()
+        r = self.lib.ufe_ctx_create(int(device), ct.byref(h))
+        if r != UFE_OK:
+            raise UFEError(r, "ufe_ctx_create failed (no HIP device?)")
+        self.h = h
+        self.fields = field_table()
+        self.sizes = None
+
+    def close(self):
+        if self.h:
+            self.lib.ufe_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, r, what):
+        if r not in (UFE_OK,):
+            raise UFEError(r, f"{what}: {self.lib.ufe_last_error(self.h).decode()}")
+        return r
+
+    def set_params(self, p: DynParams, precision=UFE_FP64, sweep_batch=0):
+        self._params = params_struct(p, precision, sweep_batch)
+        self._chk(self.lib.ufe_set_params(self.h, ct.byref(self._params)), "ufe_set_params")
+
+    def mesh_build(self, mesh) -> MeshArrays:
+        ma = MeshArrays(mesh)
+        d = ma.desc()
+        r = self.lib.ufe_mesh_build(self.h, ct.byref(d))
+        ma.build_status = r
+        if r != UFE_OK:
+            raise UFEError(r, f"ufe_mesh_build: {self.lib.ufe_last_error(self.h).decode()}")
+        ma.colour_nV = list(d.colour_nV)
+        return ma
+
+    def mesh_upload(self, ma: MeshArrays):
+        d = ma.desc()
+        self._chk(self.lib.ufe_mesh_upload(self.h, ct.byref(d)), "ufe_mesh_upload")
+        s = (ct.c_int32 * 4)()
+        self.lib.ufe_mesh_sizes(self.h, s)
+        self.sizes = dict(nV=s[0], nAc=s[1], nVAaAc=s[2], nC_mem=s[3])
+        self.nZ = self._params.nZ if hasattr(self, "_params") else 15
+
+    def _shape(self, grid):
+        s = self.sizes
+        return {0: (s["nV"],), 1: (s["nV"], self.nZ), 2: (s["nV"], s["nC_mem"]),
+                3: (s["nAc"],), 4: (s["nAc"], self.nZ), 5: (s["nVAaAc"],)}[grid]
+
+    def set(self, name, value):
+        fid, grid, dt = self.fields[name]
+        a = np.asfortranarray(np.broadcast_to(np.asarray(value), self._shape(grid)),
+                              dtype=np.float64 if dt == 0 else np.int32)
+        self._chk(self.lib.ufe_field_set(self.h, fid, _ptr(a)), f"set {name}")
+
+    def get(self, name):
+        fid, grid, dt = self.fields[name]
+        a = np.zeros(self._shape(grid), dtype=np.float64 if dt == 0 else np.int32, order="F")
+        self._chk(self.lib.ufe_field_get(self.h, fid, _ptr(a)), f"get {name}")
+        return a
+
+    def update_general_ice_model_data(self):
+        self._chk(self.lib.ufe_update_general_ice_model_data(self.h), "update_general_ice_model_data")
+
+    def solve_SIA(self):
+        self._chk(self.lib.ufe_solve_SIA(self.h), "solve_SIA")
+
+    def solve_SSA(self, allow=(UFE_OK, UFE_SOR_NOT_CONVERGED)) -> SSAStats:
+        st = SSAStats()
+        r = self.lib.ufe_solve_SSA(self.h, ct.byref(st))
+        if r not in allow:
+            raise UFEError(r, f"solve_SSA: {self.lib.ufe_last_error(self.h).decode()}")
+        return st
+
+    def calculate_ice_thickness_change(self, dt):
+        self._chk(self.lib.ufe_calculate_ice_thickness_change(self.h, float(dt)),
+                  "calculate_ice_thickness_change")
+
+    def ssa_prepare(self):
+        self._chk(self.lib.ufe_ssa_prepare(self.h), "ssa_prepare")
+
+    def ssa_linear_solve(self, rel_tol, max_sweeps) -> LinSolveStats:
+        st = LinSolveStats()
+        self._chk(self.lib.ufe_ssa_linear_solve(self.h, float(rel_tol), int(max_sweeps),
+                                                ct.byref(st)), "ssa_linear_solve")
+        return st
+
+    def ssa_residual(self) -> float:
+        v = ct.c_double()
+        self._chk(self.lib.ufe_ssa_residual(self.h, ct.byref(v)), "ssa_residual")
+        return v.value
+
+    def op(self, name, *ptrs, nz=None):
+        """Call ufe_op_<name> on device pointers (ints)."""
+        args = [self.h] + [ct.c_void_p(int(p)) for p in ptrs]
+        if nz is not None:
+            args.append(ct.c_int(int(nz)))
+        self._chk(getattr(self.lib, "ufe_op_" + name)(*args), name)
+
+    def synchronize(self):
+        self._chk(self.lib.ufe_synchronize(self.h), "synchronize")

@@ -1,0 +1,1199 @@
+// ufe_dyn.hip -- context, device layouts, drivers and the C-ABI of libufe_dyn.
+//
+// The reference runs its ice-dynamics loops on MPI ranks sharing memory
+// windows (src/parallel_module.f90).  Here one context owns device-resident
+// copies of the mesh and of every ice-model field, and each reference routine
+// becomes a short sequence of kernels on the context's stream.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ufe_internal.h"
+
+#include "ufe_kernels.inc"
+
+namespace ufe {
+
+// ---------------------------------------------------------------------------
+// Field table (type_ice_model subset, src/data_types_module.f90:15-214)
+// ---------------------------------------------------------------------------
+enum Grid { G_AA = 0, G_AAZ = 1, G_AAC = 2, G_AC = 3, G_ACZ = 4, G_AAAC = 5 };
+enum DType { D_F64 = 0, D_I32 = 1 };
+struct FieldDef {
+  const char *name;
+  int grid, dtype;
+  int pair;    // -1, or index of an interleaved double2 store (0 W, 1 EV, 2 RHS)
+  int comp;
+};
+
+static const FieldDef FIELDS[] = {
+    // Aa
+    {"Hi", G_AA, D_F64, -1, 0}, {"Hb", G_AA, D_F64, -1, 0}, {"SL", G_AA, D_F64, -1, 0},
+    {"Hs", G_AA, D_F64, -1, 0}, {"dHb_dt", G_AA, D_F64, -1, 0}, {"dHi_dt", G_AA, D_F64, -1, 0},
+    {"dHs_dt", G_AA, D_F64, -1, 0}, {"Hi_prev", G_AA, D_F64, -1, 0},
+    {"A_flow_mean", G_AA, D_F64, -1, 0}, {"dHi_dx", G_AA, D_F64, -1, 0},
+    {"dHi_dy", G_AA, D_F64, -1, 0}, {"dHs_dx", G_AA, D_F64, -1, 0}, {"dHs_dy", G_AA, D_F64, -1, 0},
+    {"dHs_dx_shelf", G_AA, D_F64, -1, 0}, {"dHs_dy_shelf", G_AA, D_F64, -1, 0},
+    {"U_SIA", G_AA, D_F64, -1, 0}, {"V_SIA", G_AA, D_F64, -1, 0}, {"D_SIA", G_AA, D_F64, -1, 0},
+    {"U_SSA", G_AA, D_F64, -1, 0}, {"V_SSA", G_AA, D_F64, -1, 0},
+    {"SMB_year", G_AA, D_F64, -1, 0}, {"BMB", G_AA, D_F64, -1, 0},
+    {"A_flow", G_AAZ, D_F64, -1, 0}, {"dVi_in", G_AAC, D_F64, -1, 0},
+    // Ac
+    {"Hi_Ac", G_AC, D_F64, -1, 0}, {"Hb_Ac", G_AC, D_F64, -1, 0}, {"SL_Ac", G_AC, D_F64, -1, 0},
+    {"Hs_Ac", G_AC, D_F64, -1, 0},
+    {"dHi_dx_Ac", G_AC, D_F64, -1, 0}, {"dHi_dy_Ac", G_AC, D_F64, -1, 0},
+    {"dHi_dp_Ac", G_AC, D_F64, -1, 0}, {"dHi_do_Ac", G_AC, D_F64, -1, 0},
+    {"dHb_dx_Ac", G_AC, D_F64, -1, 0}, {"dHb_dy_Ac", G_AC, D_F64, -1, 0},
+    {"dHb_dp_Ac", G_AC, D_F64, -1, 0}, {"dHb_do_Ac", G_AC, D_F64, -1, 0},
+    {"dHs_dx_Ac", G_AC, D_F64, -1, 0}, {"dHs_dy_Ac", G_AC, D_F64, -1, 0},
+    {"dHs_dp_Ac", G_AC, D_F64, -1, 0}, {"dHs_do_Ac", G_AC, D_F64, -1, 0},
+    {"dSL_dx_Ac", G_AC, D_F64, -1, 0}, {"dSL_dy_Ac", G_AC, D_F64, -1, 0},
+    {"dSL_dp_Ac", G_AC, D_F64, -1, 0}, {"dSL_do_Ac", G_AC, D_F64, -1, 0},
+    {"dHs_dx_shelf_Ac", G_AC, D_F64, -1, 0}, {"dHs_dy_shelf_Ac", G_AC, D_F64, -1, 0},
+    {"A_flow_mean_Ac", G_AC, D_F64, -1, 0},
+    {"D_SIA_Ac", G_AC, D_F64, -1, 0}, {"Ux_SIA_Ac", G_AC, D_F64, -1, 0},
+    {"Uy_SIA_Ac", G_AC, D_F64, -1, 0}, {"Up_SIA_Ac", G_AC, D_F64, -1, 0},
+    {"Uo_SIA_Ac", G_AC, D_F64, -1, 0}, {"Ux_SSA_Ac", G_AC, D_F64, -1, 0},
+    {"Uy_SSA_Ac", G_AC, D_F64, -1, 0}, {"Up_SSA_Ac", G_AC, D_F64, -1, 0},
+    {"Uo_SSA_Ac", G_AC, D_F64, -1, 0},
+    {"A_flow_Ac", G_ACZ, D_F64, -1, 0}, {"D_SIA_3D_Ac", G_ACZ, D_F64, -1, 0},
+    // AaAc (stored in colour order)
+    {"Hi_AaAc", G_AAAC, D_F64, -1, 0}, {"Hb_AaAc", G_AAAC, D_F64, -1, 0},
+    {"SL_AaAc", G_AAAC, D_F64, -1, 0}, {"dHs_dx_shelf_AaAc", G_AAAC, D_F64, -1, 0},
+    {"dHs_dy_shelf_AaAc", G_AAAC, D_F64, -1, 0}, {"A_flow_mean_AaAc", G_AAAC, D_F64, -1, 0},
+    {"U_SSA_AaAc", G_AAAC, D_F64, 0, 0}, {"V_SSA_AaAc", G_AAAC, D_F64, 0, 1},
+    {"dU_SSA_dx_AaAc", G_AAAC, D_F64, -1, 0}, {"dU_SSA_dy_AaAc", G_AAAC, D_F64, -1, 0},
+    {"dV_SSA_dx_AaAc", G_AAAC, D_F64, -1, 0}, {"dV_SSA_dy_AaAc", G_AAAC, D_F64, -1, 0},
+    {"eta_AaAc", G_AAAC, D_F64, -1, 0}, {"N_AaAc", G_AAAC, D_F64, -1, 0},
+    {"N_AaAc_prev", G_AAAC, D_F64, -1, 0}, {"S_AaAc", G_AAAC, D_F64, -1, 0},
+    {"phi_fric_AaAc", G_AAAC, D_F64, -1, 0}, {"tau_c_AaAc", G_AAAC, D_F64, -1, 0},
+    {"RHSx_AaAc", G_AAAC, D_F64, 2, 0}, {"RHSy_AaAc", G_AAAC, D_F64, 2, 1},
+    {"eu_i_AaAc", G_AAAC, D_F64, 1, 0}, {"ev_i_AaAc", G_AAAC, D_F64, 1, 1},
+    {"tau_c_override_AaAc", G_AAAC, D_F64, -1, 0},
+    // masks
+    {"mask_land", G_AA, D_I32, -1, 0}, {"mask_ocean", G_AA, D_I32, -1, 0},
+    {"mask_lake", G_AA, D_I32, -1, 0}, {"mask_ice", G_AA, D_I32, -1, 0},
+    {"mask_sheet", G_AA, D_I32, -1, 0}, {"mask_shelf", G_AA, D_I32, -1, 0},
+    {"mask_coast", G_AA, D_I32, -1, 0}, {"mask_margin", G_AA, D_I32, -1, 0},
+    {"mask_gl", G_AA, D_I32, -1, 0}, {"mask_cf", G_AA, D_I32, -1, 0}, {"mask", G_AA, D_I32, -1, 0},
+    {"mask_land_Ac", G_AC, D_I32, -1, 0}, {"mask_ocean_Ac", G_AC, D_I32, -1, 0},
+    {"mask_lake_Ac", G_AC, D_I32, -1, 0}, {"mask_ice_Ac", G_AC, D_I32, -1, 0},
+    {"mask_sheet_Ac", G_AC, D_I32, -1, 0}, {"mask_shelf_Ac", G_AC, D_I32, -1, 0},
+    {"mask_coast_Ac", G_AC, D_I32, -1, 0}, {"mask_margin_Ac", G_AC, D_I32, -1, 0},
+    {"mask_gl_Ac", G_AC, D_I32, -1, 0}, {"mask_cf_Ac", G_AC, D_I32, -1, 0},
+    {"mask_Ac", G_AC, D_I32, -1, 0}, {"mask_noice", G_AA, D_I32, -1, 0},
+};
+static const int NFIELDS = (int)(sizeof(FIELDS) / sizeof(FIELDS[0]));
+
+static int field_id(const char *name) {
+  for (int i = 0; i < NFIELDS; i++)
+    if (std::strcmp(FIELDS[i].name, name) == 0) return i;
+  return -1;
+}
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace ufe
+
+using namespace ufe;
+
+struct ufe_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  ufe_params params{};
+  bool have_mesh = false;
+  // sizes
+  int nV = 0, nTri = 0, nCm = 0, nAc = 0, nA = 0, nZ = 15;
+  // Aa / Ac mesh (reference layout on the device)
+  int32_t *d_nC = nullptr, *d_C = nullptr, *d_edge = nullptr, *d_iAci = nullptr, *d_Crev = nullptr;
+  double *d_Nx = nullptr, *d_Ny = nullptr, *d_Nxx = nullptr, *d_Nxy = nullptr, *d_Nyy = nullptr;
+  double *d_A = nullptr, *d_Cw = nullptr, *d_V = nullptr;
+  int32_t *d_Aci = nullptr, *d_edgeAc = nullptr, *d_slot_i = nullptr, *d_slot_j = nullptr;
+  double *d_NxAc = nullptr, *d_NyAc = nullptr, *d_NoAc = nullptr, *d_NpAc = nullptr;
+  // AaAc colour order
+  int32_t *d_perm = nullptr;
+  std::vector<int32_t> h_perm;
+  int64_t *d_chunk_ptr = nullptr;
+  int32_t *d_chunk_w = nullptr, *d_col = nullptr;
+  uint8_t *d_deg = nullptr;
+  double *d_cNx = nullptr, *d_cNy = nullptr;
+  double2 *d_cU = nullptr;
+  double *d_hNx = nullptr, *d_hNy = nullptr, *d_hNxx = nullptr, *d_hNyy = nullptr;
+  double *d_Sxx = nullptr, *d_Sxy = nullptr, *d_Syy = nullptr;
+  int colour_start[6] = {0, 0, 0, 0, 0, 0};
+  int colour_active[5] = {0, 0, 0, 0, 0};
+  // Neumann lists (colour-order indices)
+  int32_t *d_neu_row = nullptr, *d_neu_ptr = nullptr, *d_neu_col = nullptr;
+  int32_t *d_cor_row = nullptr, *d_cor_ptr = nullptr, *d_cor_col = nullptr;
+  int n_neu = 0, n_cor = 0;
+  // Aa Neumann for the Aa operator: uses the Aa mesh directly
+  // fields
+  std::vector<void *> field_ptr;    // per field id (nullptr for pair members)
+  double2 *d_pair[3] = {nullptr, nullptr, nullptr};   // W, EV, RHS
+  // scratch
+  double *d_part = nullptr;         // block partial sums
+  int n_part = 0;
+  double *d_scalar = nullptr;       // small device scalars
+  double *d_tmp[4] = {nullptr, nullptr, nullptr, nullptr};   // nA-sized temporaries
+  double *d_ViSMB = nullptr;
+  unsigned long long *d_count = nullptr;
+  SorFlags *d_flags = nullptr;
+  SorFlags *h_flags = nullptr;      // pinned
+  double *h_scalar = nullptr;       // pinned
+  std::vector<void *> allocs;       // everything to free with the mesh
+};
+
+#define HIPCHK(ctx, x)                                                            \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      (ctx)->err = std::string(#x) + ": " + hipGetErrorString(e_);                \
+      return UFE_HIP_ERROR;                                                        \
+    }                                                                              \
+  } while (0)
+
+#define LAUNCHCHK(ctx)                                                            \
+  do {                                                                             \
+    hipError_t e_ = hipGetLastError();                                             \
+    if (e_ != hipSuccess) {                                                        \
+      (ctx)->err = std::string("kernel launch: ") + hipGetErrorString(e_);        \
+      return UFE_HIP_ERROR;                                                        \
+    }                                                                              \
+  } while (0)
+
+static inline int nblk(int64_t n, int b = BLOCK) { return (int)((n + b - 1) / b); }
+// grid for XCD-major kernels: a multiple of the 8 XCDs
+static inline int xcd_grid(int64_t n) { return (nblk(n) + 7) / 8 * 8; }
+
+template <typename T>
+static int dalloc(ufe_ctx *ctx, T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(ctx, hipMalloc((void **)p, n * sizeof(T)));
+  ctx->allocs.push_back((void *)*p);
+  return UFE_OK;
+}
+template <typename T>
+static int dupload(ufe_ctx *ctx, T **p, const T *h, size_t n) {
+  int r = dalloc(ctx, p, n);
+  if (r) return r;
+  if (n) HIPCHK(ctx, hipMemcpyAsync(*p, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return UFE_OK;
+}
+
+static void free_mesh(ufe_ctx *ctx) {
+  (void)hipStreamSynchronize(ctx->stream);
+  for (void *p : ctx->allocs) (void)hipFree(p);
+  ctx->allocs.clear();
+  ctx->field_ptr.clear();
+  ctx->have_mesh = false;
+}
+
+static size_t grid_len(const ufe_ctx *ctx, int grid) {
+  switch (grid) {
+    case G_AA: return (size_t)ctx->nV;
+    case G_AAZ: return (size_t)ctx->nV * ctx->nZ;
+    case G_AAC: return (size_t)ctx->nV * ctx->nCm;
+    case G_AC: return (size_t)ctx->nAc;
+    case G_ACZ: return (size_t)ctx->nAc * ctx->nZ;
+    case G_AAAC: return (size_t)ctx->nA;
+  }
+  return 0;
+}
+
+static double *F(ufe_ctx *ctx, const char *name) {
+  return (double *)ctx->field_ptr[field_id(name)];
+}
+static int32_t *FI(ufe_ctx *ctx, const char *name) {
+  return (int32_t *)ctx->field_ptr[field_id(name)];
+}
+
+static AaMesh aa_mesh(ufe_ctx *c) {
+  AaMesh m;
+  m.nV = c->nV;
+  m.nCm = c->nCm;
+  m.nC = c->d_nC;
+  m.C = c->d_C;
+  m.edge = c->d_edge;
+  m.iAci = c->d_iAci;
+  m.Crev = c->d_Crev;
+  m.Nx = c->d_Nx;
+  m.Ny = c->d_Ny;
+  m.Nxx = c->d_Nxx;
+  m.Nxy = c->d_Nxy;
+  m.Nyy = c->d_Nyy;
+  m.A = c->d_A;
+  m.Cw = c->d_Cw;
+  return m;
+}
+static AcMesh ac_mesh(ufe_ctx *c) {
+  AcMesh m;
+  m.nAc = c->nAc;
+  m.nV = c->nV;
+  m.Aci = c->d_Aci;
+  m.Nx = c->d_NxAc;
+  m.Ny = c->d_NyAc;
+  m.No = c->d_NoAc;
+  m.Np = c->d_NpAc;
+  m.edge = c->d_edgeAc;
+  m.slot_i = c->d_slot_i;
+  m.slot_j = c->d_slot_j;
+  m.V = c->d_V;
+  return m;
+}
+static AaAcMesh aaac_mesh(ufe_ctx *c) {
+  AaAcMesh m;
+  m.nA = c->nA;
+  m.nV = c->nV;
+  m.perm = c->d_perm;
+  m.chunk_ptr = c->d_chunk_ptr;
+  m.chunk_w = c->d_chunk_w;
+  m.deg = c->d_deg;
+  m.col = c->d_col;
+  m.cNx = c->d_cNx;
+  m.cNy = c->d_cNy;
+  m.cU = c->d_cU;
+  m.hNx = c->d_hNx;
+  m.hNy = c->d_hNy;
+  m.hNxx = c->d_hNxx;
+  m.hNyy = c->d_hNyy;
+  m.Sxx = c->d_Sxx;
+  m.Sxy = c->d_Sxy;
+  m.Syy = c->d_Syy;
+  return m;
+}
+static SSAFields ssa_fields(ufe_ctx *c) {
+  SSAFields f;
+  f.Hi = F(c, "Hi_AaAc");
+  f.Hb = F(c, "Hb_AaAc");
+  f.SL = F(c, "SL_AaAc");
+  f.dsx = F(c, "dHs_dx_shelf_AaAc");
+  f.dsy = F(c, "dHs_dy_shelf_AaAc");
+  f.Aflow = F(c, "A_flow_mean_AaAc");
+  f.ux = F(c, "dU_SSA_dx_AaAc");
+  f.uy = F(c, "dU_SSA_dy_AaAc");
+  f.vx = F(c, "dV_SSA_dx_AaAc");
+  f.vy = F(c, "dV_SSA_dy_AaAc");
+  f.eta = F(c, "eta_AaAc");
+  f.N = F(c, "N_AaAc");
+  f.Nprev = F(c, "N_AaAc_prev");
+  f.S = F(c, "S_AaAc");
+  f.phi = F(c, "phi_fric_AaAc");
+  f.tau = F(c, "tau_c_AaAc");
+  f.tau_ovr = F(c, "tau_c_override_AaAc");
+  f.W = c->d_pair[0];
+  f.EV = c->d_pair[1];
+  f.RHS = c->d_pair[2];
+  return f;
+}
+
+static ufe_params default_params() {
+  ufe_params p{};
+  const double z[15] = {0.00, 0.10, 0.20, 0.30, 0.40, 0.50, 0.60, 0.70,
+                        0.80, 0.90, 0.925, 0.95, 0.975, 0.99, 1.00};
+  p.nZ = 15;
+  for (int k = 0; k < 15; k++) p.zeta[k] = z[k];
+  p.m_enh_sia = 5.0;
+  p.m_enh_ssa = 0.7;
+  p.SSA_RN_tol = 1e-5;
+  p.SSA_max_outer_loops = 50;
+  p.SSA_max_residual_UV = 2.5;
+  p.SSA_SOR_omega = 1.2;
+  p.SSA_max_inner_loops = 10000;
+  p.zero_Hi_at_edges = 1;
+  p.precision = UFE_FP64;
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int ufe_version(void) { return 1; }
+
+const char *ufe_last_error(ufe_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ufe_ctx_create(int device, ufe_ctx **out) {
+  if (!out) return UFE_BAD_INPUT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return UFE_HIP_ERROR;
+  ufe_ctx *ctx = new ufe_ctx();
+  ctx->device = device;
+  ctx->params = default_params();
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void **)&ctx->h_flags, sizeof(SorFlags), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&ctx->h_scalar, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    delete ctx;
+    return UFE_HIP_ERROR;
+  }
+  *out = ctx;
+  return UFE_OK;
+}
+
+int ufe_ctx_destroy(ufe_ctx *ctx) {
+  if (!ctx) return UFE_OK;
+  (void)hipSetDevice(ctx->device);
+  free_mesh(ctx);
+  if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
+  if (ctx->h_scalar) (void)hipHostFree(ctx->h_scalar);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return UFE_OK;
+}
+
+int ufe_set_params(ufe_ctx *ctx, const ufe_params *p) {
+  if (!ctx || !p) return UFE_BAD_INPUT;
+  if (p->nZ < 2 || p->nZ > MAX_NZ) {
+    ctx->err = "nZ out of range";
+    return UFE_BAD_INPUT;
+  }
+  if (ctx->have_mesh && p->nZ != ctx->nZ) {
+    ctx->err = "nZ cannot change after mesh upload";
+    return UFE_BAD_INPUT;
+  }
+  ctx->params = *p;
+  ctx->nZ = p->nZ;
+  return UFE_OK;
+}
+
+int ufe_field_count(void) { return NFIELDS; }
+const char *ufe_field_name(int id) { return (id >= 0 && id < NFIELDS) ? FIELDS[id].name : nullptr; }
+int ufe_field_id(const char *name) { return name ? field_id(name) : -1; }
+int ufe_field_info(int id, int32_t *grid, int32_t *dtype) {
+  if (id < 0 || id >= NFIELDS) return UFE_BAD_INPUT;
+  if (grid) *grid = FIELDS[id].grid;
+  if (dtype) *dtype = FIELDS[id].dtype;
+  return UFE_OK;
+}
+
+int ufe_synchronize(ufe_ctx *ctx) {
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return UFE_OK;
+}
+int ufe_device_alloc(ufe_ctx *ctx, size_t bytes, void **dev) {
+  HIPCHK(ctx, hipMalloc(dev, bytes));
+  return UFE_OK;
+}
+int ufe_device_free(ufe_ctx *ctx, void *dev) {
+  HIPCHK(ctx, hipFree(dev));
+  return UFE_OK;
+}
+int ufe_memcpy_h2d(ufe_ctx *ctx, void *dev, const void *host, size_t bytes) {
+  HIPCHK(ctx, hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return UFE_OK;
+}
+int ufe_memcpy_d2h(ufe_ctx *ctx, void *host, const void *dev, size_t bytes) {
+  HIPCHK(ctx, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return UFE_OK;
+}
+
+int ufe_mesh_sizes(ufe_ctx *ctx, int32_t *out4) {
+  if (!ctx || !ctx->have_mesh || !out4) return UFE_BAD_INPUT;
+  out4[0] = ctx->nV;
+  out4[1] = ctx->nAc;
+  out4[2] = ctx->nA;
+  out4[3] = ctx->nCm;
+  return UFE_OK;
+}
+
+// ---- mesh build ------------------------------------------------------------
+static int check_primary(ufe_ctx *ctx, const ufe_mesh_desc *m) {
+  if (!m || m->nV < 5 || m->nTri < 4 || m->nC_mem < 3 || m->nC_mem > NF_MAX || !m->V || !m->nC ||
+      !m->C || !m->niTri || !m->iTri || !m->edge_index || !m->Tri) {
+    ctx->err = "mesh descriptor incomplete or nC_mem outside [3,16]";
+    return UFE_BAD_INPUT;
+  }
+  int64_t s = 0;
+  for (int v = 0; v < m->nV; v++) {
+    if (m->nC[v] < 2 || m->nC[v] > m->nC_mem) {
+      ctx->err = "nC out of range";
+      return UFE_BAD_INPUT;
+    }
+    s += m->nC[v];
+  }
+  if (s % 2 || s / 2 != m->nAc) {
+    ctx->err = "nAc must equal (sum of nC)/2";
+    return UFE_BAD_INPUT;
+  }
+  for (int v = 0; v < m->nV; v++)
+    for (int c = 0; c < m->nC[v]; c++) {
+      const int j = m->C[(int64_t)c * m->nV + v];
+      if (j < 1 || j > m->nV) {
+        ctx->err = "C holds an index outside 1..nV";
+        return UFE_BAD_INPUT;
+      }
+    }
+  return UFE_OK;
+}
+
+static int gpu_neighbour_functions(ufe_ctx *ctx, int np, int nCm, const double *V, const int32_t *nc,
+                                   const int32_t *cl, int split, const int32_t *edge_aa, int nV,
+                                   const int32_t *edge_ac, int nAc, double *Nx, double *Ny,
+                                   double *Nxx, double *Nxy, double *Nyy) {
+  double *dV, *dN[5];
+  int32_t *dnc, *dcl, *dea, *dec;
+  const size_t w = (size_t)np * (nCm + 1);
+  HIPCHK(ctx, hipMalloc((void **)&dV, sizeof(double) * 2 * np));
+  HIPCHK(ctx, hipMalloc((void **)&dnc, sizeof(int32_t) * np));
+  HIPCHK(ctx, hipMalloc((void **)&dcl, sizeof(int32_t) * (size_t)np * nCm));
+  HIPCHK(ctx, hipMalloc((void **)&dea, sizeof(int32_t) * std::max(nV, 1)));
+  HIPCHK(ctx, hipMalloc((void **)&dec, sizeof(int32_t) * std::max(nAc, 1)));
+  for (int k = 0; k < 5; k++) HIPCHK(ctx, hipMalloc((void **)&dN[k], sizeof(double) * w));
+  HIPCHK(ctx, hipMemcpyAsync(dV, V, sizeof(double) * 2 * np, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(dnc, nc, sizeof(int32_t) * np, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(dcl, cl, sizeof(int32_t) * (size_t)np * nCm, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(dea, edge_aa, sizeof(int32_t) * nV, hipMemcpyHostToDevice, ctx->stream));
+  if (nAc) HIPCHK(ctx, hipMemcpyAsync(dec, edge_ac, sizeof(int32_t) * nAc, hipMemcpyHostToDevice, ctx->stream));
+  k_neighbour_functions<<<nblk(np, 128), 128, 0, ctx->stream>>>(np, nCm, dV, dnc, dcl, split, dea, dec,
+                                                              dN[0], dN[1], dN[2], dN[3], dN[4]);
+  LAUNCHCHK(ctx);
+  double *out[5] = {Nx, Ny, Nxx, Nxy, Nyy};
+  for (int k = 0; k < 5; k++)
+    HIPCHK(ctx, hipMemcpyAsync(out[k], dN[k], sizeof(double) * w, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  (void)hipFree(dV);
+  (void)hipFree(dnc);
+  (void)hipFree(dcl);
+  (void)hipFree(dea);
+  (void)hipFree(dec);
+  for (int k = 0; k < 5; k++) (void)hipFree(dN[k]);
+  return UFE_OK;
+}
+
+int ufe_mesh_build(ufe_ctx *ctx, ufe_mesh_desc *m) {
+  if (!ctx) return UFE_BAD_INPUT;
+  int r = check_primary(ctx, m);
+  if (r) return r;
+  if (!m->iAci || !m->Aci || !m->VAc || !m->Nx_Ac || !m->Ny_Ac || !m->No_Ac || !m->Np_Ac ||
+      !m->edge_index_Ac || !m->VAaAc || !m->nCAaAc || !m->CAaAc || !m->Nx_AaAc || !m->Ny_AaAc ||
+      !m->Nxx_AaAc || !m->Nxy_AaAc || !m->Nyy_AaAc || !m->Nx || !m->Ny || !m->Nxx || !m->Nxy ||
+      !m->Nyy || !m->colour || !m->colour_vi) {
+    ctx->err = "mesh descriptor lacks secondary output arrays";
+    return UFE_BAD_INPUT;
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if ((r = build_Ac_mesh(m, ctx->err))) return r;
+  if ((r = build_AaAc_connectivity(m, ctx->err))) return r;
+  const int nV = m->nV, nA = m->nV + m->nAc;
+  // neighbour functions of the AaAc points (src/mesh_ArakawaC_module.f90:511-537)
+  if ((r = gpu_neighbour_functions(ctx, nA, m->nC_mem, m->VAaAc, m->nCAaAc, m->CAaAc, nV,
+                                   m->edge_index, nV, m->edge_index_Ac, m->nAc, m->Nx_AaAc,
+                                   m->Ny_AaAc, m->Nxx_AaAc, m->Nxy_AaAc, m->Nyy_AaAc)))
+    return r;
+  // and of the Aa vertices (src/mesh_derivatives_module.f90:51-70)
+  if ((r = gpu_neighbour_functions(ctx, nV, m->nC_mem, m->V, m->nC, m->C, nV, m->edge_index, nV,
+                                   m->edge_index_Ac, 0, m->Nx, m->Ny, m->Nxx, m->Nxy, m->Nyy)))
+    return r;
+  return five_colouring(m, ctx->err);
+}
+
+// ---- mesh upload -----------------------------------------------------------
+int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
+  if (!ctx) return UFE_BAD_INPUT;
+  int r = check_primary(ctx, m);
+  if (r) return r;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  free_mesh(ctx);
+  const int nV = m->nV, nAc = m->nAc, nA = nV + nAc, nCm = m->nC_mem;
+  ctx->nV = nV;
+  ctx->nTri = m->nTri;
+  ctx->nCm = nCm;
+  ctx->nAc = nAc;
+  ctx->nA = nA;
+  ctx->nZ = ctx->params.nZ;
+  const int64_t ldV = nV, ldA = nAc, ldAA = nA;
+  // colour lists must partition the AaAc points
+  {
+    int64_t tot = 0;
+    for (int c = 0; c < 5; c++) tot += m->colour_nV[c];
+    if (tot != nA) {
+      ctx->err = "colour_nV does not cover the AaAc points";
+      return UFE_BAD_INPUT;
+    }
+  }
+  // --- Aa / Ac arrays, reference layout -----------------------------------
+  std::vector<int32_t> Crev((size_t)nV * nCm, 0), slot_i(nAc), slot_j(nAc);
+  for (int v = 1; v <= nV; v++)
+    for (int c = 1; c <= m->nC[v - 1]; c++) {
+      const int vj = at(m->C, ldV, v, c);
+      int found = -1;
+      for (int k = 1; k <= m->nC[vj - 1]; k++)
+        if (at(m->C, ldV, vj, k) == v) { found = k - 1; break; }
+      if (found < 0) {
+        ctx->err = "connectivity is not symmetric";
+        return UFE_BAD_INPUT;
+      }
+      Crev[(size_t)(c - 1) * nV + (v - 1)] = found;
+    }
+  for (int a = 1; a <= nAc; a++) {
+    const int vi = at(m->Aci, ldA, a, 1), vj = at(m->Aci, ldA, a, 2);
+    int ci = -1, cj = -1;
+    for (int k = 1; k <= m->nC[vi - 1]; k++) if (at(m->C, ldV, vi, k) == vj) { ci = k - 1; break; }
+    for (int k = 1; k <= m->nC[vj - 1]; k++) if (at(m->C, ldV, vj, k) == vi) { cj = k - 1; break; }
+    if (ci < 0 || cj < 0) {
+      ctx->err = "Aci does not match the connectivity";
+      return UFE_BAD_INPUT;
+    }
+    slot_i[a - 1] = ci;
+    slot_j[a - 1] = cj;
+  }
+  const size_t w1 = (size_t)nV * (nCm + 1);
+  if ((r = dupload(ctx, &ctx->d_nC, m->nC, nV))) return r;
+  if ((r = dupload(ctx, &ctx->d_C, m->C, (size_t)nV * nCm))) return r;
+  if ((r = dupload(ctx, &ctx->d_edge, m->edge_index, nV))) return r;
+  if ((r = dupload(ctx, &ctx->d_iAci, (const int32_t *)m->iAci, (size_t)nV * nCm))) return r;
+  if ((r = dupload(ctx, &ctx->d_Crev, Crev.data(), Crev.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_Nx, (const double *)m->Nx, w1))) return r;
+  if ((r = dupload(ctx, &ctx->d_Ny, (const double *)m->Ny, w1))) return r;
+  if ((r = dupload(ctx, &ctx->d_Nxx, (const double *)m->Nxx, w1))) return r;
+  if ((r = dupload(ctx, &ctx->d_Nxy, (const double *)m->Nxy, w1))) return r;
+  if ((r = dupload(ctx, &ctx->d_Nyy, (const double *)m->Nyy, w1))) return r;
+  if (m->A) {
+    if ((r = dupload(ctx, &ctx->d_A, m->A, nV))) return r;
+  } else if ((r = dalloc(ctx, &ctx->d_A, nV))) return r;
+  if (m->Cw) {
+    if ((r = dupload(ctx, &ctx->d_Cw, m->Cw, (size_t)nV * nCm))) return r;
+  } else if ((r = dalloc(ctx, &ctx->d_Cw, (size_t)nV * nCm))) return r;
+  if ((r = dupload(ctx, &ctx->d_V, m->V, (size_t)nV * 2))) return r;
+  if ((r = dupload(ctx, &ctx->d_Aci, (const int32_t *)m->Aci, (size_t)nAc * 4))) return r;
+  if ((r = dupload(ctx, &ctx->d_edgeAc, (const int32_t *)m->edge_index_Ac, nAc))) return r;
+  if ((r = dupload(ctx, &ctx->d_slot_i, slot_i.data(), nAc))) return r;
+  if ((r = dupload(ctx, &ctx->d_slot_j, slot_j.data(), nAc))) return r;
+  if ((r = dupload(ctx, &ctx->d_NxAc, (const double *)m->Nx_Ac, (size_t)nAc * 4))) return r;
+  if ((r = dupload(ctx, &ctx->d_NyAc, (const double *)m->Ny_Ac, (size_t)nAc * 4))) return r;
+  if ((r = dupload(ctx, &ctx->d_NoAc, (const double *)m->No_Ac, (size_t)nAc * 4))) return r;
+  if ((r = dupload(ctx, &ctx->d_NpAc, (const double *)m->Np_Ac, nAc))) return r;
+
+  // --- colour order -------------------------------------------------------
+  auto is_edge = [&](int ai1) {
+    return ai1 <= nV ? m->edge_index[ai1 - 1] > 0 : m->edge_index_Ac[ai1 - nV - 1] > 0;
+  };
+  std::vector<int32_t> perm;
+  perm.reserve(nA);
+  for (int c = 1; c <= 5; c++) {
+    ctx->colour_start[c - 1] = (int)perm.size();
+    int act = 0;
+    for (int k = 1; k <= m->colour_nV[c - 1]; k++) {
+      const int ai = at(m->colour_vi, ldAA, k, c);
+      if (ai < 1 || ai > nA) {
+        ctx->err = "colour_vi holds an index outside 1..nVAaAc";
+        return UFE_BAD_INPUT;
+      }
+      if (!is_edge(ai)) { perm.push_back(ai - 1); act++; }
+    }
+    for (int k = 1; k <= m->colour_nV[c - 1]; k++) {
+      const int ai = at(m->colour_vi, ldAA, k, c);
+      if (is_edge(ai)) perm.push_back(ai - 1);
+    }
+    ctx->colour_active[c - 1] = act;
+  }
+  ctx->colour_start[5] = nA;
+  if ((int)perm.size() != nA) {
+    ctx->err = "colour lists are not a partition";
+    return UFE_BAD_INPUT;
+  }
+  std::vector<int32_t> iperm(nA, -1);
+  for (int p = 0; p < nA; p++) {
+    if (iperm[perm[p]] != -1) {
+      ctx->err = "colour lists repeat a point";
+      return UFE_BAD_INPUT;
+    }
+    iperm[perm[p]] = p;
+  }
+  // SELL-64 arrays over the colour order
+  const int nchunk = (nA + SELL - 1) / SELL;
+  std::vector<int64_t> cptr(nchunk + 1, 0);
+  std::vector<int32_t> cw(nchunk, 0);
+  std::vector<uint8_t> deg(nA);
+  for (int p = 0; p < nA; p++) {
+    const int n = m->nCAaAc[perm[p]];
+    if (n < 2 || n > nCm) {
+      ctx->err = "nCAaAc out of range";
+      return UFE_BAD_INPUT;
+    }
+    deg[p] = (uint8_t)n;
+    cw[p / SELL] = std::max(cw[p / SELL], n);
+  }
+  for (int k = 0; k < nchunk; k++) cptr[k + 1] = cptr[k] + (int64_t)cw[k] * SELL;
+  const int64_t ne = cptr[nchunk];
+  std::vector<int32_t> col(ne);
+  std::vector<double> cNx(ne, 0.0), cNy(ne, 0.0);
+  std::vector<double2> cU(ne, make_double2(0.0, 0.0));
+  std::vector<double> hNx(nA), hNy(nA), hNxx(nA), hNyy(nA), Sxx(nA), Sxy(nA), Syy(nA);
+  for (int p = 0; p < nA; p++) {
+    const int ai = perm[p] + 1, n = deg[p];
+    const int64_t base = cptr[p / SELL] + (p % SELL);
+    for (int s = 0; s < cw[p / SELL]; s++) col[base + (int64_t)s * SELL] = p;
+    for (int s = 1; s <= n; s++) {
+      const int64_t e = base + (int64_t)(s - 1) * SELL;
+      col[e] = iperm[at(m->CAaAc, ldAA, ai, s) - 1];
+      cNx[e] = at(m->Nx_AaAc, ldAA, ai, s);
+      cNy[e] = at(m->Ny_AaAc, ldAA, ai, s);
+      const double nxx = at(m->Nxx_AaAc, ldAA, ai, s), nyy = at(m->Nyy_AaAc, ldAA, ai, s);
+      cU[e] = make_double2(4.0 * nxx + nyy, 4.0 * nyy + nxx);
+    }
+    hNx[p] = at(m->Nx_AaAc, ldAA, ai, n + 1);
+    hNy[p] = at(m->Ny_AaAc, ldAA, ai, n + 1);
+    hNxx[p] = at(m->Nxx_AaAc, ldAA, ai, n + 1);
+    hNyy[p] = at(m->Nyy_AaAc, ldAA, ai, n + 1);
+    double a = at(m->Nxx_AaAc, ldAA, ai, n + 1), b = at(m->Nxy_AaAc, ldAA, ai, n + 1),
+           c = at(m->Nyy_AaAc, ldAA, ai, n + 1);
+    for (int s = 1; s <= n; s++) {
+      a = a + at(m->Nxx_AaAc, ldAA, ai, s);
+      b = b + at(m->Nxy_AaAc, ldAA, ai, s);
+      c = c + at(m->Nyy_AaAc, ldAA, ai, s);
+    }
+    Sxx[p] = a;
+    Sxy[p] = b;
+    Syy[p] = c;
+  }
+  // Neumann lists: edge points with index >= 5 (1-based) average their
+  // non-edge neighbours; corners 1..4 average all neighbours, afterwards.
+  std::vector<int32_t> nrow, nptr{0}, ncol, crow, cptr2{0}, ccol;
+  for (int ai = 5; ai <= nA; ai++) {
+    if (!is_edge(ai)) continue;
+    nrow.push_back(iperm[ai - 1]);
+    int cnt = 0;
+    for (int s = 1; s <= m->nCAaAc[ai - 1]; s++) {
+      const int aj = at(m->CAaAc, ldAA, ai, s);
+      if (is_edge(aj)) continue;
+      ncol.push_back(iperm[aj - 1]);
+      cnt++;
+    }
+    if (cnt == 0) {
+      ctx->err = "an edge point has no non-edge neighbour (Neumann mean undefined)";
+      return UFE_BAD_INPUT;
+    }
+    nptr.push_back((int32_t)ncol.size());
+  }
+  for (int ai = 1; ai <= 4; ai++) {
+    crow.push_back(iperm[ai - 1]);
+    for (int s = 1; s <= m->nCAaAc[ai - 1]; s++) ccol.push_back(iperm[at(m->CAaAc, ldAA, ai, s) - 1]);
+    cptr2.push_back((int32_t)ccol.size());
+  }
+  ctx->n_neu = (int)nrow.size();
+  ctx->n_cor = (int)crow.size();
+  ctx->h_perm = perm;
+  if ((r = dupload(ctx, &ctx->d_perm, perm.data(), nA))) return r;
+  if ((r = dupload(ctx, &ctx->d_chunk_ptr, cptr.data(), cptr.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_chunk_w, cw.data(), cw.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_deg, deg.data(), deg.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_col, col.data(), col.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_cNx, cNx.data(), cNx.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_cNy, cNy.data(), cNy.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_cU, cU.data(), cU.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_hNx, hNx.data(), nA))) return r;
+  if ((r = dupload(ctx, &ctx->d_hNy, hNy.data(), nA))) return r;
+  if ((r = dupload(ctx, &ctx->d_hNxx, hNxx.data(), nA))) return r;
+  if ((r = dupload(ctx, &ctx->d_hNyy, hNyy.data(), nA))) return r;
+  if ((r = dupload(ctx, &ctx->d_Sxx, Sxx.data(), nA))) return r;
+  if ((r = dupload(ctx, &ctx->d_Sxy, Sxy.data(), nA))) return r;
+  if ((r = dupload(ctx, &ctx->d_Syy, Syy.data(), nA))) return r;
+  if ((r = dupload(ctx, &ctx->d_neu_row, nrow.data(), nrow.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_neu_ptr, nptr.data(), nptr.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_neu_col, ncol.data(), ncol.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_cor_row, crow.data(), crow.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_cor_ptr, cptr2.data(), cptr2.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_cor_col, ccol.data(), ccol.size()))) return r;
+
+  // --- fields (zero-initialised, like the reference's shared windows) ------
+  ctx->field_ptr.assign(NFIELDS, nullptr);
+  for (int i = 0; i < NFIELDS; i++) {
+    if (FIELDS[i].pair >= 0) continue;
+    const size_t n = grid_len(ctx, FIELDS[i].grid);
+    if (FIELDS[i].dtype == D_F64) {
+      double *p;
+      if ((r = dalloc(ctx, &p, n))) return r;
+      HIPCHK(ctx, hipMemsetAsync(p, 0, n * sizeof(double), ctx->stream));
+      ctx->field_ptr[i] = p;
+    } else {
+      int32_t *p;
+      if ((r = dalloc(ctx, &p, n))) return r;
+      HIPCHK(ctx, hipMemsetAsync(p, 0, n * sizeof(int32_t), ctx->stream));
+      ctx->field_ptr[i] = p;
+    }
+  }
+  for (int k = 0; k < 3; k++) {
+    if ((r = dalloc(ctx, &ctx->d_pair[k], nA))) return r;
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_pair[k], 0, sizeof(double2) * nA, ctx->stream));
+  }
+  ctx->n_part = nblk(std::max(nA, nV)) + 16;
+  if ((r = dalloc(ctx, &ctx->d_part, 2 * (size_t)ctx->n_part))) return r;
+  if ((r = dalloc(ctx, &ctx->d_scalar, 8))) return r;
+  for (int k = 0; k < 4; k++)
+    if ((r = dalloc(ctx, &ctx->d_tmp[k], (size_t)std::max(nA, nV * std::max(ctx->nZ, nCm))))) return r;
+  if ((r = dalloc(ctx, &ctx->d_ViSMB, nV))) return r;
+  if ((r = dalloc(ctx, &ctx->d_count, 4))) return r;
+  if ((r = dalloc(ctx, &ctx->d_flags, 1))) return r;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->have_mesh = true;
+  return UFE_OK;
+}
+
+// ---- fields ----------------------------------------------------------------
+static int field_xfer(ufe_ctx *ctx, int id, void *buf, bool to_dev, bool buf_is_dev) {
+  if (!ctx || !ctx->have_mesh) return UFE_BAD_INPUT;
+  if (id < 0 || id >= NFIELDS || !buf) {
+    ctx->err = "bad field id or buffer";
+    return UFE_BAD_INPUT;
+  }
+  const FieldDef &fd = FIELDS[id];
+  const size_t n = grid_len(ctx, fd.grid);
+  const size_t bytes = n * (fd.dtype == D_F64 ? 8 : 4);
+  const hipMemcpyKind k_in = buf_is_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  const hipMemcpyKind k_out = buf_is_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (fd.grid != G_AAAC) {
+    void *d = ctx->field_ptr[id];
+    if (to_dev) HIPCHK(ctx, hipMemcpyAsync(d, buf, bytes, k_in, ctx->stream));
+    else HIPCHK(ctx, hipMemcpyAsync(buf, d, bytes, k_out, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return UFE_OK;
+  }
+  double *tmp = ctx->d_tmp[3];
+  if (to_dev) {
+    HIPCHK(ctx, hipMemcpyAsync(tmp, buf, bytes, k_in, ctx->stream));
+    if (fd.pair >= 0)
+      k_gather_perm_pair<<<nblk(n), BLOCK, 0, ctx->stream>>>((int)n, ctx->d_perm, tmp, ctx->d_pair[fd.pair], fd.comp);
+    else
+      k_gather_perm<<<nblk(n), BLOCK, 0, ctx->stream>>>((int)n, ctx->d_perm, tmp, (double *)ctx->field_ptr[id]);
+    LAUNCHCHK(ctx);
+  } else {
+    if (fd.pair >= 0)
+      k_scatter_perm_pair<<<nblk(n), BLOCK, 0, ctx->stream>>>((int)n, ctx->d_perm, ctx->d_pair[fd.pair], tmp, fd.comp);
+    else
+      k_scatter_perm<<<nblk(n), BLOCK, 0, ctx->stream>>>((int)n, ctx->d_perm, (const double *)ctx->field_ptr[id], tmp);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, hipMemcpyAsync(buf, tmp, bytes, k_out, ctx->stream));
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return UFE_OK;
+}
+
+int ufe_field_set(ufe_ctx *ctx, int id, const void *host) { return field_xfer(ctx, id, (void *)host, true, false); }
+int ufe_field_get(ufe_ctx *ctx, int id, void *host) { return field_xfer(ctx, id, host, false, false); }
+int ufe_field_set_device(ufe_ctx *ctx, int id, const void *dev) { return field_xfer(ctx, id, (void *)dev, true, true); }
+int ufe_field_get_device(ufe_ctx *ctx, int id, void *dev) { return field_xfer(ctx, id, dev, false, true); }
+
+// ---- update_general_ice_model_data ------------------------------------------
+static GenFields gen_fields(ufe_ctx *c) {
+  GenFields f;
+  f.Hi = F(c, "Hi"); f.Hb = F(c, "Hb"); f.SL = F(c, "SL"); f.dHb_dt = F(c, "dHb_dt"); f.dHi_dt = F(c, "dHi_dt");
+  f.Hs = F(c, "Hs"); f.dHs_dt = F(c, "dHs_dt"); f.dHi_dx = F(c, "dHi_dx"); f.dHi_dy = F(c, "dHi_dy");
+  f.dHs_dx = F(c, "dHs_dx"); f.dHs_dy = F(c, "dHs_dy"); f.dHs_dx_shelf = F(c, "dHs_dx_shelf");
+  f.dHs_dy_shelf = F(c, "dHs_dy_shelf");
+  f.land = FI(c, "mask_land"); f.ocean = FI(c, "mask_ocean"); f.lake = FI(c, "mask_lake");
+  f.ice = FI(c, "mask_ice"); f.sheet = FI(c, "mask_sheet"); f.shelf = FI(c, "mask_shelf");
+  f.coast = FI(c, "mask_coast"); f.margin = FI(c, "mask_margin"); f.gl = FI(c, "mask_gl");
+  f.cf = FI(c, "mask_cf"); f.mask = FI(c, "mask");
+  f.Hi_Ac = F(c, "Hi_Ac"); f.Hb_Ac = F(c, "Hb_Ac"); f.SL_Ac = F(c, "SL_Ac"); f.Hs_Ac = F(c, "Hs_Ac");
+  f.dHi_dx_Ac = F(c, "dHi_dx_Ac"); f.dHi_dy_Ac = F(c, "dHi_dy_Ac"); f.dHi_dp_Ac = F(c, "dHi_dp_Ac");
+  f.dHi_do_Ac = F(c, "dHi_do_Ac"); f.dHb_dx_Ac = F(c, "dHb_dx_Ac"); f.dHb_dy_Ac = F(c, "dHb_dy_Ac");
+  f.dHb_dp_Ac = F(c, "dHb_dp_Ac"); f.dHb_do_Ac = F(c, "dHb_do_Ac"); f.dHs_dx_Ac = F(c, "dHs_dx_Ac");
+  f.dHs_dy_Ac = F(c, "dHs_dy_Ac"); f.dHs_dp_Ac = F(c, "dHs_dp_Ac"); f.dHs_do_Ac = F(c, "dHs_do_Ac");
+  f.dSL_dx_Ac = F(c, "dSL_dx_Ac"); f.dSL_dy_Ac = F(c, "dSL_dy_Ac"); f.dSL_dp_Ac = F(c, "dSL_dp_Ac");
+  f.dSL_do_Ac = F(c, "dSL_do_Ac"); f.dHs_dx_shelf_Ac = F(c, "dHs_dx_shelf_Ac");
+  f.dHs_dy_shelf_Ac = F(c, "dHs_dy_shelf_Ac");
+  f.landA = FI(c, "mask_land_Ac"); f.oceanA = FI(c, "mask_ocean_Ac"); f.lakeA = FI(c, "mask_lake_Ac");
+  f.iceA = FI(c, "mask_ice_Ac"); f.sheetA = FI(c, "mask_sheet_Ac"); f.shelfA = FI(c, "mask_shelf_Ac");
+  f.coastA = FI(c, "mask_coast_Ac"); f.marginA = FI(c, "mask_margin_Ac"); f.glA = FI(c, "mask_gl_Ac");
+  f.cfA = FI(c, "mask_cf_Ac"); f.maskA = FI(c, "mask_Ac");
+  return f;
+}
+
+int ufe_update_general_ice_model_data(ufe_ctx *ctx) {
+  if (!ctx || !ctx->have_mesh) return UFE_BAD_INPUT;
+  GenFields f = gen_fields(ctx);
+  k_gen_aa1<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(ctx->nV, f);
+  k_gen_ac<<<nblk(ctx->nAc), BLOCK, 0, ctx->stream>>>(ac_mesh(ctx), f);
+  k_gen_aa2<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(aa_mesh(ctx), f);
+  LAUNCHCHK(ctx);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return UFE_OK;
+}
+
+// ---- solve_SIA ---------------------------------------------------------------
+int ufe_solve_SIA(ufe_ctx *ctx) {
+  if (!ctx || !ctx->have_mesh) return UFE_BAD_INPUT;
+  SIAParams p;
+  p.nZ = ctx->nZ;
+  p.m_enh = ctx->params.m_enh_sia;
+  for (int k = 0; k < ctx->nZ; k++) {
+    p.zeta[k] = ctx->params.zeta[k];
+    p.zeta_n[k] = std::pow(ctx->params.zeta[k], N_FLOW);
+  }
+  k_sia<<<nblk(ctx->nAc), BLOCK, 0, ctx->stream>>>(
+      ctx->nAc, p, F(ctx, "Hi_Ac"), F(ctx, "dHs_dx_Ac"), F(ctx, "dHs_dy_Ac"), F(ctx, "dHs_dp_Ac"),
+      F(ctx, "dHs_do_Ac"), FI(ctx, "mask_sheet_Ac"), F(ctx, "A_flow_Ac"), F(ctx, "D_SIA_3D_Ac"),
+      F(ctx, "D_SIA_Ac"), F(ctx, "Ux_SIA_Ac"), F(ctx, "Uy_SIA_Ac"), F(ctx, "Up_SIA_Ac"), F(ctx, "Uo_SIA_Ac"));
+  AaMesh am = aa_mesh(ctx);
+  k_map_Ac_to_Aa<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(am, ctx->nAc, 1, F(ctx, "Ux_SIA_Ac"), F(ctx, "U_SIA"));
+  k_map_Ac_to_Aa<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(am, ctx->nAc, 1, F(ctx, "Uy_SIA_Ac"), F(ctx, "V_SIA"));
+  k_map_Ac_to_Aa<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(am, ctx->nAc, 1, F(ctx, "D_SIA_Ac"), F(ctx, "D_SIA"));
+  LAUNCHCHK(ctx);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return UFE_OK;
+}
+
+// ---- SOR ---------------------------------------------------------------------
+static NeumannList neu_list(ufe_ctx *c) {
+  NeumannList L;
+  L.n = c->n_neu;
+  L.row = c->d_neu_row;
+  L.ptr = c->d_neu_ptr;
+  L.col = c->d_neu_col;
+  return L;
+}
+static NeumannList cor_list(ufe_ctx *c) {
+  NeumannList L;
+  L.n = c->n_cor;
+  L.row = c->d_cor_row;
+  L.ptr = c->d_cor_ptr;
+  L.col = c->d_cor_col;
+  return L;
+}
+
+static void enqueue_sweep_fp64(ufe_ctx *ctx, const AaAcMesh &m, double omega, double tol, int max_sweeps) {
+  for (int c = 0; c < 5; c++) {
+    const int n = ctx->colour_active[c];
+    if (n == 0) continue;
+    k_sor_colour<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(m, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
+                                                      ctx->colour_start[c], n, omega, ctx->d_flags);
+  }
+  if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], ctx->d_flags);
+  k_sor_finalize<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0], ctx->d_flags, tol, max_sweeps);
+}
+
+// solve_SSA_linearised sweep loop (:601-691): returns the flags at exit.
+static int run_sor_reference(ufe_ctx *ctx, double tol, int max_sweeps, SorFlags *out, double *seconds) {
+  const AaAcMesh m = aaac_mesh(ctx);
+  const double omega = ctx->params.SSA_SOR_omega;
+  k_sor_flags_init<<<1, 1, 0, ctx->stream>>>(ctx->d_flags);
+  LAUNCHCHK(ctx);
+  int batch = ctx->params.sweep_batch > 0 ? ctx->params.sweep_batch : 1;
+  int queued = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  while (true) {
+    const int nb = std::min(batch, max_sweeps - queued);
+    for (int b = 0; b < nb; b++) enqueue_sweep_fp64(ctx, m, omega, tol, max_sweeps);
+    queued += nb;
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_flags, ctx->d_flags, sizeof(SorFlags), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->h_flags->done || queued >= max_sweeps) break;
+    if (ctx->params.sweep_batch <= 0) batch = std::min(batch * 2, 64);
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds += std::chrono::duration<double>(t1 - t0).count();
+  *out = *ctx->h_flags;
+  return UFE_OK;
+}
+
+// ---- solve_SSA ---------------------------------------------------------------
+static int ssa_viscosity_rn(ufe_ctx *ctx, double *RN) {
+  const AaAcMesh m = aaac_mesh(ctx);
+  SSAFields f = ssa_fields(ctx);
+  const int nb = nblk(ctx->nA);
+  k_ssa_viscosity<<<nb, BLOCK, 0, ctx->stream>>>(m, f, ctx->params.m_enh_ssa * 0.5, ctx->d_part);
+  k_ssa_rn<<<1, 1024, 0, ctx->stream>>>(nb, ctx->d_part, ctx->d_scalar);
+  LAUNCHCHK(ctx);
+  if (RN) {
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *RN = ctx->h_scalar[0];
+  }
+  return UFE_OK;
+}
+
+static int ssa_prepare_coeffs(ufe_ctx *ctx) {
+  k_ssa_prepare<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(aaac_mesh(ctx), ssa_fields(ctx), std::pow(100.0, 0.30));
+  LAUNCHCHK(ctx);
+  return UFE_OK;
+}
+
+static int ssa_gather(ufe_ctx *ctx) {
+  k_ssa_gather<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(
+      aaac_mesh(ctx), ssa_fields(ctx), F(ctx, "Hi"), F(ctx, "Hb"), F(ctx, "SL"), F(ctx, "dHs_dx_shelf"),
+      F(ctx, "dHs_dy_shelf"), F(ctx, "A_flow_mean"), F(ctx, "U_SSA"), F(ctx, "V_SSA"), F(ctx, "Hi_Ac"),
+      F(ctx, "Hb_Ac"), F(ctx, "SL_Ac"), F(ctx, "dHs_dx_shelf_Ac"), F(ctx, "dHs_dy_shelf_Ac"),
+      F(ctx, "A_flow_mean_Ac"), F(ctx, "Ux_SSA_Ac"), F(ctx, "Uy_SSA_Ac"), ctx->params.use_tau_c_override);
+  LAUNCHCHK(ctx);
+  return UFE_OK;
+}
+
+static int ssa_finish(ufe_ctx *ctx) {
+  k_ssa_finish<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(
+      aaac_mesh(ctx), ac_mesh(ctx), ctx->d_pair[0], F(ctx, "U_SSA"), F(ctx, "V_SSA"), F(ctx, "Ux_SSA_Ac"),
+      F(ctx, "Uy_SSA_Ac"), F(ctx, "Up_SSA_Ac"), F(ctx, "Uo_SSA_Ac"));
+  LAUNCHCHK(ctx);
+  return UFE_OK;
+}
+
+static int zero_ssa_outputs(ufe_ctx *ctx) {
+  const char *aa[] = {"U_SSA", "V_SSA"};
+  const char *ac[] = {"Ux_SSA_Ac", "Uy_SSA_Ac", "Up_SSA_Ac", "Uo_SSA_Ac"};
+  for (auto n : aa) k_fill<double><<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(F(ctx, n), ctx->nV, 0.0);
+  for (auto n : ac) k_fill<double><<<nblk(ctx->nAc), BLOCK, 0, ctx->stream>>>(F(ctx, n), ctx->nAc, 0.0);
+  k_fill<double2><<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->d_pair[0], ctx->nA, make_double2(0.0, 0.0));
+  LAUNCHCHK(ctx);
+  return UFE_OK;
+}
+
+int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) {
+  if (!ctx || !ctx->have_mesh) return UFE_BAD_INPUT;
+  ufe_ssa_stats local;
+  if (!st) st = &local;
+  std::memset(st, 0, sizeof(*st));
+  const ufe_params &P = ctx->params;
+  if (P.use_analytical_GL_flux) {
+    ctx->err = "use_analytical_GL_flux is not supported";
+    return st->status = UFE_NOT_SUPPORTED;
+  }
+  if (P.precision != UFE_FP64) {
+    ctx->err = "solve_SSA runs the reference arithmetic (FP64); mixed precision is ufe_ssa_linear_solve";
+    return st->status = UFE_NOT_SUPPORTED;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  // gating (:430-465)
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_count, 0, sizeof(unsigned long long), ctx->stream));
+  k_sum_int<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(ctx->nV, FI(ctx, "mask_sheet"), ctx->d_count);
+  LAUNCHCHK(ctx);
+  unsigned long long nsheet = 0;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->h_scalar, ctx->d_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  std::memcpy(&nsheet, ctx->h_scalar, sizeof(nsheet));
+  int r;
+  if (P.ssa_forced_zero || nsheet == 0) {
+    if ((r = zero_ssa_outputs(ctx))) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return UFE_OK;
+  }
+  if ((r = ssa_gather(ctx))) return r;
+  int it = 0, did_reset_before = 0, did_reset_now = 0;
+  while (it < P.SSA_max_outer_loops) {
+    it++;
+    double RN = 0.0;
+    if ((r = ssa_viscosity_rn(ctx, &RN))) return r;
+    st->last_RN = RN;
+    st->picard_iterations = it;
+    if (RN < P.SSA_RN_tol) {
+      st->picard_converged = 1;
+      break;
+    }
+    if ((r = ssa_prepare_coeffs(ctx))) return r;
+    SorFlags fl;
+    if ((r = run_sor_reference(ctx, P.SSA_max_residual_UV, P.SSA_max_inner_loops, &fl, &st->sor_seconds))) return r;
+    if (it <= 64) st->sweeps[it - 1] = fl.sweep;
+    st->total_sweeps += fl.sweep;
+    st->last_max_residual = fl.last_maxres;
+    if (fl.converged) did_reset_now = 0;
+    if (fl.reset) {
+      k_fill<double2><<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->d_pair[0], ctx->nA, make_double2(0.0, 0.0));
+      LAUNCHCHK(ctx);
+      did_reset_now = 1;
+    }
+    if (fl.hit_max && !fl.converged && !fl.reset) st->status = UFE_SOR_NOT_CONVERGED;
+    if (did_reset_now) {
+      st->resets++;
+      if (!did_reset_before) did_reset_before = 1;
+      else {
+        ctx->err = "solve_SSA: SSA remains unstable after resetting velocities to zero";
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return st->status = UFE_SSA_UNSTABLE;
+      }
+    }
+  }
+  if ((r = ssa_finish(ctx))) return r;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  st->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return st->status;
+}
+
+// ---- linearised solve at fixed viscosity (bench path) ------------------------
+int ufe_ssa_prepare(ufe_ctx *ctx) {
+  if (!ctx || !ctx->have_mesh) return UFE_BAD_INPUT;
+  int r;
+  if ((r = ssa_gather(ctx))) return r;
+  if ((r = ssa_viscosity_rn(ctx, nullptr))) return r;
+  if ((r = ssa_prepare_coeffs(ctx))) return r;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return UFE_OK;
+}
+
+static int residual_fp64(ufe_ctx *ctx, double *rel) {
+  const AaAcMesh m = aaac_mesh(ctx);
+  int nb_tot = 0;
+  for (int c = 0; c < 5; c++) {
+    const int n = ctx->colour_active[c];
+    if (!n) continue;
+    const int nb = nblk(n);
+    k_ssa_residual<<<nb, BLOCK, 0, ctx->stream>>>(m, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
+                                                   ctx->colour_start[c], n, ctx->d_part + 2 * nb_tot);
+    nb_tot += nb;
+  }
+  k_ssa_rn<<<1, 1024, 0, ctx->stream>>>(nb_tot, ctx->d_part, ctx->d_scalar);
+  LAUNCHCHK(ctx);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *rel = ctx->h_scalar[0];
+  return UFE_OK;
+}
+
+int ufe_ssa_residual(ufe_ctx *ctx, double *rel_residual) {
+  if (!ctx || !ctx->have_mesh || !rel_residual) return UFE_BAD_INPUT;
+  return residual_fp64(ctx, rel_residual);
+}
+
+int ufe_ssa_linear_solve(ufe_ctx *ctx, double rel_tol, int32_t max_sweeps, ufe_linsolve_stats *st) {
+  if (!ctx || !ctx->have_mesh || !st) return UFE_BAD_INPUT;
+  std::memset(st, 0, sizeof(*st));
+  const AaAcMesh m = aaac_mesh(ctx);
+  const double omega = ctx->params.SSA_SOR_omega;
+  const int check = ctx->params.sweep_batch > 0 ? ctx->params.sweep_batch : 16;
+  auto t0 = std::chrono::steady_clock::now();
+  double rel = 0.0;
+  int r;
+  if ((r = residual_fp64(ctx, &rel))) return r;
+  st->refinements = 1;
+  if (ctx->params.precision != UFE_FP64) {
+    ctx->err = "mixed-precision linear solve not built yet";
+    return UFE_NOT_SUPPORTED;
+  }
+  while (rel > rel_tol && st->sweeps < max_sweeps) {
+    k_sor_flags_init<<<1, 1, 0, ctx->stream>>>(ctx->d_flags);
+    const int nb = std::min(check, max_sweeps - st->sweeps);
+    // the reference stopping test is disabled here (tolerance 0, no cap)
+    for (int b = 0; b < nb; b++) enqueue_sweep_fp64(ctx, m, omega, 0.0, 1 << 30);
+    st->sweeps += nb;
+    if ((r = residual_fp64(ctx, &rel))) return r;
+    st->refinements++;
+  }
+  st->rel_residual = rel;
+  st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  st->status = rel <= rel_tol ? UFE_OK : UFE_SOR_NOT_CONVERGED;
+  return UFE_OK;
+}
+
+// ---- calculate_ice_thickness_change -----------------------------------------
+int ufe_calculate_ice_thickness_change(ufe_ctx *ctx, double dt) {
+  if (!ctx || !ctx->have_mesh) return UFE_BAD_INPUT;
+  double *dVi = F(ctx, "dVi_in");
+  k_fill<double><<<nblk((int64_t)ctx->nV * ctx->nCm), BLOCK, 0, ctx->stream>>>(dVi, (int64_t)ctx->nV * ctx->nCm, 0.0);
+  k_mc_flux<<<nblk(ctx->nAc), BLOCK, 0, ctx->stream>>>(ac_mesh(ctx), F(ctx, "Hi"), F(ctx, "Up_SIA_Ac"),
+                                                         F(ctx, "Up_SSA_Ac"), ctx->d_Cw, dt, dVi);
+  AaMesh am = aa_mesh(ctx);
+  k_mc_rescale<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(am, F(ctx, "Hi"), F(ctx, "SMB_year"), F(ctx, "BMB"), dt,
+                                                           dVi, ctx->d_ViSMB);
+  k_mc_update<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(am, dVi, ctx->d_ViSMB, dt, FI(ctx, "mask_noice"),
+                                                          F(ctx, "dHi_dt"), F(ctx, "Hi"), F(ctx, "Hi_prev"),
+                                                          ctx->params.zero_Hi_at_edges);
+  LAUNCHCHK(ctx);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return UFE_OK;
+}
+
+// ---- operators -------------------------------------------------------------
+#define NEED_MESH(ctx) \
+  if (!(ctx) || !(ctx)->have_mesh) return UFE_BAD_INPUT
+
+int ufe_op_derivatives_Aa(ufe_ctx *ctx, const double *d, double *ddx, double *ddy) {
+  NEED_MESH(ctx);
+  k_deriv_Aa<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(aa_mesh(ctx), d, ddx, ddy);
+  LAUNCHCHK(ctx);
+  return ufe_synchronize(ctx);
+}
+int ufe_op_curvatures_Aa(ufe_ctx *ctx, const double *d, double *a, double *b, double *c) {
+  NEED_MESH(ctx);
+  k_curv_Aa<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(aa_mesh(ctx), d, a, b, c);
+  LAUNCHCHK(ctx);
+  return ufe_synchronize(ctx);
+}
+int ufe_op_derivatives_Aa_3D(ufe_ctx *ctx, const double *d, double *ddx, double *ddy, int nz) {
+  NEED_MESH(ctx);
+  k_deriv_Aa_3D<<<nblk((int64_t)ctx->nV * nz), BLOCK, 0, ctx->stream>>>(aa_mesh(ctx), nz, d, ddx, ddy);
+  LAUNCHCHK(ctx);
+  return ufe_synchronize(ctx);
+}
+int ufe_op_neumann_Aa(ufe_ctx *ctx, double *d) { return ufe_op_neumann_Aa_3D(ctx, d, 1); }
+int ufe_op_neumann_Aa_3D(ufe_ctx *ctx, double *d, int nz) {
+  NEED_MESH(ctx);
+  k_neumann_Aa_edges<<<nblk((int64_t)ctx->nV * nz), BLOCK, 0, ctx->stream>>>(aa_mesh(ctx), nz, d);
+  k_neumann_Aa_corners<<<nblk(nz, 64), 64, 0, ctx->stream>>>(aa_mesh(ctx), nz, d);
+  LAUNCHCHK(ctx);
+  return ufe_synchronize(ctx);
+}
+int ufe_op_derivatives_Ac(ufe_ctx *ctx, const double *d, double *ddx, double *ddy, double *ddp, double *ddo) {
+  NEED_MESH(ctx);
+  k_deriv_Ac<<<nblk(ctx->nAc), BLOCK, 0, ctx->stream>>>(ac_mesh(ctx), d, ddx, ddy, ddp, ddo);
+  LAUNCHCHK(ctx);
+  return ufe_synchronize(ctx);
+}
+int ufe_op_derivatives_AaAc(ufe_ctx *ctx, const double *d, double *ddx, double *ddy) {
+  NEED_MESH(ctx);
+  const int n = ctx->nA;
+  k_gather_perm<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, d, ctx->d_tmp[0]);
+  k_deriv_AaAc<<<nblk(n), BLOCK, 0, ctx->stream>>>(aaac_mesh(ctx), ctx->d_tmp[0], ctx->d_tmp[1], ctx->d_tmp[2]);
+  k_scatter_perm<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, ctx->d_tmp[1], ddx);
+  k_scatter_perm<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, ctx->d_tmp[2], ddy);
+  LAUNCHCHK(ctx);
+  return ufe_synchronize(ctx);
+}
+int ufe_op_curvatures_AaAc(ufe_ctx *ctx, const double *d, double *a, double *b, double *c) {
+  NEED_MESH(ctx);
+  const int n = ctx->nA;
+  k_gather_perm<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, d, ctx->d_tmp[0]);
+  k_curv_AaAc<<<nblk(n), BLOCK, 0, ctx->stream>>>(aaac_mesh(ctx), ctx->d_tmp[0], ctx->d_tmp[1], ctx->d_tmp[2], ctx->d_tmp[3]);
+  k_scatter_perm<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, ctx->d_tmp[1], a);
+  k_scatter_perm<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, ctx->d_tmp[2], b);
+  k_scatter_perm<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, ctx->d_tmp[3], c);
+  LAUNCHCHK(ctx);
+  return ufe_synchronize(ctx);
+}
+int ufe_op_neumann_AaAc(ufe_ctx *ctx, double *d) {
+  NEED_MESH(ctx);
+  const int n = ctx->nA;
+  k_gather_perm<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, d, ctx->d_tmp[0]);
+  if (ctx->n_neu) k_neumann_scalar<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_tmp[0]);
+  k_neumann_scalar_serial<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_tmp[0]);
+  k_scatter_perm<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, ctx->d_tmp[0], d);
+  LAUNCHCHK(ctx);
+  return ufe_synchronize(ctx);
+}
+int ufe_op_map_Aa_to_Ac(ufe_ctx *ctx, const double *a, double *b) { return ufe_op_map_Aa_to_Ac_3D(ctx, a, b, 1); }
+int ufe_op_map_Aa_to_Ac_3D(ufe_ctx *ctx, const double *a, double *b, int nz) {
+  NEED_MESH(ctx);
+  k_map_Aa_to_Ac<<<nblk((int64_t)ctx->nAc * nz), BLOCK, 0, ctx->stream>>>(ac_mesh(ctx), nz, a, b);
+  LAUNCHCHK(ctx);
+  return ufe_synchronize(ctx);
+}
+int ufe_op_map_Ac_to_Aa(ufe_ctx *ctx, const double *a, double *b) { return ufe_op_map_Ac_to_Aa_3D(ctx, a, b, 1); }
+int ufe_op_map_Ac_to_Aa_3D(ufe_ctx *ctx, const double *a, double *b, int nz) {
+  NEED_MESH(ctx);
+  k_map_Ac_to_Aa<<<nblk((int64_t)ctx->nV * nz), BLOCK, 0, ctx->stream>>>(aa_mesh(ctx), ctx->nAc, nz, a, b);
+  LAUNCHCHK(ctx);
+  return ufe_synchronize(ctx);
+}
+int ufe_op_rotate_xy_to_po(ufe_ctx *ctx, const double *x, const double *y, double *p, double *o) {
+  NEED_MESH(ctx);
+  k_rotate<<<nblk(ctx->nAc), BLOCK, 0, ctx->stream>>>(ac_mesh(ctx), x, y, p, o);
+  LAUNCHCHK(ctx);
+  return ufe_synchronize(ctx);
+}
+
+}  // extern "C"
