@@ -105,14 +105,38 @@ typedef struct ufe_ssa_stats {
   double total_seconds;
 } ufe_ssa_stats;
 
-/* Statistics of one linearised solve to a relative residual (bench path). */
+/* Options of one linearised solve to a relative residual.  The reference
+ * stops its SOR on max|dU| < SSA_max_residual_UV (src/ice_dynamics_module.f90:
+ * 651-676); this solve instead stops on the FP64 relative residual
+ * ||b - A w||_2 / ||b||_2 <= rel_tol, checked every check_interval sweeps.
+ * UFE_FP64: FP64 sweeps.  UFE_MIXED: FP64 iterative refinement around FP32
+ * sweeps (the correction equation A d = b - A w is swept in FP32, w += d in
+ * FP64). */
+typedef struct ufe_linsolve_opts {
+  double rel_tol;
+  int32_t max_sweeps;
+  int32_t precision;                /* UFE_FP64 or UFE_MIXED */
+  double omega;                     /* over-relaxation; <= 0: params.SSA_SOR_omega */
+  int32_t check_interval;           /* sweeps between FP64 residual evaluations; <= 0: 16 */
+  int32_t use_graph;                /* replay check_interval sweeps as one HIP graph */
+} ufe_linsolve_opts;
+
+/* Statistics of one linearised solve. */
 typedef struct ufe_linsolve_stats {
   int32_t sweeps;                   /* SOR sweeps performed (FP32 sweeps in mixed mode) */
   int32_t refinements;              /* FP64 residual evaluations */
   double rel_residual;              /* ||b - A w||_2 / ||b||_2 in FP64 at exit */
-  double seconds;                   /* device time */
+  double seconds;                   /* wall time of the solve on the host */
   int32_t status;
 } ufe_linsolve_stats;
+
+/* Live kernel timing (HIP events around every SOR colour-kernel launch). */
+typedef struct ufe_kernel_profile {
+  int64_t launches;                 /* colour-kernel launches timed */
+  double seconds;                   /* summed event time of those launches */
+  double algorithmic_bytes;         /* summed algorithmic bytes of those launches */
+  int64_t rows;                     /* summed rows (vertex-sweeps) of those launches */
+} ufe_kernel_profile;
 
 typedef struct ufe_ctx ufe_ctx;
 
@@ -149,6 +173,7 @@ int ufe_field_set(ufe_ctx *ctx, int id, const void *host);   /* reference layout
 int ufe_field_get(ufe_ctx *ctx, int id, void *host);
 int ufe_field_set_device(ufe_ctx *ctx, int id, const void *dev);
 int ufe_field_get_device(ufe_ctx *ctx, int id, void *dev);
+int ufe_field_fill(ufe_ctx *ctx, int id, double value);     /* every element := value */
 
 /* ---- the hot path (operate on the device-resident fields) --------------- */
 /* update_general_ice_model_data, src/general_ice_model_data_module.f90:23-96
@@ -168,11 +193,8 @@ int ufe_calculate_ice_thickness_change(ufe_ctx *ctx, double dt);
 /* Recompute eta, N, S and the SOR coefficients from the current velocities
  * (one Picard step without the solve). */
 int ufe_ssa_prepare(ufe_ctx *ctx);
-/* Solve the linearised system to ||b - A w|| / ||b|| <= rel_tol (FP64 norm),
- * in the precision of params.precision, starting from the current
- * velocities. */
-int ufe_ssa_linear_solve(ufe_ctx *ctx, double rel_tol, int32_t max_sweeps,
-                         ufe_linsolve_stats *st);
+/* Solve the linearised system from the current velocities. */
+int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *opts, ufe_linsolve_stats *st);
 /* FP64 relative residual of the current velocities. */
 int ufe_ssa_residual(ufe_ctx *ctx, double *rel_residual);
 
@@ -195,6 +217,15 @@ int ufe_op_map_Aa_to_Ac_3D(ufe_ctx *ctx, const double *d_Aa, double *d_Ac, int n
 int ufe_op_map_Ac_to_Aa(ufe_ctx *ctx, const double *d_Ac, double *d_Aa);
 int ufe_op_map_Ac_to_Aa_3D(ufe_ctx *ctx, const double *d_Ac, double *d_Aa, int nz);
 int ufe_op_rotate_xy_to_po(ufe_ctx *ctx, const double *x, const double *y, double *p, double *o);
+
+/* ---- measurement -------------------------------------------------------- */
+/* Enable/disable event timing of the SOR colour kernels; reading resets. */
+int ufe_profile_enable(ufe_ctx *ctx, int enable);
+int ufe_profile_read(ufe_ctx *ctx, ufe_kernel_profile *out);
+/* Algorithmic bytes of one FP64 / FP32 sweep over all SOR rows, and the
+ * mean AaAc degree of those rows: B(w) = 4 (n+1) + w (2 n + 9) per row. */
+int ufe_sweep_bytes(ufe_ctx *ctx, double *fp64_bytes, double *fp32_bytes, double *mean_degree,
+                    int64_t *rows);
 
 /* ---- device helpers for callers that manage their own buffers ----------- */
 int ufe_device_alloc(ufe_ctx *ctx, size_t bytes, void **dev);

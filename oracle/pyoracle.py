@@ -238,3 +238,38 @@ def run_step(wl, threads=None):
     out["stats"] = st
     out["mesh"] = om
     return out
+
+
+def mesh_from_arrays(mesh, arrays, colour_nV):
+    """OracleMesh filled with secondary arrays built elsewhere (used only to
+    time the oracle's sweeps on a mesh too large for its literal colouring)."""
+    om = OracleMesh(mesh)
+    for k, a in om.arr.items():
+        if k in arrays and k not in ("V", "A", "Cw", "nC", "C", "niTri", "iTri", "edge_index", "Tri"):
+            a[...] = arrays[k]
+    for k in range(5):
+        om.s.d.colour_nV[k] = int(colour_nV[k])
+    return om
+
+
+def time_sor_sweeps(om, fields, p: DynParams, sweeps, threads=None):
+    """Run `sweeps` oracle SOR sweeps of solve_SSA_linearised on the given
+    AaAc fields (original order) and return the wall time in seconds."""
+    import time
+    if threads:
+        set_threads(threads)
+    ice = OracleIce(om)
+    for k, v in fields.items():
+        ice[k][...] = v
+    q = DynParams(**{f.name: getattr(p, f.name) for f in __import__("dataclasses").fields(p)})
+    q.SSA_max_inner_loops = int(sweeps)
+    q.SSA_max_residual_UV = 0.0
+    ps = params_struct(q)
+    did = ct.c_int(0)
+    lm = ct.c_double(0)
+    ns = ct.c_int(0)
+    t0 = time.perf_counter()
+    load().om_solve_SSA_linearised(ct.byref(om.s), ct.byref(ice.s), ct.byref(ps), ct.byref(did),
+                                   ct.byref(lm), ct.byref(ns))
+    dt = time.perf_counter() - t0
+    return dt, ns.value

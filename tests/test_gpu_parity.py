@@ -89,3 +89,62 @@ def test_dynamics_step_matches_oracle(ctx, name):
         e = _rel(out[k], ref[k])
         print(f"  {k}: rel L2 {e:.2e}", flush=True)
         assert e <= TOL_FP64, (k, e)
+
+
+def _linear_system(ctx, n=20000, seed=5, picard=4):
+    """A frozen linearised SSA system on a small marine mesh."""
+    wl = geometry.marine(n, seed=seed, picard=picard)
+    step = Step(ctx, wl)
+    ctx.update_general_ice_model_data()
+    ctx.solve_SIA()
+    ctx.solve_SSA()
+    ctx.ssa_prepare()
+    return wl, step
+
+
+def test_linear_sweeps_match_oracle(ctx):
+    """K FP64 sweeps of the GPU linear solve equal K oracle sweeps."""
+    print("[linear] fp64 sweeps vs oracle", flush=True)
+    wl, step = _linear_system(ctx)
+    po = _oracle()
+    names = ("Hi_AaAc", "Hb_AaAc", "SL_AaAc", "eta_AaAc", "S_AaAc", "dHs_dx_shelf_AaAc",
+             "dHs_dy_shelf_AaAc", "U_SSA_AaAc", "V_SSA_AaAc")
+    fields = {k: ctx.get(k) for k in names}
+    K = 40
+    ctx.ssa_linear_solve(0.0, K, precision=abi.UFE_FP64, check_interval=K)
+    u, v = ctx.get("U_SSA_AaAc"), ctx.get("V_SSA_AaAc")
+    om = po.mesh_from_arrays(wl.mesh, step.ma.arr, step.ma.colour_nV)
+    ice = po.OracleIce(om)
+    for k, a in fields.items():
+        ice[k][...] = a
+    q = wl.params
+    q.SSA_max_inner_loops, q.SSA_max_residual_UV = K, 0.0
+    import ctypes as ct
+    ps = po.params_struct(q)
+    did, lm, ns = ct.c_int(0), ct.c_double(0), ct.c_int(0)
+    po.load().om_solve_SSA_linearised(ct.byref(om.s), ct.byref(ice.s), ct.byref(ps), ct.byref(did),
+                                      ct.byref(lm), ct.byref(ns))
+    assert ns.value == K
+    eu, ev = _rel(u, ice["U_SSA_AaAc"]), _rel(v, ice["V_SSA_AaAc"])
+    print(f"  after {K} sweeps: rel L2 u {eu:.2e} v {ev:.2e}", flush=True)
+    assert eu <= TOL_FP64 and ev <= TOL_FP64
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+def test_linear_solve_fp64_and_mixed(ctx, omega):
+    print(f"[linear] solve to 1e-6, omega={omega}", flush=True)
+    _linear_system(ctx)
+    res = {}
+    for mode in (abi.UFE_FP64, abi.UFE_MIXED):
+        ctx.fill("U_SSA_AaAc", 0.0)
+        ctx.fill("V_SSA_AaAc", 0.0)
+        st = ctx.ssa_linear_solve(1e-6, 100000, precision=mode, omega=omega, check_interval=32)
+        assert st.status == 0 and st.rel_residual <= 1e-6, (mode, st.rel_residual)
+        # the reported residual is the FP64 residual of the returned velocities
+        assert abs(ctx.ssa_residual() - st.rel_residual) <= 1e-12
+        res[mode] = (ctx.get("U_SSA_AaAc"), ctx.get("V_SSA_AaAc"), st.sweeps)
+        print(f"  mode {mode}: sweeps {st.sweeps} rel {st.rel_residual:.2e}", flush=True)
+    (u0, v0, _), (u1, v1, _) = res[abi.UFE_FP64], res[abi.UFE_MIXED]
+    err = np.sqrt(np.sum((u1 - u0) ** 2 + (v1 - v0) ** 2) / np.sum(u0 ** 2 + v0 ** 2))
+    print(f"  mixed vs fp64 rel L2 {err:.2e}", flush=True)
+    assert err <= 1e-4

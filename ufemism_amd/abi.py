@@ -40,6 +40,7 @@ EXPORTS = [
     "ufe_op_map_Aa_to_Ac", "ufe_op_map_Aa_to_Ac_3D", "ufe_op_map_Ac_to_Aa",
     "ufe_op_map_Ac_to_Aa_3D", "ufe_op_rotate_xy_to_po",
     "ufe_device_alloc", "ufe_device_free", "ufe_memcpy_h2d", "ufe_memcpy_d2h", "ufe_synchronize",
+    "ufe_field_fill", "ufe_profile_enable", "ufe_profile_read", "ufe_sweep_bytes",
 ]
 
 _P = ct.c_void_p
@@ -74,6 +75,16 @@ class SSAStats(ct.Structure):
                 ("status", ct.c_int32), ("sor_seconds", ct.c_double), ("total_seconds", ct.c_double)]
 
 
+class LinSolveOpts(ct.Structure):
+    _fields_ = [("rel_tol", ct.c_double), ("max_sweeps", ct.c_int32), ("precision", ct.c_int32),
+                ("omega", ct.c_double), ("check_interval", ct.c_int32), ("use_graph", ct.c_int32)]
+
+
+class KernelProfile(ct.Structure):
+    _fields_ = [("launches", ct.c_int64), ("seconds", ct.c_double),
+                ("algorithmic_bytes", ct.c_double), ("rows", ct.c_int64)]
+
+
 class LinSolveStats(ct.Structure):
     _fields_ = [("sweeps", ct.c_int32), ("refinements", ct.c_int32), ("rel_residual", ct.c_double),
                 ("seconds", ct.c_double), ("status", ct.c_int32)]
@@ -94,7 +105,8 @@ def load():
         lib.ufe_field_name.restype = ct.c_char_p
         lib.ufe_field_id.argtypes = [ct.c_char_p]
         lib.ufe_calculate_ice_thickness_change.argtypes = [_P, ct.c_double]
-        lib.ufe_ssa_linear_solve.argtypes = [_P, ct.c_double, ct.c_int32, _P]
+        lib.ufe_ssa_linear_solve.argtypes = [_P, _P, _P]
+        lib.ufe_field_fill.argtypes = [_P, ct.c_int, ct.c_double]
         for name in EXPORTS:
             getattr(lib, name)  # raises AttributeError if a symbol is missing
         _lib = lib
@@ -259,11 +271,33 @@ class Context:
     def ssa_prepare(self):
         self._chk(self.lib.ufe_ssa_prepare(self.h), "ssa_prepare")
 
-    def ssa_linear_solve(self, rel_tol, max_sweeps) -> LinSolveStats:
+    def ssa_linear_solve(self, rel_tol, max_sweeps, precision=UFE_FP64, omega=0.0,
+                         check_interval=16, use_graph=False) -> LinSolveStats:
+        o = LinSolveOpts(float(rel_tol), int(max_sweeps), int(precision), float(omega),
+                         int(check_interval), int(bool(use_graph)))
         st = LinSolveStats()
-        self._chk(self.lib.ufe_ssa_linear_solve(self.h, float(rel_tol), int(max_sweeps),
-                                                ct.byref(st)), "ssa_linear_solve")
+        self._chk(self.lib.ufe_ssa_linear_solve(self.h, ct.byref(o), ct.byref(st)),
+                  "ssa_linear_solve")
         return st
+
+    def fill(self, name, value):
+        fid = self.fields[name][0]
+        self._chk(self.lib.ufe_field_fill(self.h, fid, float(value)), f"fill {name}")
+
+    def profile_enable(self, on=True):
+        self._chk(self.lib.ufe_profile_enable(self.h, int(bool(on))), "profile_enable")
+
+    def profile_read(self) -> KernelProfile:
+        p = KernelProfile()
+        self._chk(self.lib.ufe_profile_read(self.h, ct.byref(p)), "profile_read")
+        return p
+
+    def sweep_bytes(self):
+        b64, b32, nd = ct.c_double(), ct.c_double(), ct.c_double()
+        rows = ct.c_int64()
+        self._chk(self.lib.ufe_sweep_bytes(self.h, ct.byref(b64), ct.byref(b32), ct.byref(nd),
+                                           ct.byref(rows)), "sweep_bytes")
+        return dict(fp64=b64.value, fp32=b32.value, mean_degree=nd.value, rows=rows.value)
 
     def ssa_residual(self) -> float:
         v = ct.c_double()

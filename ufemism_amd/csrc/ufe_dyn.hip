@@ -151,6 +151,18 @@ struct ufe_ctx {
   double *d_ViSMB = nullptr;
   unsigned long long *d_count = nullptr;
   SorFlags *d_flags = nullptr;
+  // FP32 copies for the mixed-precision sweeps
+  float2 *d_cU32 = nullptr, *d_EV32 = nullptr, *d_R32 = nullptr, *d_D32 = nullptr;
+  float *d_Sxy32 = nullptr;
+  // algorithmic bytes per colour (SOR rows only)
+  double bytes64[5] = {0, 0, 0, 0, 0}, bytes32[5] = {0, 0, 0, 0, 0};
+  double degsum = 0.0;
+  // live kernel profile
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct Pending { hipEvent_t a, b; double bytes; int64_t rows; };
+  std::vector<Pending> pending;
+  ufe_kernel_profile prof_acc{};
   SorFlags *h_flags = nullptr;      // pinned
   double *h_scalar = nullptr;       // pinned
   std::vector<void *> allocs;       // everything to free with the mesh
@@ -349,6 +361,11 @@ int ufe_ctx_destroy(ufe_ctx *ctx) {
   if (!ctx) return UFE_OK;
   (void)hipSetDevice(ctx->device);
   free_mesh(ctx);
+  for (auto &p : ctx->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
   if (ctx->h_scalar) (void)hipHostFree(ctx->h_scalar);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -713,6 +730,31 @@ int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
   if ((r = dupload(ctx, &ctx->d_cor_ptr, cptr2.data(), cptr2.size()))) return r;
   if ((r = dupload(ctx, &ctx->d_cor_col, ccol.data(), ccol.size()))) return r;
 
+  // --- FP32 copies and algorithmic bytes ---------------------------------
+  {
+    std::vector<float2> cU32(ne);
+    for (int64_t e = 0; e < ne; e++) cU32[e] = make_float2((float)cU[e].x, (float)cU[e].y);
+    std::vector<float> Sxy32(nA);
+    for (int p = 0; p < nA; p++) Sxy32[p] = (float)Sxy[p];
+    if ((r = dupload(ctx, &ctx->d_cU32, cU32.data(), cU32.size()))) return r;
+    if ((r = dupload(ctx, &ctx->d_Sxy32, Sxy32.data(), Sxy32.size()))) return r;
+    if ((r = dalloc(ctx, &ctx->d_EV32, nA))) return r;
+    if ((r = dalloc(ctx, &ctx->d_R32, nA))) return r;
+    if ((r = dalloc(ctx, &ctx->d_D32, nA))) return r;
+    ctx->degsum = 0.0;
+    for (int c = 0; c < 5; c++) {
+      double b64 = 0.0, b32 = 0.0;
+      for (int p = ctx->colour_start[c]; p < ctx->colour_start[c] + ctx->colour_active[c]; p++) {
+        const double n = deg[p];
+        b64 += 4.0 * (n + 1.0) + 8.0 * (2.0 * n + 9.0);
+        b32 += 4.0 * (n + 1.0) + 4.0 * (2.0 * n + 9.0);
+        ctx->degsum += n;
+      }
+      ctx->bytes64[c] = b64;
+      ctx->bytes32[c] = b32;
+    }
+  }
+
   // --- fields (zero-initialised, like the reference's shared windows) ------
   ctx->field_ptr.assign(NFIELDS, nullptr);
   for (int i = 0; i < NFIELDS; i++) {
@@ -869,15 +911,82 @@ static NeumannList cor_list(ufe_ctx *c) {
   return L;
 }
 
-static void enqueue_sweep_fp64(ufe_ctx *ctx, const AaAcMesh &m, double omega, double tol, int max_sweeps) {
+static hipEvent_t prof_event(ufe_ctx *ctx) {
+  if (ctx->ev_pool.empty()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  hipEvent_t e = ctx->ev_pool.back();
+  ctx->ev_pool.pop_back();
+  return e;
+}
+// wrap one colour-kernel launch in events when profiling is on
+extern "C++" template <typename F>
+static void timed_launch(ufe_ctx *ctx, double bytes, int64_t rows, F &&launch) {
+  if (!ctx->prof) {
+    launch();
+    return;
+  }
+  hipEvent_t a = prof_event(ctx), b = prof_event(ctx);
+  if (!a || !b) {
+    launch();
+    return;
+  }
+  (void)hipEventRecord(a, ctx->stream);
+  launch();
+  (void)hipEventRecord(b, ctx->stream);
+  ctx->pending.push_back({a, b, bytes, rows});
+}
+// after a stream synchronisation: fold the recorded pairs into the profile
+static void prof_collect(ufe_ctx *ctx) {
+  for (auto &p : ctx->pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      ctx->prof_acc.launches++;
+      ctx->prof_acc.seconds += ms * 1e-3;
+      ctx->prof_acc.algorithmic_bytes += p.bytes;
+      ctx->prof_acc.rows += p.rows;
+    }
+    ctx->ev_pool.push_back(p.a);
+    ctx->ev_pool.push_back(p.b);
+  }
+  ctx->pending.clear();
+}
+
+static void enqueue_sweep_fp64(ufe_ctx *ctx, const AaAcMesh &m, double omega, double tol, int max_sweeps,
+                               bool with_flags = true) {
+  SorFlags *fl = with_flags ? ctx->d_flags : nullptr;
   for (int c = 0; c < 5; c++) {
     const int n = ctx->colour_active[c];
     if (n == 0) continue;
-    k_sor_colour<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(m, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
-                                                      ctx->colour_start[c], n, omega, ctx->d_flags);
+    timed_launch(ctx, ctx->bytes64[c], n, [&] {
+      k_sor_colour<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(m, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
+                                                          ctx->colour_start[c], n, omega, fl);
+    });
   }
-  if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], ctx->d_flags);
-  k_sor_finalize<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0], ctx->d_flags, tol, max_sweeps);
+  if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], fl);
+  if (with_flags)
+    k_sor_finalize<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0], ctx->d_flags, tol, max_sweeps);
+  else
+    k_neumann_scalar_pair_corners<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
+}
+
+static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega) {
+  Mesh32 q;
+  q.cU = ctx->d_cU32;
+  q.Sxy = ctx->d_Sxy32;
+  q.EV = ctx->d_EV32;
+  for (int c = 0; c < 5; c++) {
+    const int n = ctx->colour_active[c];
+    if (n == 0) continue;
+    timed_launch(ctx, ctx->bytes32[c], n, [&] {
+      k_sor_colour_f32<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(m, q, ctx->d_D32, ctx->d_R32, ctx->colour_start[c],
+                                                              n, omega);
+    });
+  }
+  if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_D32);
+  k_neumann_corners_f32<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_D32);
 }
 
 // solve_SSA_linearised sweep loop (:601-691): returns the flags at exit.
@@ -896,6 +1005,7 @@ static int run_sor_reference(ufe_ctx *ctx, double tol, int max_sweeps, SorFlags 
     LAUNCHCHK(ctx);
     HIPCHK(ctx, hipMemcpyAsync(ctx->h_flags, ctx->d_flags, sizeof(SorFlags), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
     if (ctx->h_flags->done || queued >= max_sweeps) break;
     if (ctx->params.sweep_batch <= 0) batch = std::min(batch * 2, 64);
   }
@@ -1036,21 +1146,27 @@ int ufe_ssa_prepare(ufe_ctx *ctx) {
   return UFE_OK;
 }
 
-static int residual_fp64(ufe_ctx *ctx, double *rel) {
+// FP64 relative residual; with store=true also hands r to the FP32 sweeps.
+static int residual_fp64(ufe_ctx *ctx, double *rel, bool store = false) {
   const AaAcMesh m = aaac_mesh(ctx);
   int nb_tot = 0;
   for (int c = 0; c < 5; c++) {
     const int n = ctx->colour_active[c];
     if (!n) continue;
     const int nb = nblk(n);
-    k_ssa_residual<<<nb, BLOCK, 0, ctx->stream>>>(m, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
-                                                   ctx->colour_start[c], n, ctx->d_part + 2 * nb_tot);
+    if (store)
+      k_residual_store<<<nb, BLOCK, 0, ctx->stream>>>(m, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
+                                                       ctx->colour_start[c], n, ctx->d_R32, ctx->d_part + 2 * nb_tot);
+    else
+      k_ssa_residual<<<nb, BLOCK, 0, ctx->stream>>>(m, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
+                                                     ctx->colour_start[c], n, ctx->d_part + 2 * nb_tot);
     nb_tot += nb;
   }
   k_ssa_rn<<<1, 1024, 0, ctx->stream>>>(nb_tot, ctx->d_part, ctx->d_scalar);
   LAUNCHCHK(ctx);
   HIPCHK(ctx, hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  prof_collect(ctx);
   *rel = ctx->h_scalar[0];
   return UFE_OK;
 }
@@ -1060,33 +1176,85 @@ int ufe_ssa_residual(ufe_ctx *ctx, double *rel_residual) {
   return residual_fp64(ctx, rel_residual);
 }
 
-int ufe_ssa_linear_solve(ufe_ctx *ctx, double rel_tol, int32_t max_sweeps, ufe_linsolve_stats *st) {
-  if (!ctx || !ctx->have_mesh || !st) return UFE_BAD_INPUT;
+int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_stats *st) {
+  if (!ctx || !ctx->have_mesh || !st || !o) return UFE_BAD_INPUT;
   std::memset(st, 0, sizeof(*st));
   const AaAcMesh m = aaac_mesh(ctx);
-  const double omega = ctx->params.SSA_SOR_omega;
-  const int check = ctx->params.sweep_batch > 0 ? ctx->params.sweep_batch : 16;
+  const double omega = o->omega > 0.0 ? o->omega : ctx->params.SSA_SOR_omega;
+  const int check = o->check_interval > 0 ? o->check_interval : 16;
+  const bool mixed = o->precision == UFE_MIXED;
   auto t0 = std::chrono::steady_clock::now();
   double rel = 0.0;
   int r;
-  if ((r = residual_fp64(ctx, &rel))) return r;
-  st->refinements = 1;
-  if (ctx->params.precision != UFE_FP64) {
-    ctx->err = "mixed-precision linear solve not built yet";
-    return UFE_NOT_SUPPORTED;
+  if (mixed) {  // FP32 centre coefficients of the current system
+    k_to_f32_pair<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_pair[1], ctx->d_EV32);
+    LAUNCHCHK(ctx);
   }
-  while (rel > rel_tol && st->sweeps < max_sweeps) {
-    k_sor_flags_init<<<1, 1, 0, ctx->stream>>>(ctx->d_flags);
-    const int nb = std::min(check, max_sweeps - st->sweeps);
-    // the reference stopping test is disabled here (tolerance 0, no cap)
-    for (int b = 0; b < nb; b++) enqueue_sweep_fp64(ctx, m, omega, 0.0, 1 << 30);
+  if ((r = residual_fp64(ctx, &rel, mixed))) return r;
+  st->refinements = 1;
+  while (rel > o->rel_tol && st->sweeps < o->max_sweeps) {
+    const int nb = std::min(check, o->max_sweeps - st->sweeps);
+    if (mixed) {
+      k_fill<float2><<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->d_D32, ctx->nA, make_float2(0.0f, 0.0f));
+      for (int b = 0; b < nb; b++) enqueue_sweep_fp32(ctx, m, (float)omega);
+      k_apply_correction<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_D32, ctx->d_pair[0]);
+    } else {
+      for (int b = 0; b < nb; b++) enqueue_sweep_fp64(ctx, m, omega, 0.0, 1 << 30, false);
+    }
+    LAUNCHCHK(ctx);
     st->sweeps += nb;
-    if ((r = residual_fp64(ctx, &rel))) return r;
+    if ((r = residual_fp64(ctx, &rel, mixed))) return r;
     st->refinements++;
   }
   st->rel_residual = rel;
   st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  st->status = rel <= rel_tol ? UFE_OK : UFE_SOR_NOT_CONVERGED;
+  st->status = rel <= o->rel_tol ? UFE_OK : UFE_SOR_NOT_CONVERGED;
+  return UFE_OK;
+}
+
+int ufe_profile_enable(ufe_ctx *ctx, int enable) {
+  if (!ctx) return UFE_BAD_INPUT;
+  ctx->prof = enable != 0;
+  return UFE_OK;
+}
+int ufe_profile_read(ufe_ctx *ctx, ufe_kernel_profile *out) {
+  if (!ctx || !out) return UFE_BAD_INPUT;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  prof_collect(ctx);
+  *out = ctx->prof_acc;
+  ctx->prof_acc = ufe_kernel_profile{};
+  return UFE_OK;
+}
+int ufe_sweep_bytes(ufe_ctx *ctx, double *b64, double *b32, double *mean_deg, int64_t *rows) {
+  if (!ctx || !ctx->have_mesh) return UFE_BAD_INPUT;
+  double s64 = 0.0, s32 = 0.0;
+  int64_t n = 0;
+  for (int c = 0; c < 5; c++) {
+    s64 += ctx->bytes64[c];
+    s32 += ctx->bytes32[c];
+    n += ctx->colour_active[c];
+  }
+  if (b64) *b64 = s64;
+  if (b32) *b32 = s32;
+  if (mean_deg) *mean_deg = n ? ctx->degsum / (double)n : 0.0;
+  if (rows) *rows = n;
+  return UFE_OK;
+}
+
+int ufe_field_fill(ufe_ctx *ctx, int id, double value) {
+  if (!ctx || !ctx->have_mesh || id < 0 || id >= NFIELDS) return UFE_BAD_INPUT;
+  const FieldDef &fd = FIELDS[id];
+  const int64_t n = (int64_t)grid_len(ctx, fd.grid);
+  if (fd.pair >= 0) {
+    // fill one component of an interleaved pair
+    k_fill_pair_comp<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->d_pair[fd.pair], fd.comp, value);
+  } else if (fd.dtype == D_F64) {
+    k_fill<double><<<nblk(n), BLOCK, 0, ctx->stream>>>((double *)ctx->field_ptr[id], n, value);
+  } else {
+    k_fill<int32_t><<<nblk(n), BLOCK, 0, ctx->stream>>>((int32_t *)ctx->field_ptr[id], n, (int32_t)value);
+  }
+  LAUNCHCHK(ctx);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return UFE_OK;
 }
 
