@@ -117,14 +117,19 @@ typedef struct ufe_linsolve_opts {
   int32_t max_sweeps;
   int32_t precision;                /* UFE_FP64 or UFE_MIXED */
   double omega;                     /* over-relaxation; <= 0: params.SSA_SOR_omega */
-  int32_t check_interval;           /* sweeps between FP64 residual evaluations; <= 0: 16 */
-  int32_t use_graph;                /* replay check_interval sweeps as one HIP graph */
+  int32_t check_interval;           /* plain SOR: sweeps between FP64 residual evaluations (<= 0: 16) */
+  int32_t use_graph;                /* reserved, must be 0 */
+  int32_t gmres_restart;            /* 0: plain SOR sweeps; m > 0: restarted GMRES(m) preconditioned
+                                       by precond_sweeps SOR sweeps, one FP64 refinement per cycle */
+  int32_t precond_sweeps;           /* SOR sweeps per preconditioner application (<= 0: 2) */
+  int32_t orth_passes;              /* classical Gram-Schmidt passes, 1 or 2 (<= 0: 2) */
 } ufe_linsolve_opts;
 
 /* Statistics of one linearised solve. */
 typedef struct ufe_linsolve_stats {
   int32_t sweeps;                   /* SOR sweeps performed (FP32 sweeps in mixed mode) */
   int32_t refinements;              /* FP64 residual evaluations */
+  int32_t matvecs;                  /* operator applications outside the SOR sweeps (GMRES) */
   double rel_residual;              /* ||b - A w||_2 / ||b||_2 in FP64 at exit */
   double seconds;                   /* wall time of the solve on the host */
   int32_t status;

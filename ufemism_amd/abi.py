@@ -77,7 +77,9 @@ class SSAStats(ct.Structure):
 
 class LinSolveOpts(ct.Structure):
     _fields_ = [("rel_tol", ct.c_double), ("max_sweeps", ct.c_int32), ("precision", ct.c_int32),
-                ("omega", ct.c_double), ("check_interval", ct.c_int32), ("use_graph", ct.c_int32)]
+                ("omega", ct.c_double), ("check_interval", ct.c_int32), ("use_graph", ct.c_int32),
+                ("gmres_restart", ct.c_int32), ("precond_sweeps", ct.c_int32),
+                ("orth_passes", ct.c_int32)]
 
 
 class KernelProfile(ct.Structure):
@@ -86,7 +88,8 @@ class KernelProfile(ct.Structure):
 
 
 class LinSolveStats(ct.Structure):
-    _fields_ = [("sweeps", ct.c_int32), ("refinements", ct.c_int32), ("rel_residual", ct.c_double),
+    _fields_ = [("sweeps", ct.c_int32), ("refinements", ct.c_int32), ("matvecs", ct.c_int32),
+                ("rel_residual", ct.c_double),
                 ("seconds", ct.c_double), ("status", ct.c_int32)]
 
 
@@ -272,9 +275,10 @@ class Context:
         self._chk(self.lib.ufe_ssa_prepare(self.h), "ssa_prepare")
 
     def ssa_linear_solve(self, rel_tol, max_sweeps, precision=UFE_FP64, omega=0.0,
-                         check_interval=16, use_graph=False) -> LinSolveStats:
+                         check_interval=16, gmres=0, precond_sweeps=2,
+                         orth_passes=2) -> LinSolveStats:
         o = LinSolveOpts(float(rel_tol), int(max_sweeps), int(precision), float(omega),
-                         int(check_interval), int(bool(use_graph)))
+                         int(check_interval), 0, int(gmres), int(precond_sweeps), int(orth_passes))
         st = LinSolveStats()
         self._chk(self.lib.ufe_ssa_linear_solve(self.h, ct.byref(o), ct.byref(st)),
                   "ssa_linear_solve")

@@ -154,6 +154,13 @@ struct ufe_ctx {
   // FP32 copies for the mixed-precision sweeps
   float2 *d_cU32 = nullptr, *d_EV32 = nullptr, *d_R32 = nullptr, *d_D32 = nullptr;
   float *d_Sxy32 = nullptr;
+  // Krylov work space (allocated on first use)
+  int kry_m = 0;
+  bool kry_f32 = true;
+  void *kry_V = nullptr, *kry_Z = nullptr, *kry_w = nullptr, *kry_x = nullptr;
+  double2 *kry_R64 = nullptr;
+  double *kry_part = nullptr, *kry_H1 = nullptr, *kry_H2 = nullptr, *kry_Hs = nullptr, *kry_y = nullptr,
+         *kry_nrm = nullptr;
   // algorithmic bytes per colour (SOR rows only)
   double bytes64[5] = {0, 0, 0, 0, 0}, bytes32[5] = {0, 0, 0, 0, 0};
   double degsum = 0.0;
@@ -185,6 +192,23 @@ struct ufe_ctx {
       return UFE_HIP_ERROR;                                                        \
     }                                                                              \
   } while (0)
+
+// Hilbert-curve index of a point on a 2^16 x 2^16 grid
+static uint64_t hilbert_index(uint32_t x, uint32_t y) {
+  uint64_t d = 0;
+  for (uint32_t s = 1u << 15; s > 0; s >>= 1) {
+    const uint32_t rx = (x & s) > 0, ry = (y & s) > 0;
+    d += (uint64_t)s * s * ((3 * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = s - 1 - x;
+        y = s - 1 - y;
+      }
+      std::swap(x, y);
+    }
+  }
+  return d;
+}
 
 static inline int nblk(int64_t n, int b = BLOCK) { return (int)((n + b - 1) / b); }
 // grid for XCD-major kernels: a multiple of the 8 XCDs
@@ -602,24 +626,39 @@ int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
   auto is_edge = [&](int ai1) {
     return ai1 <= nV ? m->edge_index[ai1 - 1] > 0 : m->edge_index_Ac[ai1 - nV - 1] > 0;
   };
+  // Inside a colour the update order is free (same-colour points are never
+  // neighbours), so rows are sorted along a Hilbert curve of their
+  // coordinates: a wavefront then works on a compact patch and its
+  // neighbour gathers from the other colours hit few cache lines.
+  std::vector<uint64_t> hkey(nA);
+  {
+    const double span = std::max(m->xmax - m->xmin, m->ymax - m->ymin);
+    for (int a = 0; a < nA; a++) {
+      const double fx = (at(m->VAaAc, ldAA, a + 1, 1) - m->xmin) / span;
+      const double fy = (at(m->VAaAc, ldAA, a + 1, 2) - m->ymin) / span;
+      const uint32_t gx = (uint32_t)std::min(65535.0, std::max(0.0, fx * 65535.0));
+      const uint32_t gy = (uint32_t)std::min(65535.0, std::max(0.0, fy * 65535.0));
+      hkey[a] = hilbert_index(gx, gy);
+    }
+  }
   std::vector<int32_t> perm;
   perm.reserve(nA);
   for (int c = 1; c <= 5; c++) {
     ctx->colour_start[c - 1] = (int)perm.size();
-    int act = 0;
+    std::vector<int32_t> act, edg;
     for (int k = 1; k <= m->colour_nV[c - 1]; k++) {
       const int ai = at(m->colour_vi, ldAA, k, c);
       if (ai < 1 || ai > nA) {
         ctx->err = "colour_vi holds an index outside 1..nVAaAc";
         return UFE_BAD_INPUT;
       }
-      if (!is_edge(ai)) { perm.push_back(ai - 1); act++; }
+      (is_edge(ai) ? edg : act).push_back(ai - 1);
     }
-    for (int k = 1; k <= m->colour_nV[c - 1]; k++) {
-      const int ai = at(m->colour_vi, ldAA, k, c);
-      if (is_edge(ai)) perm.push_back(ai - 1);
-    }
-    ctx->colour_active[c - 1] = act;
+    auto by_key = [&](int32_t a, int32_t b) { return hkey[a] < hkey[b] || (hkey[a] == hkey[b] && a < b); };
+    std::sort(act.begin(), act.end(), by_key);
+    perm.insert(perm.end(), act.begin(), act.end());
+    perm.insert(perm.end(), edg.begin(), edg.end());
+    ctx->colour_active[c - 1] = (int)act.size();
   }
   ctx->colour_start[5] = nA;
   if ((int)perm.size() != nA) {
@@ -740,6 +779,7 @@ int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
     if ((r = dupload(ctx, &ctx->d_Sxy32, Sxy32.data(), Sxy32.size()))) return r;
     if ((r = dalloc(ctx, &ctx->d_EV32, nA))) return r;
     if ((r = dalloc(ctx, &ctx->d_R32, nA))) return r;
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_R32, 0, sizeof(float2) * nA, ctx->stream));
     if ((r = dalloc(ctx, &ctx->d_D32, nA))) return r;
     ctx->degsum = 0.0;
     for (int c = 0; c < 5; c++) {
@@ -972,21 +1012,40 @@ static void enqueue_sweep_fp64(ufe_ctx *ctx, const AaAcMesh &m, double omega, do
     k_neumann_scalar_pair_corners<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
 }
 
-static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega) {
+static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, float2 *D = nullptr,
+                               const float2 *R = nullptr, bool backward = false) {
   Mesh32 q;
   q.cU = ctx->d_cU32;
   q.Sxy = ctx->d_Sxy32;
   q.EV = ctx->d_EV32;
-  for (int c = 0; c < 5; c++) {
+  if (!D) D = ctx->d_D32;
+  if (!R) R = ctx->d_R32;
+  for (int k = 0; k < 5; k++) {
+    const int c = backward ? 4 - k : k;
     const int n = ctx->colour_active[c];
     if (n == 0) continue;
     timed_launch(ctx, ctx->bytes32[c], n, [&] {
-      k_sor_colour_f32<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(m, q, ctx->d_D32, ctx->d_R32, ctx->colour_start[c],
-                                                              n, omega);
+      k_sor_colour_f32<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(m, q, D, R, ctx->colour_start[c], n, omega);
     });
   }
-  if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_D32);
-  k_neumann_corners_f32<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_D32);
+  if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D);
+  k_neumann_corners_f32<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), D);
+}
+
+// FP64 sweep on an arbitrary vector pair (Krylov preconditioner)
+static void enqueue_sweep_fp64_vec(ufe_ctx *ctx, const AaAcMesh &m, double omega, double2 *D, const double2 *R,
+                                   bool backward) {
+  for (int k = 0; k < 5; k++) {
+    const int c = backward ? 4 - k : k;
+    const int n = ctx->colour_active[c];
+    if (n == 0) continue;
+    timed_launch(ctx, ctx->bytes64[c], n, [&] {
+      k_sor_colour<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(m, D, ctx->d_pair[1], R, ctx->colour_start[c], n, omega,
+                                                          nullptr);
+    });
+  }
+  if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D, nullptr);
+  k_neumann_scalar_pair_corners<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), D);
 }
 
 // solve_SSA_linearised sweep loop (:601-691): returns the flags at exit.
@@ -1176,6 +1235,143 @@ int ufe_ssa_residual(ufe_ctx *ctx, double *rel_residual) {
   return residual_fp64(ctx, rel_residual);
 }
 
+static ActiveRanges active_ranges(ufe_ctx *ctx) {
+  ActiveRanges a;
+  for (int c = 0; c < 5; c++) {
+    a.start[c] = ctx->colour_start[c];
+    a.count[c] = ctx->colour_active[c];
+  }
+  return a;
+}
+
+static int krylov_alloc(ufe_ctx *ctx, int m, bool f32) {
+  if (ctx->kry_V && ctx->kry_m >= m && ctx->kry_f32 == f32) return UFE_OK;
+  const size_t n = ctx->nA, es = f32 ? sizeof(float2) : sizeof(double2);
+  int r;
+  char *V, *Z, *w, *x;
+  if ((r = dalloc(ctx, &V, es * n * (m + 1)))) return r;
+  if ((r = dalloc(ctx, &Z, es * n * m))) return r;
+  if ((r = dalloc(ctx, &w, es * n))) return r;
+  if ((r = dalloc(ctx, &x, es * n))) return r;
+  ctx->kry_V = V;
+  ctx->kry_Z = Z;
+  ctx->kry_w = w;
+  ctx->kry_x = x;
+  HIPCHK(ctx, hipMemsetAsync(V, 0, es * n * (m + 1), ctx->stream));
+  if (!ctx->kry_R64 && (r = dalloc(ctx, &ctx->kry_R64, n))) return r;
+  if (!ctx->kry_part && (r = dalloc(ctx, &ctx->kry_part, (size_t)64 * DOT_BLOCKS))) return r;
+  if (!ctx->kry_H1) {
+    if ((r = dalloc(ctx, &ctx->kry_H1, 65 * 64))) return r;
+    if ((r = dalloc(ctx, &ctx->kry_H2, 65 * 64))) return r;
+    if ((r = dalloc(ctx, &ctx->kry_Hs, 65))) return r;
+    if ((r = dalloc(ctx, &ctx->kry_y, 65))) return r;
+    if ((r = dalloc(ctx, &ctx->kry_nrm, 4))) return r;
+  }
+  ctx->kry_m = m;
+  ctx->kry_f32 = f32;
+  return UFE_OK;
+}
+
+// One GMRES(m) cycle on A d = r (r held in R32 or kry_R64 with |r|^2 in d_scalar[1]),
+// then w += d in FP64.  Templated on the Krylov precision.
+extern "C++" template <typename T>
+static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passes, int *sweeps, int *matvecs) {
+  using T2 = typename V2T<T>::t;
+  const bool f32 = sizeof(T) == 4;
+  const int n = ctx->nA;
+  const AaAcMesh am = aaac_mesh(ctx);
+  const ActiveRanges ar = active_ranges(ctx);
+  T2 *V = (T2 *)ctx->kry_V, *Z = (T2 *)ctx->kry_Z, *w = (T2 *)ctx->kry_w, *x = (T2 *)ctx->kry_x;
+  const int nb = nblk(n);
+  constexpr int KMAX = 64;
+  if (f32)
+    k_r32_to_v<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_R32, ctx->d_scalar + 1, (float2 *)V);
+  else
+    k_r32_to_v64<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_R32, ctx->d_scalar + 1, (double2 *)V);
+  if (!f32) {  // exact FP64 residual as v0 for the FP64 Krylov path
+    k_residual_store64<<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2], ar,
+                                                       ctx->kry_R64);
+    k_normalize<double><<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->kry_R64, ctx->d_scalar + 1, (double2 *)V, ctx->kry_nrm + 1);
+  }
+  for (int j = 0; j < m; j++) {
+    T2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;
+    k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
+    for (int k = 0; k < ksweeps; k++) {
+      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1);
+      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1);
+    }
+    *sweeps += ksweeps;
+    if (f32)
+      k_matvec<float><<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32, ar, (const float2 *)zj, (float2 *)w);
+    else
+      k_matvec<double><<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_cU, ctx->d_Sxy, ctx->d_pair[1], ar, (const double2 *)zj, (double2 *)w);
+    (*matvecs)++;
+    // classical Gram-Schmidt, one or two passes
+    double *h1 = ctx->kry_H1 + (int64_t)j * 65, *h2 = ctx->kry_H2 + (int64_t)j * 65;
+    k_multi_dot<T><<<DOT_BLOCKS, BLOCK, 0, ctx->stream>>>(n, j + 1, ar, V, w, ctx->kry_part);
+    k_multi_dot_reduce<<<j + 1, 256, 0, ctx->stream>>>(DOT_BLOCKS, j + 1, ctx->kry_part, h1, 0);
+    k_multi_axpy<T><<<nb, BLOCK, 0, ctx->stream>>>(n, j + 1, V, h1, w);
+    if (passes > 1) {
+      k_multi_dot<T><<<DOT_BLOCKS, BLOCK, 0, ctx->stream>>>(n, j + 1, ar, V, w, ctx->kry_part);
+      k_multi_dot_reduce<<<j + 1, 256, 0, ctx->stream>>>(DOT_BLOCKS, j + 1, ctx->kry_part, h2, 0);
+      k_multi_axpy<T><<<nb, BLOCK, 0, ctx->stream>>>(n, j + 1, V, h2, w);
+    } else {
+      k_fill<double><<<1, 64, 0, ctx->stream>>>(h2, j + 1, 0.0);
+    }
+    k_multi_dot<T><<<DOT_BLOCKS, BLOCK, 0, ctx->stream>>>(n, 1, ar, w, w, ctx->kry_part);
+    k_multi_dot_reduce<<<1, 256, 0, ctx->stream>>>(DOT_BLOCKS, 1, ctx->kry_part, ctx->kry_nrm, 0);
+    k_normalize<T><<<nb, BLOCK, 0, ctx->stream>>>(n, w, ctx->kry_nrm, V + (int64_t)(j + 1) * n, ctx->kry_Hs + j);
+  }
+  LAUNCHCHK(ctx);
+  // least squares min |beta e1 - H y| on the host (Givens rotations)
+  std::vector<double> H1(65 * 64), H2(65 * 64), Hs(65), sc(4);
+  HIPCHK(ctx, hipMemcpyAsync(H1.data(), ctx->kry_H1, sizeof(double) * 65 * m, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(H2.data(), ctx->kry_H2, sizeof(double) * 65 * m, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(Hs.data(), ctx->kry_Hs, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(sc.data(), ctx->d_scalar, sizeof(double) * 3, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  prof_collect(ctx);
+  const double beta = std::sqrt(sc[1]);
+  // H is (m+1) x m, column j: H(i,j) = H1 + H2 for i <= j, H(j+1,j) = Hs[j]
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), g(m + 1, 0.0), cs(m), sn(m);
+  auto Hat = [&](int i, int j) -> double & { return H[(size_t)j * (m + 1) + i]; };
+  for (int j = 0; j < m; j++) {
+    for (int i = 0; i <= j; i++) Hat(i, j) = H1[(size_t)j * 65 + i] + H2[(size_t)j * 65 + i];
+    Hat(j + 1, j) = Hs[j];
+  }
+  g[0] = beta;
+  int kk = m;
+  for (int j = 0; j < m; j++) {
+    for (int i = 0; i < j; i++) {
+      const double a = cs[i] * Hat(i, j) + sn[i] * Hat(i + 1, j);
+      const double b = -sn[i] * Hat(i, j) + cs[i] * Hat(i + 1, j);
+      Hat(i, j) = a;
+      Hat(i + 1, j) = b;
+    }
+    const double a = Hat(j, j), b = Hat(j + 1, j);
+    const double rr = std::hypot(a, b);
+    if (!(rr > 0.0) || !std::isfinite(rr)) { kk = j; break; }
+    cs[j] = a / rr;
+    sn[j] = b / rr;
+    Hat(j, j) = rr;
+    Hat(j + 1, j) = 0.0;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+  }
+  std::vector<double> y(std::max(kk, 1), 0.0);
+  for (int i = kk - 1; i >= 0; i--) {
+    double t = g[i];
+    for (int l = i + 1; l < kk; l++) t -= Hat(i, l) * y[l];
+    y[i] = t / Hat(i, i);
+  }
+  if (kk == 0) return UFE_OK;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->kry_y, y.data(), sizeof(double) * kk, hipMemcpyHostToDevice, ctx->stream));
+  k_lincomb<T><<<nb, BLOCK, 0, ctx->stream>>>(n, kk, Z, ctx->kry_y, x);
+  k_add_to_w<T><<<nb, BLOCK, 0, ctx->stream>>>(n, x, ctx->d_pair[0]);
+  LAUNCHCHK(ctx);
+  return UFE_OK;
+}
+
 int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_stats *st) {
   if (!ctx || !ctx->have_mesh || !st || !o) return UFE_BAD_INPUT;
   std::memset(st, 0, sizeof(*st));
@@ -1189,6 +1385,32 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
   if (mixed) {  // FP32 centre coefficients of the current system
     k_to_f32_pair<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_pair[1], ctx->d_EV32);
     LAUNCHCHK(ctx);
+  }
+  const int mr = o->gmres_restart;
+  if (mr < 0 || mr > 64) {
+    ctx->err = "gmres_restart must be in 0..64";
+    return UFE_BAD_INPUT;
+  }
+  if (mr > 0) {
+    if ((r = krylov_alloc(ctx, mr, mixed))) return r;
+    const int ks = o->precond_sweeps > 0 ? o->precond_sweeps : 2;
+    if ((r = residual_fp64(ctx, &rel, true))) return r;
+    st->refinements = 1;
+    while (rel > o->rel_tol && st->sweeps < o->max_sweeps) {
+      int mv = 0;
+      const int passes = o->orth_passes > 0 ? o->orth_passes : 2;
+      r = mixed ? gmres_cycle<float>(ctx, mr, ks, omega, passes, &st->sweeps, &mv)
+                : gmres_cycle<double>(ctx, mr, ks, omega, passes, &st->sweeps, &mv);
+      if (r) return r;
+      st->matvecs += mv;
+      if ((r = residual_fp64(ctx, &rel, true))) return r;
+      st->refinements++;
+      if (!std::isfinite(rel)) break;
+    }
+    st->rel_residual = rel;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    st->status = rel <= o->rel_tol ? UFE_OK : UFE_SOR_NOT_CONVERGED;
+    return UFE_OK;
   }
   if ((r = residual_fp64(ctx, &rel, mixed))) return r;
   st->refinements = 1;
