@@ -273,3 +273,23 @@ def time_sor_sweeps(om, fields, p: DynParams, sweeps, threads=None):
                                    ct.byref(lm), ct.byref(ns))
     dt = time.perf_counter() - t0
     return dt, ns.value
+
+
+def colour_graph(adj, nC_mem=16):
+    """Run the oracle's five-colouring on an arbitrary graph given as 1-based
+    adjacency lists (one list per point); returns (status, colour array)."""
+    n = len(adj)
+    s = _Mesh()
+    s.d.nV, s.d.nAc, s.d.nVAaAc, s.d.nC_mem = n, 0, n, nC_mem
+    nC = np.array([len(a) for a in adj], np.int32)
+    C = np.zeros((n, nC_mem), np.int32, order="F")
+    for i, a in enumerate(adj):
+        C[i, :len(a)] = a
+    colour = np.zeros(n, np.int32)
+    colour_vi = np.zeros((n, 5), np.int32, order="F")
+    keep = {"nCAaAc": nC, "CAaAc": C, "colour": colour, "colour_vi": colour_vi}
+    for k, name in enumerate(_names("om_mesh_field_name_i", N_MESH_I)):
+        if name in keep:
+            s.i[k] = keep[name].ctypes.data
+    st = load().om_calculate_five_colouring_AaAc(ct.byref(s))
+    return st, colour

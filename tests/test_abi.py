@@ -1,0 +1,72 @@
+"""The C-ABI library loads on the host and exports every declared symbol.
+(No compute calls here: those need a GPU and live in the gpu-marked tests.)"""
+import ctypes as ct
+import os
+import re
+
+from ufemism_amd import abi, fields
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "ufe_dyn.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ufe_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declarations_are_exported():
+    lib = abi.load()
+    names = declared_functions()
+    assert len(names) >= 40
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(abi.EXPORTS)
+
+
+def test_field_table_matches_reference_names():
+    table = abi.field_table()
+    for name, (grid, dtype) in fields.ICE_FIELDS.items():
+        if name in ("LHSx_AaAc", "LHSy_AaAc", "resU_AaAc", "resV_AaAc"):
+            continue   # SOR intermediates stay in registers on the device
+        assert name in table, name
+        code = {"Aa": 0, "Aaz": 1, "Aac": 2, "Ac": 3, "Acz": 4, "AaAc": 5}[grid]
+        assert table[name][1] == code, name
+        assert table[name][2] == (0 if dtype.__name__ == "float64" else 1), name
+
+
+STRUCTS = {"ufe_mesh_desc": abi.MeshDesc, "ufe_params": abi.Params, "ufe_ssa_stats": abi.SSAStats,
+           "ufe_linsolve_opts": abi.LinSolveOpts, "ufe_linsolve_stats": abi.LinSolveStats,
+           "ufe_kernel_profile": abi.KernelProfile}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Compile a probe against include/ufe_dyn.h and compare every offset."""
+    import subprocess
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ufe_dyn.h"', "int main(void){"]
+    for cname, py in STRUCTS.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines.append("return 0;}")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    for line in filter(None, out):
+        cname, field, val = line.split()
+        py = STRUCTS[cname]
+        if field == "size":
+            assert ct.sizeof(py) == int(val), cname
+        else:
+            assert getattr(py, field).offset == int(val), (cname, field)
+
+
+def test_version_and_no_device_context():
+    lib = abi.load()
+    assert lib.ufe_version() >= 1
+    h = ct.c_void_p()
+    # creating a context needs a device; without one it reports an error
+    r = lib.ufe_ctx_create(99, ct.byref(h))
+    assert r != 0 and not h.value

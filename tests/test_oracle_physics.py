@@ -1,0 +1,266 @@
+"""The CPU oracle's physics, pinned by closed forms, a direct sparse solve,
+conservation laws and the reference's failure paths."""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+from ufemism_amd import geometry, meshgen
+from ufemism_amd.config import DynParams, tau_c_from_beta
+from ufemism_amd.fields import ZETA_DEFAULT
+
+po = pytest.importorskip("oracle.pyoracle")
+
+RHO_G = 910.0 * 9.81
+
+
+def _vert(p, f, name):
+    lib = po.load()
+    ps = po.params_struct(p)
+    f = np.ascontiguousarray(f, float)
+    if name == "average":
+        return lib.om_vertical_average(ct.byref(ps), f.ctypes.data_as(ct.c_void_p))
+    out = np.zeros_like(f)
+    lib.om_vertical_integrate(ct.byref(ps), f.ctypes.data_as(ct.c_void_p), out.ctypes.data_as(ct.c_void_p))
+    return out
+
+
+def test_vertical_integrals_trapezoid():
+    p = DynParams()
+    z = ZETA_DEFAULT
+    f = 2.0 + 3.0 * z                               # trapezoid is exact on linear f
+    assert np.isclose(_vert(p, f, "average"), 3.5, rtol=1e-15)
+    I = _vert(p, f, "integrate")
+    exact = -((2.0 * (1 - z)) + 1.5 * (1 - z ** 2))  # -int_z^1 f
+    assert np.allclose(I, exact, rtol=1e-14, atol=1e-15)
+    # I_zeta = average(integrate(zeta^3)) tends to -1/(n+2) = -0.2 on fine grids
+    fine = DynParams(zeta=np.linspace(0, 1, 32))
+    Iz = _vert(fine, _vert(fine, fine.zeta ** 3, "integrate"), "average")
+    assert abs(Iz + 0.2) < 1e-3
+
+
+@pytest.fixture(scope="module")
+def dome_state():
+    wl = geometry.dome(3000, seed=21, picard=3)
+    om = po.OracleMesh(wl.mesh).build()
+    ice = po.OracleIce(om)
+    ice.set(Hi=wl.Hi, Hb=wl.Hb, SL=wl.SL, A_flow=wl.A_flow, A_flow_Ac=wl.A_flow,
+            A_flow_mean=wl.A_flow, A_flow_mean_Ac=wl.A_flow)
+    ice["tau_c_override_AaAc"][:] = tau_c_from_beta(wl.beta)
+    ice.update_general_ice_model_data()
+    return wl, om, ice
+
+
+def test_sia_closed_form_on_the_dome(dome_state):
+    """With uniform A and m_enh = 1 the SIA diffusivity separates:
+    D3D(zeta) = 2 A (rho g)^n H^(n+1) |grad s|^(n-1) J(zeta), J = -int_zeta^1 z^n dz,
+    cut off at -1e5, and D_SIA = H * (trapezoid mean of D3D)."""
+    wl, om, ice = dome_state
+    ice.solve_SIA(wl.params)
+    z = ZETA_DEFAULT
+    cum = np.concatenate([[0.0], np.cumsum(0.5 * (z[1:] ** 3 + z[:-1] ** 3) * np.diff(z))])
+    J = -(cum[-1] - cum)
+    H = ice["Hi_Ac"]
+    g2 = ice["dHs_dp_Ac"] ** 2 + ice["dHs_do_Ac"] ** 2
+    D3 = 2 * wl.A_flow * RHO_G ** 3 * H[:, None] ** 4 * g2[:, None] * J[None, :]
+    D3 = np.maximum(D3, -1e5)
+    D = H * np.sum(0.5 * (D3[:, 1:] + D3[:, :-1]) * np.diff(z)[None, :], axis=1)
+    sheet = ice["mask_sheet_Ac"] == 1
+    assert sheet.sum() > 100
+    assert np.allclose(ice["D_SIA_Ac"][sheet], D[sheet], rtol=1e-10, atol=1e-12)
+    assert np.all(ice["D_SIA_Ac"][~sheet] == 0.0)
+    # closed form without the cut-off, where it is inactive: I_zeta of the grid
+    Iz = np.sum(0.5 * (J[1:] + J[:-1]) * np.diff(z))
+    free = sheet & (D3.min(axis=1) > -1e5)
+    assert np.allclose(ice["D_SIA_Ac"][free],
+                       2 * wl.A_flow * RHO_G ** 3 * H[free] ** 5 * g2[free] * Iz, rtol=1e-10)
+    # flux runs downhill: U_p = D2D * ds/dp with D2D <= 0
+    assert np.all(ice["Up_SIA_Ac"][sheet] * ice["dHs_dp_Ac"][sheet] <= 0)
+
+
+def test_yield_stress_closed_forms(dome_state):
+    wl, om, ice = dome_state
+    lib = po.load()
+    Hb = ice["Hb"]
+    for hb, phi in ((0.0, 20.0), (-1000.0, 5.0), (-500.0, 12.5), (1000.0, 20.0)):
+        ice["Hb"][:] = hb
+        ice["Hb_Ac"][:] = hb
+        lib.om_basal_yield_stress(ct.byref(om.s), ct.byref(ice.s))
+        H = np.maximum(0.1, ice["Hi"])
+        lam = min(1.0, max(0.0, 1.0 - hb / 1000.0))
+        tau = np.tan(np.pi / 180 * phi) * (RHO_G * H - 0.96 * RHO_G * H * lam)
+        n = om.mesh.nV
+        assert np.allclose(ice["phi_fric_AaAc"][:n], phi, rtol=1e-14)
+        assert np.allclose(ice["tau_c_AaAc"][:n], tau, rtol=1e-12)
+    ice["Hb"][:] = Hb
+    ice["Hb_Ac"][:] = 0.0
+
+
+def test_viscosity_closed_form_for_linear_flow(dome_state):
+    wl, om, ice = dome_state
+    P = om["VAaAc"]
+    a, b, c, d = 2e-3, -1e-3, 5e-4, -3e-3
+    ice["U_SSA_AaAc"][:] = a * P[:, 0] + b * P[:, 1]
+    ice["V_SSA_AaAc"][:] = c * P[:, 0] + d * P[:, 1]
+    ice["A_flow_mean_AaAc"][:] = 1e-16
+    ice["Hi_AaAc"][:] = 500.0
+    p = DynParams(m_enh_ssa=0.7)
+    po.load().om_SSA_effective_viscosity(ct.byref(om.s), ct.byref(ice.s), ct.byref(po.params_struct(p)))
+    eps2 = a * a + d * d + a * d + 0.25 * (b + c) ** 2 + 1e-12
+    eta = (0.7 * 0.5 * 1e-16) ** (-1 / 3) * eps2 ** (-1 / 3)
+    ok = np.isfinite(ice["eta_AaAc"])
+    assert np.allclose(ice["eta_AaAc"][ok], eta, rtol=1e-8)
+    assert np.allclose(ice["N_AaAc"][ok], eta * 500.0, rtol=1e-8)
+
+
+def test_sliding_closed_form(dome_state):
+    wl, om, ice = dome_state
+    ice["tau_c_AaAc"][:] = 7.0e4
+    ice["U_SSA_AaAc"][:] = 0.0
+    ice["V_SSA_AaAc"][:] = 0.0
+    po.load().om_SSA_sliding_term(ct.byref(om.s), ct.byref(ice.s))
+    # (1e-6)^(-0.35) / 100^0.3 = 10^2.1 / 10^0.6 = 10^1.5
+    assert np.allclose(ice["S_AaAc"], 7.0e4 * 10 ** 1.5, rtol=1e-12)
+    ice["U_SSA_AaAc"][:] = 300.0
+    ice["V_SSA_AaAc"][:] = 400.0
+    po.load().om_SSA_sliding_term(ct.byref(om.s), ct.byref(ice.s))
+    assert np.allclose(ice["S_AaAc"], 7.0e4 * (1e-6 + 500.0 ** 2) ** (-0.35) / 100 ** 0.3, rtol=1e-12)
+
+
+def _small_ssa():
+    wl = geometry.dome(600, seed=5, picard=1)
+    om = po.OracleMesh(wl.mesh).build()
+    ice = po.OracleIce(om)
+    ice.set(Hi=wl.Hi, Hb=wl.Hb, SL=wl.SL, A_flow=wl.A_flow, A_flow_Ac=wl.A_flow,
+            A_flow_mean=wl.A_flow, A_flow_mean_Ac=wl.A_flow)
+    ice["tau_c_override_AaAc"][:] = tau_c_from_beta(wl.beta)
+    ice.update_general_ice_model_data()
+    return wl, om, ice
+
+
+def test_sor_fixed_point_is_the_direct_solution():
+    """SOR run to a tiny tolerance reaches the solution of the assembled
+    linear system (SSA stencil on SOR points, Neumann rows on edge points)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    wl, om, ice = _small_ssa()
+    p = wl.params
+    p.SSA_max_outer_loops, p.SSA_max_residual_UV, p.SSA_max_inner_loops = 1, 1e-11, 200000
+    st = ice.solve_SSA(p)
+    assert st.status == 0 and st.sweeps[0] < 200000
+    n, nV = om.nVAaAc, om.mesh.nV
+    nc, C = om["nCAaAc"], om["CAaAc"] - 1
+    edge = np.concatenate([om.mesh.edge_index > 0, om["edge_index_Ac"] > 0])
+    I, J, X = [], [], []
+    bu, bv = np.zeros(n), np.zeros(n)
+    for i in range(n):
+        k = nc[i]
+        if i < 4 or edge[i]:
+            nb = [j for j in C[i, :k] if (i < 4 or not edge[j])]
+            for comp in (0, 1):
+                I.append(comp * n + i); J.append(comp * n + i); X.append(1.0)
+                for j in nb:
+                    I.append(comp * n + i); J.append(comp * n + j); X.append(-1.0 / len(nb))
+            continue
+        sxy = om["Nxy_AaAc"][i, k] + om["Nxy_AaAc"][i, :k].sum()
+        for c in range(k):
+            j = C[i, c]
+            I += [i, n + i]; J += [j, n + j]
+            X += [4 * om["Nxx_AaAc"][i, c] + om["Nyy_AaAc"][i, c], 4 * om["Nyy_AaAc"][i, c] + om["Nxx_AaAc"][i, c]]
+        I += [i, i, n + i, n + i]; J += [i, n + i, n + i, i]
+        X += [ice["eu_i_AaAc"][i], 3 * sxy, ice["ev_i_AaAc"][i], 3 * sxy]
+        bu[i], bv[i] = ice["RHSx_AaAc"][i], ice["RHSy_AaAc"][i]
+    A = sp.csr_matrix((X, (I, J)), shape=(2 * n, 2 * n))
+    x = spl.spsolve(A.tocsc(), np.concatenate([bu, bv]))
+    u, v = ice["U_SSA_AaAc"], ice["V_SSA_AaAc"]
+    ref = np.sqrt(np.sum(x ** 2))
+    err = np.sqrt(np.sum((x[:n] - u) ** 2 + (x[n:] - v) ** 2)) / ref
+    assert err < 1e-8, err
+
+
+def test_first_rn_is_one_and_failure_paths():
+    wl, om, ice = _small_ssa()
+    p = wl.params
+    p.SSA_max_outer_loops = 1
+    st = ice.solve_SSA(p)
+    assert st.last_RN == 1.0                         # N starts at zero
+    # max inner loops reached: warning status, not an error
+    wl, om, ice = _small_ssa()
+    p = wl.params
+    p.SSA_max_outer_loops, p.SSA_max_inner_loops, p.SSA_max_residual_UV = 2, 1, 1e-12
+    st = ice.solve_SSA(p)
+    assert st.status == 1 and st.sweeps[0] == 1
+    # divergent over-relaxation: the velocities are reset, and a second reset aborts
+    wl, om, ice = _small_ssa()
+    p = wl.params
+    p.SSA_SOR_omega, p.SSA_max_outer_loops = 2.6, 5
+    st = ice.solve_SSA(p)
+    assert st.resets == 2 and st.status == 3
+    # no grounded ice: velocities set to zero
+    wl, om, ice = _small_ssa()
+    ice["mask_sheet"][:] = 0
+    ice["U_SSA"][:] = 5.0
+    st = ice.solve_SSA(wl.params)
+    assert st.picard_iterations == 0 and np.all(ice["U_SSA"] == 0.0)
+
+
+def _mc_state(n=1500, seed=9):
+    m = meshgen.generate_mesh(geometry.dome_resolution(), n, seed=seed)
+    om = po.OracleMesh(m).build(colour=False)
+    ice = po.OracleIce(om)
+    return m, om, ice
+
+
+def test_mass_conservation_and_positivity():
+    m, om, ice = _mc_state()
+    rng = np.random.default_rng(1)
+    H = 500 + 500 * rng.random(m.nV)
+    ice["Hi"][:] = H
+    ice["Up_SIA_Ac"][:] = rng.standard_normal(om.nAc) * 50.0
+    ice["Up_SSA_Ac"][:] = rng.standard_normal(om.nAc) * 50.0
+    SMB = 0.3 * np.ones(m.nV)
+    p = DynParams(zero_Hi_at_edges=False)
+    ice.calculate_ice_thickness_change(SMB, np.zeros(m.nV), 2.0, np.zeros(m.nV, np.int32), p)
+    before = np.sum(m.A * H) + np.sum(SMB * m.A * 2.0)
+    assert np.isclose(np.sum(m.A * ice["Hi"]), before, rtol=1e-12)
+    # strong outflow and melt: the limiter keeps H >= 0 and volume is still conserved
+    m, om, ice = _mc_state(seed=10)
+    H = 50 * rng.random(m.nV)
+    ice["Hi"][:] = H
+    ice["Up_SIA_Ac"][:] = rng.standard_normal(om.nAc) * 5e4
+    ice["Up_SSA_Ac"][:] = 0.0
+    melt = -10 * rng.random(m.nV)
+    ice.calculate_ice_thickness_change(melt, np.zeros(m.nV), 1.0, np.zeros(m.nV, np.int32), p)
+    Hn = ice["Hi"]
+    assert Hn.min() > -1e-9 * H.max()
+    removed = np.maximum(melt * m.A, -m.A * H)       # melt limited by available ice
+    assert np.isclose(np.sum(m.A * Hn), np.sum(m.A * H) + np.sum(removed), rtol=1e-9)
+
+
+def test_uniform_translation_leaves_uniform_ice_unchanged():
+    m, om, ice = _mc_state(seed=11)
+    ice["Hi"][:] = 1000.0
+    A = om["Aci"] - 1
+    d = m.V[A[:, 1]] - m.V[A[:, 0]]
+    L = np.hypot(d[:, 0], d[:, 1])
+    U = np.array([120.0, -45.0])
+    ice["Up_SIA_Ac"][:] = (U[0] * d[:, 0] + U[1] * d[:, 1]) / L
+    ice["Up_SSA_Ac"][:] = 0.0
+    p = DynParams(zero_Hi_at_edges=False)
+    ice.calculate_ice_thickness_change(np.zeros(m.nV), np.zeros(m.nV), 1.0, np.zeros(m.nV, np.int32), p)
+    inner = m.edge_index == 0
+    scale = 1000.0 * np.hypot(*U) * np.sqrt(m.A)     # flux magnitude per unit area scale
+    assert np.all(np.abs(ice["dHi_dt"][inner]) * m.A[inner] <= 1e-8 * scale[inner] * np.sqrt(m.A[inner]))
+
+
+def test_zero_timestep_and_edges():
+    m, om, ice = _mc_state(seed=12)
+    ice["Hi"][:] = 100.0
+    ice["Up_SIA_Ac"][:] = 10.0
+    ice.calculate_ice_thickness_change(np.ones(m.nV), np.zeros(m.nV), 0.0, np.zeros(m.nV, np.int32), DynParams())
+    assert np.all(ice["dHi_dt"] == 0.0)
+    assert np.all(ice["Hi"][m.edge_index > 0] == 0.0) and np.all(ice["Hi"][m.edge_index == 0] == 100.0)
+    noice = (np.arange(m.nV) % 7 == 0).astype(np.int32)
+    ice["Hi"][:] = 100.0
+    ice.calculate_ice_thickness_change(np.zeros(m.nV), np.zeros(m.nV), 1.0, noice, DynParams())
+    assert np.all(ice["Hi"][noice == 1] == 0.0)

@@ -402,7 +402,11 @@ def generate_mesh(res, n_target, seed=0, xmin=-800e3, xmax=800e3, ymin=-800e3,
     """
     from scipy.spatial import Delaunay
 
-    scale = 1.0
+    # first guess from the point density of a triangular lattice, 2/(sqrt(3) h^2)
+    gx, gy = np.meshgrid(np.linspace(xmin, xmax, 129), np.linspace(ymin, ymax, 129))
+    r = res(gx.ravel(), gy.ravel())
+    dens = np.mean(2.0 / (math.sqrt(3.0) * r ** 2)) * (xmax - xmin) * (ymax - ymin)
+    scale = math.sqrt(dens / n_target)
     for _ in range(max_iter):
         rng = np.random.default_rng(seed)
         X, Y, bx, by, be = _place_points(res, xmin, xmax, ymin, ymax, scale, rng, jitter)
