@@ -39,5 +39,27 @@ def build_library(verbose=False) -> str:
     return LIB
 
 
+FORTRAN_SOURCES = ["ufe_c_binding.f90", "ice_dynamics_module.f90", "ufe_step_driver.f90"]
+FORTRAN_EXE = os.path.join(LIBDIR, "ufe_step_driver")
+
+
+def build_fortran_host(verbose=False) -> str:
+    """The Fortran host (ice_dynamics_module over ISO_C_BINDING) and its
+    one-step driver, linked against libufe_dyn.so (needs flang)."""
+    fdir = os.path.join(HERE, "fortran")
+    mod = os.path.join(LIBDIR, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    flang = os.environ.get("FC", "flang")
+    cmd = ([flang, "-O2", "-module-dir", mod] + [os.path.join(fdir, s) for s in FORTRAN_SOURCES]
+           + ["-o", FORTRAN_EXE, "-L" + LIBDIR, "-lufe_dyn", "-Wl,-rpath,$ORIGIN"])
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
+    if verbose and (r.stdout + r.stderr).strip():
+        print(r.stdout + r.stderr)
+    return FORTRAN_EXE
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_fortran_host(verbose=True))
