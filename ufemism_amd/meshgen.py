@@ -394,7 +394,7 @@ def _finish_geometry(mesh: Mesh):
 
 
 def generate_mesh(res, n_target, seed=0, xmin=-800e3, xmax=800e3, ymin=-800e3,
-                  ymax=800e3, jitter=0.15, nC_mem=NC_MEM, max_iter=3) -> Mesh:
+                  ymax=800e3, jitter=0.15, nC_mem=NC_MEM, max_iter=3, smooth=2) -> Mesh:
     """Seeded mesh with about ``n_target`` vertices following ``res(x, y)``.
 
     ``res`` gives the relative target edge length [m]; it is rescaled
@@ -424,6 +424,18 @@ def generate_mesh(res, n_target, seed=0, xmin=-800e3, xmax=800e3, ymin=-800e3,
     P = np.concatenate([corners, rest[o]])
     edge_index = np.concatenate([np.array([6, 4, 2, 8], np.int32), rest_edge[o]])
     tri = Delaunay(P)
+    # Laplacian smoothing of the interior vertices (the reference smooths its
+    # meshes with Lloyd iterations, src/mesh_creation_module.f90:743-792):
+    # skewed triangles give second-derivative stencils whose centre weight has
+    # the wrong sign, and over-relaxed SOR diverges on those rows.
+    interior = edge_index == 0
+    for _ in range(smooth):
+        ptr, nbr = tri.vertex_neighbor_vertices
+        cnt = np.diff(ptr)
+        mean = np.stack([np.add.reduceat(P[nbr, 0], ptr[:-1]), np.add.reduceat(P[nbr, 1], ptr[:-1])], 1)
+        mean /= cnt[:, None]
+        P = np.where(interior[:, None], P + 0.6 * (mean - P), P)
+        tri = Delaunay(P)
     if len(tri.coplanar):
         raise ValueError("triangulation dropped points")
     tris, nC, C, niTri, iTri, TriC = _build_topology(
