@@ -203,6 +203,49 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *opts, ufe_linsol
 /* FP64 relative residual of the current velocities. */
 int ufe_ssa_residual(ufe_ctx *ctx, double *rel_residual);
 
+/* ---- adjoint of the SSA solve ------------------------------------------- */
+/* No counterpart in the reference.  The forward runs exactly
+ * picard_iterations Picard steps of solve_SSA (src/ice_dynamics_module.f90:
+ * 498-545: viscosity :695-726, sliding :727-779, linearised system :558-694)
+ * from the current velocities, solving each linearised system to the FP64
+ * relative residual lin_rel_tol (GMRES-IR, as ufe_ssa_linear_solve) instead
+ * of the reference's SOR stopping rule, and stores every checkpoint_every-th
+ * Picard iterate.  The backward takes dJ/dU and dJ/dV of the final AaAc
+ * velocities and returns dJ/dlog(tau_c) and dJ/dA on the AaAc points, the
+ * exact gradient of the K-step map with exact linear solves: per step one
+ * transposed solve (FP64 GMRES, transposed-SOR preconditioner) and the chain
+ * through the sliding term, the right-hand side and the effective viscosity.
+ * Iterates between checkpoints are recomputed in the reverse sweep.
+ * All arrays of the backward are device pointers of nVAaAc doubles in the
+ * reference AaAc order (Aa points 1..nV, then Ac points). */
+typedef struct ufe_adjoint_opts {
+  double lin_rel_tol;               /* relative residual of every forward and adjoint linear solve */
+  int32_t picard_iterations;        /* K >= 1 Picard steps (no early exit) */
+  int32_t checkpoint_every;         /* store u_0, u_c, u_2c, ...; others recomputed (>= 1) */
+  int32_t precision;                /* forward linear solves: UFE_FP64 or UFE_MIXED; adjoint solves FP64 */
+  int32_t gmres_restart;            /* GMRES(m), 1..64, forward and adjoint */
+  int32_t precond_sweeps;           /* SOR sweeps per preconditioner application */
+  double omega;                     /* <= 0: params.SSA_SOR_omega */
+  int32_t max_sweeps;               /* per linear solve */
+} ufe_adjoint_opts;
+
+typedef struct ufe_adjoint_stats {
+  int32_t picard_iterations;
+  int32_t checkpoints;              /* iterates stored */
+  int32_t forward_sweeps;           /* SOR sweeps of the forward linear solves */
+  int32_t recompute_sweeps;         /* SOR sweeps spent recomputing iterates in the backward */
+  int32_t adjoint_sweeps;           /* transposed SOR sweeps of the adjoint solves */
+  int32_t adjoint_matvecs;
+  double max_forward_residual;      /* worst final relative residual of the forward solves */
+  double max_adjoint_residual;      /* worst final relative residual of the adjoint solves */
+  double forward_seconds, backward_seconds;
+  int32_t status;
+} ufe_adjoint_stats;
+
+int ufe_ssa_adjoint_forward(ufe_ctx *ctx, const ufe_adjoint_opts *opts, ufe_adjoint_stats *st);
+int ufe_ssa_adjoint_backward(ufe_ctx *ctx, const double *dJ_dU, const double *dJ_dV,
+                             double *grad_log_tau_c, double *grad_A, ufe_adjoint_stats *st);
+
 /* ---- operators (device pointers, original index order) ----------------- */
 /* src/mesh_derivatives_module.f90:315-432, 485-597 */
 int ufe_op_derivatives_Aa(ufe_ctx *ctx, const double *d, double *ddx, double *ddy);

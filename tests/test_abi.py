@@ -37,7 +37,8 @@ def test_field_table_matches_reference_names():
 
 STRUCTS = {"ufe_mesh_desc": abi.MeshDesc, "ufe_params": abi.Params, "ufe_ssa_stats": abi.SSAStats,
            "ufe_linsolve_opts": abi.LinSolveOpts, "ufe_linsolve_stats": abi.LinSolveStats,
-           "ufe_kernel_profile": abi.KernelProfile}
+           "ufe_kernel_profile": abi.KernelProfile,
+           "ufe_adjoint_opts": abi.AdjointOpts, "ufe_adjoint_stats": abi.AdjointStats}
 
 
 def test_struct_layouts_match_header(tmp_path):

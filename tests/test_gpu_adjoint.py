@@ -1,0 +1,107 @@
+"""The hand-written SSA adjoint against central finite differences.
+
+J = 1/2 |u - u_obs|^2 over the AaAc velocities after K Picard steps; u_obs
+comes from the forward with a perturbed friction field.  The adjoint gradient
+with respect to log(tau_c) must match central differences (step 1e-6, FP64)
+along 32 seeded random directions, and the gradient with respect to A along a
+few more.  Checkpointing every c-th Picard iterate must give the same gradient
+bit for bit as storing all of them.
+"""
+import numpy as np
+import pytest
+
+from ufemism_amd import abi, geometry
+from ufemism_amd.dynamics import Step
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def problem(ctx):
+    from ufemism_amd.adjoint import AdjointOptions, SSAProblem
+    print("[adjoint] set-up marine 20k", flush=True)
+    wl = geometry.marine(20000, seed=5, picard=5)
+    step = Step(ctx, wl)
+    ctx.update_general_ice_model_data()
+    prob = SSAProblem(ctx, AdjointOptions(picard_iterations=5, lin_rel_tol=1e-12))
+    dev = prob.dev
+    tau0 = torch.tensor(wl.tau_c_override(step.ma["VAaAc"]), dtype=torch.float64, device=dev)
+    log_tau0 = torch.log(tau0)
+    A0 = torch.full((prob.nA,), wl.A_flow, dtype=torch.float64, device=dev)
+    g = torch.tensor(geometry.gaussian_field(step.ma["VAaAc"], 50e3, 4242), dtype=torch.float64, device=dev)
+    U_obs, V_obs = prob.forward(log_tau0 + 0.3 * g, A0)
+    return prob, log_tau0, A0, U_obs, V_obs
+
+
+def _grad(prob, log_tau, A, U_obs, V_obs):
+    from ufemism_amd.adjoint import ssa_velocities
+    lt = log_tau.clone().requires_grad_(True)
+    a = A.clone().requires_grad_(True)
+    U, V = ssa_velocities(lt, a, prob)
+    J = 0.5 * (((U - U_obs) ** 2).sum() + ((V - V_obs) ** 2).sum())
+    J.backward()
+    return lt.grad, a.grad, U.detach(), V.detach()
+
+
+def _dJ(prob, log_tau, A, U_obs, V_obs, plus, minus):
+    """J(+) - J(-) as 1/2 sum (u+ - u-)(u+ + u- - 2 u_obs), without cancellation."""
+    Up, Vp = prob.forward(*plus)
+    Um, Vm = prob.forward(*minus)
+    return 0.5 * float(((Up - Um) * (Up + Um - 2 * U_obs)).sum() + ((Vp - Vm) * (Vp + Vm - 2 * V_obs)).sum())
+
+
+def test_adjoint_matches_central_differences(problem):
+    prob, log_tau0, A0, U_obs, V_obs = problem
+    g_tau, g_A, U, V = _grad(prob, log_tau0, A0, U_obs, V_obs)
+    fs, bs = prob.forward_stats, prob.backward_stats
+    print(f"  forward {fs.forward_seconds:.3f}s sweeps {fs.forward_sweeps} res {fs.max_forward_residual:.1e}; "
+          f"backward {bs.backward_seconds:.3f}s sweeps {bs.adjoint_sweeps} res {bs.max_adjoint_residual:.1e}",
+          flush=True)
+    assert fs.max_forward_residual <= 1e-12 and bs.max_adjoint_residual <= 1e-12
+    assert torch.isfinite(g_tau).all() and torch.isfinite(g_A).all()
+    assert float(g_tau.abs().max()) > 0
+    h = 1e-6
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    errs = []
+    for k in range(32):
+        d = torch.randn(prob.nA, generator=gen, dtype=torch.float64).to(prob.dev)
+        ad = float((g_tau * d).sum())
+        fd = _dJ(prob, log_tau0, A0, U_obs, V_obs, (log_tau0 + h * d, A0), (log_tau0 - h * d, A0)) / (2 * h)
+        errs.append(abs(fd - ad) / abs(ad))
+        print(f"  tau dir {k:2d}: adjoint {ad:+.10e} fd {fd:+.10e} rel {errs[-1]:.1e}", flush=True)
+    assert max(errs) < 1e-5, max(errs)
+    for k in range(4):
+        d = torch.randn(prob.nA, generator=gen, dtype=torch.float64).to(prob.dev)
+        ad = float((g_A * A0 * d).sum())
+        fd = _dJ(prob, log_tau0, A0, U_obs, V_obs, (log_tau0, A0 * (1 + h * d)), (log_tau0, A0 * (1 - h * d))) / (2 * h)
+        rel = abs(fd - ad) / abs(ad)
+        print(f"  A dir {k}: adjoint {ad:+.10e} fd {fd:+.10e} rel {rel:.1e}", flush=True)
+        assert rel < 1e-5
+
+
+def test_checkpointing_gives_identical_gradients(problem):
+    prob, log_tau0, A0, U_obs, V_obs = problem
+    prob.opts.checkpoint_every = 1
+    g1, a1, U1, V1 = _grad(prob, log_tau0, A0, U_obs, V_obs)
+    assert prob.backward_stats.recompute_sweeps == 0
+    prob.opts.checkpoint_every = 2
+    try:
+        g2, a2, U2, V2 = _grad(prob, log_tau0, A0, U_obs, V_obs)
+        assert prob.forward_stats.checkpoints == 3
+        assert prob.backward_stats.recompute_sweeps > 0
+    finally:
+        prob.opts.checkpoint_every = 1
+    print(f"  recompute sweeps {prob.backward_stats.recompute_sweeps}", flush=True)
+    assert torch.equal(U1, U2) and torch.equal(V1, V2)
+    assert torch.equal(g1, g2) and torch.equal(a1, a2)
+
+
+def test_stale_backward_is_refused(problem):
+    from ufemism_amd.adjoint import ssa_velocities
+    prob, log_tau0, A0, U_obs, V_obs = problem
+    lt = log_tau0.clone().requires_grad_(True)
+    U, V = ssa_velocities(lt, A0, prob)
+    prob.forward(log_tau0, A0)
+    with pytest.raises(RuntimeError):
+        (U.sum() + V.sum()).backward()

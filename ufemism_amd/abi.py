@@ -41,6 +41,7 @@ EXPORTS = [
     "ufe_op_map_Ac_to_Aa_3D", "ufe_op_rotate_xy_to_po",
     "ufe_device_alloc", "ufe_device_free", "ufe_memcpy_h2d", "ufe_memcpy_d2h", "ufe_synchronize",
     "ufe_field_fill", "ufe_profile_enable", "ufe_profile_read", "ufe_sweep_bytes",
+    "ufe_ssa_adjoint_forward", "ufe_ssa_adjoint_backward",
 ]
 
 _P = ct.c_void_p
@@ -91,6 +92,22 @@ class LinSolveStats(ct.Structure):
     _fields_ = [("sweeps", ct.c_int32), ("refinements", ct.c_int32), ("matvecs", ct.c_int32),
                 ("rel_residual", ct.c_double),
                 ("seconds", ct.c_double), ("status", ct.c_int32)]
+
+
+class AdjointOpts(ct.Structure):
+    _fields_ = [("lin_rel_tol", ct.c_double), ("picard_iterations", ct.c_int32),
+                ("checkpoint_every", ct.c_int32), ("precision", ct.c_int32),
+                ("gmres_restart", ct.c_int32), ("precond_sweeps", ct.c_int32),
+                ("omega", ct.c_double), ("max_sweeps", ct.c_int32)]
+
+
+class AdjointStats(ct.Structure):
+    _fields_ = [("picard_iterations", ct.c_int32), ("checkpoints", ct.c_int32),
+                ("forward_sweeps", ct.c_int32), ("recompute_sweeps", ct.c_int32),
+                ("adjoint_sweeps", ct.c_int32), ("adjoint_matvecs", ct.c_int32),
+                ("max_forward_residual", ct.c_double), ("max_adjoint_residual", ct.c_double),
+                ("forward_seconds", ct.c_double), ("backward_seconds", ct.c_double),
+                ("status", ct.c_int32)]
 
 
 _lib = None
@@ -282,6 +299,38 @@ class Context:
         st = LinSolveStats()
         self._chk(self.lib.ufe_ssa_linear_solve(self.h, ct.byref(o), ct.byref(st)),
                   "ssa_linear_solve")
+        return st
+
+    def set_device(self, name, ptr):
+        """Copy a device buffer (reference layout) into a field."""
+        fid = self.fields[name][0]
+        self._chk(self.lib.ufe_field_set_device(self.h, fid, ct.c_void_p(int(ptr))), f"set_device {name}")
+
+    def get_device(self, name, ptr):
+        """Copy a field into a device buffer (reference layout)."""
+        fid = self.fields[name][0]
+        self._chk(self.lib.ufe_field_get_device(self.h, fid, ct.c_void_p(int(ptr))), f"get_device {name}")
+
+    def adjoint_forward(self, picard_iterations, lin_rel_tol=1e-12, checkpoint_every=1,
+                        precision=UFE_FP64, gmres_restart=30, precond_sweeps=4, omega=0.0,
+                        max_sweeps=1_000_000, allow=(UFE_OK,)) -> AdjointStats:
+        o = AdjointOpts(float(lin_rel_tol), int(picard_iterations), int(checkpoint_every),
+                        int(precision), int(gmres_restart), int(precond_sweeps), float(omega),
+                        int(max_sweeps))
+        st = AdjointStats()
+        r = self.lib.ufe_ssa_adjoint_forward(self.h, ct.byref(o), ct.byref(st))
+        if r not in allow:
+            raise UFEError(r, f"ssa_adjoint_forward: {self.lib.ufe_last_error(self.h).decode()}")
+        return st
+
+    def adjoint_backward(self, dJ_dU, dJ_dV, grad_log_tau_c, grad_A, allow=(UFE_OK,)) -> AdjointStats:
+        """Device pointers (ints) of nVAaAc doubles, reference AaAc order."""
+        st = AdjointStats()
+        r = self.lib.ufe_ssa_adjoint_backward(self.h, *(ct.c_void_p(int(x)) for x in
+                                                        (dJ_dU, dJ_dV, grad_log_tau_c, grad_A)),
+                                              ct.byref(st))
+        if r not in allow:
+            raise UFEError(r, f"ssa_adjoint_backward: {self.lib.ufe_last_error(self.h).decode()}")
         return st
 
     def fill(self, name, value):

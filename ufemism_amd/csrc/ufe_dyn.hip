@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "ufe_internal.h"
+#include <hip/hip_ext.h>
 
 #include "ufe_kernels.inc"
 
@@ -139,6 +140,12 @@ struct ufe_ctx {
   int32_t *d_neu_row = nullptr, *d_neu_ptr = nullptr, *d_neu_col = nullptr;
   int32_t *d_cor_row = nullptr, *d_cor_ptr = nullptr, *d_cor_col = nullptr;
   int n_neu = 0, n_cor = 0;
+  // transposed operator (adjoint): slot coefficients of row q in its neighbours'
+  // equations, and the reverse edge-average lists
+  double2 *d_cUT = nullptr;
+  double *d_cNxT = nullptr, *d_cNyT = nullptr;
+  int32_t *d_rev_tgt = nullptr, *d_rev_ptr = nullptr, *d_rev_src = nullptr, *d_rev_cnt = nullptr;
+  int n_rev = 0;
   // Aa Neumann for the Aa operator: uses the Aa mesh directly
   // fields
   std::vector<void *> field_ptr;    // per field id (nullptr for pair members)
@@ -170,6 +177,16 @@ struct ufe_ctx {
   struct Pending { hipEvent_t a, b; double bytes; int64_t rows; };
   std::vector<Pending> pending;
   ufe_kernel_profile prof_acc{};
+  // adjoint work space and Picard checkpoints (allocated on first use)
+  int adj_m = 0, adj_K = 0, adj_every = 1, adj_nck = 0;
+  bool adj_ready = false;
+  double2 *adj_V = nullptr, *adj_Z = nullptr, *adj_w = nullptr, *adj_r = nullptr, *adj_g = nullptr,
+          *adj_lam = nullptr, *adj_ubar = nullptr, *adj_uK = nullptr;
+  double *adj_gs = nullptr, *adj_gtau = nullptr, *adj_gA = nullptr, *adj_H1 = nullptr, *adj_Hs = nullptr,
+         *adj_y = nullptr, *adj_nrm = nullptr;
+  std::vector<double2 *> adj_ck;    // stored iterates u_0, u_c, u_2c, ...
+  std::vector<double2 *> adj_seg;   // recomputed iterates of one segment
+  ufe_linsolve_opts adj_lin{};
   SorFlags *h_flags = nullptr;      // pinned
   double *h_scalar = nullptr;       // pinned
   std::vector<void *> allocs;       // everything to free with the mesh
@@ -236,6 +253,18 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->allocs.clear();
   ctx->field_ptr.clear();
   ctx->have_mesh = false;
+  // work spaces live in allocs too: forget them with the mesh
+  ctx->kry_m = 0;
+  ctx->kry_V = ctx->kry_Z = ctx->kry_w = ctx->kry_x = nullptr;
+  ctx->kry_R64 = nullptr;
+  ctx->kry_part = ctx->kry_H1 = ctx->kry_H2 = ctx->kry_Hs = ctx->kry_y = ctx->kry_nrm = nullptr;
+  ctx->adj_m = ctx->adj_K = ctx->adj_nck = 0;
+  ctx->adj_ready = false;
+  ctx->adj_V = ctx->adj_Z = ctx->adj_w = ctx->adj_r = ctx->adj_g = ctx->adj_lam = ctx->adj_ubar = ctx->adj_uK =
+      nullptr;
+  ctx->adj_gs = ctx->adj_gtau = ctx->adj_gA = ctx->adj_H1 = ctx->adj_Hs = ctx->adj_y = ctx->adj_nrm = nullptr;
+  ctx->adj_ck.clear();
+  ctx->adj_seg.clear();
 }
 
 static size_t grid_len(const ufe_ctx *ctx, int grid) {
@@ -744,6 +773,61 @@ int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
     for (int s = 1; s <= m->nCAaAc[ai - 1]; s++) ccol.push_back(iperm[at(m->CAaAc, ldAA, ai, s) - 1]);
     cptr2.push_back((int32_t)ccol.size());
   }
+  // transposed slot coefficients: row q, slot s, neighbour p = col[q,s]: the
+  // coefficient of q in p's equation (zero when p is an edge row)
+  std::vector<uint8_t> active(nA, 0);
+  for (int c = 0; c < 5; c++)
+    for (int p = ctx->colour_start[c]; p < ctx->colour_start[c] + ctx->colour_active[c]; p++) active[p] = 1;
+  std::vector<double2> cUT(ne, make_double2(0.0, 0.0));
+  std::vector<double> cNxT(ne, 0.0), cNyT(ne, 0.0);
+  for (int q = 0; q < nA; q++) {
+    const int64_t bq = cptr[q / SELL] + (q % SELL);
+    for (int s = 0; s < deg[q]; s++) {
+      const int64_t e = bq + (int64_t)s * SELL;
+      const int p = col[e];
+      if (!active[p]) continue;
+      const int64_t bp = cptr[p / SELL] + (p % SELL);
+      int64_t f = -1;
+      for (int t = 0; t < deg[p]; t++)
+        if (col[bp + (int64_t)t * SELL] == q) { f = bp + (int64_t)t * SELL; break; }
+      if (f < 0) {
+        ctx->err = "AaAc connectivity is not symmetric";
+        return UFE_BAD_INPUT;
+      }
+      cUT[e] = cU[f];
+      cNxT[e] = cNx[f];
+      cNyT[e] = cNy[f];
+    }
+  }
+  // reverse edge-average lists: SOR row a <- edge rows whose mean includes a
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> rev(nA);
+  for (size_t i = 0; i < nrow.size(); i++)
+    for (int k = nptr[i]; k < nptr[i + 1]; k++)
+      rev[ncol[k]].push_back({nrow[i], nptr[i + 1] - nptr[i]});
+  std::vector<int32_t> rtgt, rptr{0}, rsrc, rcnt;
+  for (int a = 0; a < nA; a++) {
+    if (rev[a].empty()) continue;
+    if (!active[a]) {
+      ctx->err = "an edge point averages another edge point";
+      return UFE_BAD_INPUT;
+    }
+    rtgt.push_back(a);
+    for (auto &pr : rev[a]) {
+      rsrc.push_back(pr.first);
+      rcnt.push_back(pr.second);
+    }
+    rptr.push_back((int32_t)rsrc.size());
+  }
+  ctx->n_rev = (int)rtgt.size();
+  if ((r = dupload(ctx, &ctx->d_cUT, cUT.data(), cUT.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_cNxT, cNxT.data(), cNxT.size()))) return r;
+  if ((r = dupload(ctx, &ctx->d_cNyT, cNyT.data(), cNyT.size()))) return r;
+  if (ctx->n_rev) {
+    if ((r = dupload(ctx, &ctx->d_rev_tgt, rtgt.data(), rtgt.size()))) return r;
+    if ((r = dupload(ctx, &ctx->d_rev_ptr, rptr.data(), rptr.size()))) return r;
+    if ((r = dupload(ctx, &ctx->d_rev_src, rsrc.data(), rsrc.size()))) return r;
+    if ((r = dupload(ctx, &ctx->d_rev_cnt, rcnt.data(), rcnt.size()))) return r;
+  }
   ctx->n_neu = (int)nrow.size();
   ctx->n_cor = (int)crow.size();
   ctx->h_perm = perm;
@@ -961,21 +1045,27 @@ static hipEvent_t prof_event(ufe_ctx *ctx) {
   ctx->ev_pool.pop_back();
   return e;
 }
-// wrap one colour-kernel launch in events when profiling is on
-extern "C++" template <typename F>
-static void timed_launch(ufe_ctx *ctx, double bytes, int64_t rows, F &&launch) {
-  if (!ctx->prof) {
-    launch();
+// Launch one colour kernel; with profiling on, the start/stop events ride on
+// the dispatch itself (hipExtLaunchKernelGGL), so they time the kernel's own
+// execution and not the queueing around it.
+extern "C++" template <typename K, typename... Args>
+static void timed_kernel(ufe_ctx *ctx, double bytes, int64_t rows, K kernel, dim3 grid, dim3 block,
+                         Args... args) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (ctx->prof) {
+    a = prof_event(ctx);
+    b = prof_event(ctx);
+    if (!a || !b) {
+      if (a) ctx->ev_pool.push_back(a);
+      if (b) ctx->ev_pool.push_back(b);
+      a = b = nullptr;
+    }
+  }
+  if (!a) {
+    hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, args...);
     return;
   }
-  hipEvent_t a = prof_event(ctx), b = prof_event(ctx);
-  if (!a || !b) {
-    launch();
-    return;
-  }
-  (void)hipEventRecord(a, ctx->stream);
-  launch();
-  (void)hipEventRecord(b, ctx->stream);
+  hipExtLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, a, b, 0, args...);
   ctx->pending.push_back({a, b, bytes, rows});
 }
 // after a stream synchronisation: fold the recorded pairs into the profile
@@ -1000,10 +1090,9 @@ static void enqueue_sweep_fp64(ufe_ctx *ctx, const AaAcMesh &m, double omega, do
   for (int c = 0; c < 5; c++) {
     const int n = ctx->colour_active[c];
     if (n == 0) continue;
-    timed_launch(ctx, ctx->bytes64[c], n, [&] {
-      k_sor_colour<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(m, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
-                                                          ctx->colour_start[c], n, omega, fl);
-    });
+    timed_kernel(ctx, ctx->bytes64[c], n, k_sor_colour, dim3(xcd_grid(n)), dim3(BLOCK), m, ctx->d_pair[0],
+                 (const double2 *)ctx->d_pair[1], (const double2 *)ctx->d_pair[2], ctx->colour_start[c], n, omega,
+                 fl);
   }
   if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], fl);
   if (with_flags)
@@ -1024,9 +1113,8 @@ static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, flo
     const int c = backward ? 4 - k : k;
     const int n = ctx->colour_active[c];
     if (n == 0) continue;
-    timed_launch(ctx, ctx->bytes32[c], n, [&] {
-      k_sor_colour_f32<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(m, q, D, R, ctx->colour_start[c], n, omega);
-    });
+    timed_kernel(ctx, ctx->bytes32[c], n, k_sor_colour_f32, dim3(xcd_grid(n)), dim3(BLOCK), m, q, D, R,
+                 ctx->colour_start[c], n, omega);
   }
   if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D);
   k_neumann_corners_f32<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), D);
@@ -1039,10 +1127,8 @@ static void enqueue_sweep_fp64_vec(ufe_ctx *ctx, const AaAcMesh &m, double omega
     const int c = backward ? 4 - k : k;
     const int n = ctx->colour_active[c];
     if (n == 0) continue;
-    timed_launch(ctx, ctx->bytes64[c], n, [&] {
-      k_sor_colour<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(m, D, ctx->d_pair[1], R, ctx->colour_start[c], n, omega,
-                                                          nullptr);
-    });
+    timed_kernel(ctx, ctx->bytes64[c], n, k_sor_colour, dim3(xcd_grid(n)), dim3(BLOCK), m, D,
+                 (const double2 *)ctx->d_pair[1], R, ctx->colour_start[c], n, omega, (SorFlags *)nullptr);
   }
   if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D, nullptr);
   k_neumann_scalar_pair_corners<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), D);
@@ -1272,6 +1358,46 @@ static int krylov_alloc(ufe_ctx *ctx, int m, bool f32) {
   return UFE_OK;
 }
 
+// Least squares min |beta e1 - H y| for the (m+1) x m upper Hessenberg matrix
+// of a GMRES cycle, by Givens rotations on the host.  Column j holds
+// H(i,j) = H1[65 j + i] (+ H2[65 j + i]) for i <= j and H(j+1,j) = Hs[j].
+// Returns the number of columns used (fewer on breakdown).
+static int hessenberg_lsq(int m, const double *H1, const double *H2, const double *Hs, double beta,
+                          std::vector<double> &y) {
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), g(m + 1, 0.0), cs(m), sn(m);
+  auto Hat = [&](int i, int j) -> double & { return H[(size_t)j * (m + 1) + i]; };
+  for (int j = 0; j < m; j++) {
+    for (int i = 0; i <= j; i++) Hat(i, j) = H1[(size_t)j * 65 + i] + (H2 ? H2[(size_t)j * 65 + i] : 0.0);
+    Hat(j + 1, j) = Hs[j];
+  }
+  g[0] = beta;
+  int kk = m;
+  for (int j = 0; j < m; j++) {
+    for (int i = 0; i < j; i++) {
+      const double a = cs[i] * Hat(i, j) + sn[i] * Hat(i + 1, j);
+      const double b = -sn[i] * Hat(i, j) + cs[i] * Hat(i + 1, j);
+      Hat(i, j) = a;
+      Hat(i + 1, j) = b;
+    }
+    const double a = Hat(j, j), b = Hat(j + 1, j);
+    const double rr = std::hypot(a, b);
+    if (!(rr > 0.0) || !std::isfinite(rr)) { kk = j; break; }
+    cs[j] = a / rr;
+    sn[j] = b / rr;
+    Hat(j, j) = rr;
+    Hat(j + 1, j) = 0.0;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+  }
+  y.assign(std::max(kk, 1), 0.0);
+  for (int i = kk - 1; i >= 0; i--) {
+    double t = g[i];
+    for (int l = i + 1; l < kk; l++) t -= Hat(i, l) * y[l];
+    y[i] = t / Hat(i, i);
+  }
+  return kk;
+}
+
 // One GMRES(m) cycle on A d = r (r held in R32 or kry_R64 with |r|^2 in d_scalar[1]),
 // then w += d in FP64.  Templated on the Krylov precision.
 extern "C++" template <typename T>
@@ -1331,39 +1457,8 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
   HIPCHK(ctx, hipMemcpyAsync(sc.data(), ctx->d_scalar, sizeof(double) * 3, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   prof_collect(ctx);
-  const double beta = std::sqrt(sc[1]);
-  // H is (m+1) x m, column j: H(i,j) = H1 + H2 for i <= j, H(j+1,j) = Hs[j]
-  std::vector<double> H((size_t)(m + 1) * m, 0.0), g(m + 1, 0.0), cs(m), sn(m);
-  auto Hat = [&](int i, int j) -> double & { return H[(size_t)j * (m + 1) + i]; };
-  for (int j = 0; j < m; j++) {
-    for (int i = 0; i <= j; i++) Hat(i, j) = H1[(size_t)j * 65 + i] + H2[(size_t)j * 65 + i];
-    Hat(j + 1, j) = Hs[j];
-  }
-  g[0] = beta;
-  int kk = m;
-  for (int j = 0; j < m; j++) {
-    for (int i = 0; i < j; i++) {
-      const double a = cs[i] * Hat(i, j) + sn[i] * Hat(i + 1, j);
-      const double b = -sn[i] * Hat(i, j) + cs[i] * Hat(i + 1, j);
-      Hat(i, j) = a;
-      Hat(i + 1, j) = b;
-    }
-    const double a = Hat(j, j), b = Hat(j + 1, j);
-    const double rr = std::hypot(a, b);
-    if (!(rr > 0.0) || !std::isfinite(rr)) { kk = j; break; }
-    cs[j] = a / rr;
-    sn[j] = b / rr;
-    Hat(j, j) = rr;
-    Hat(j + 1, j) = 0.0;
-    g[j + 1] = -sn[j] * g[j];
-    g[j] = cs[j] * g[j];
-  }
-  std::vector<double> y(std::max(kk, 1), 0.0);
-  for (int i = kk - 1; i >= 0; i--) {
-    double t = g[i];
-    for (int l = i + 1; l < kk; l++) t -= Hat(i, l) * y[l];
-    y[i] = t / Hat(i, i);
-  }
+  std::vector<double> y;
+  const int kk = hessenberg_lsq(m, H1.data(), H2.data(), Hs.data(), std::sqrt(sc[1]), y);
   if (kk == 0) return UFE_OK;
   HIPCHK(ctx, hipMemcpyAsync(ctx->kry_y, y.data(), sizeof(double) * kk, hipMemcpyHostToDevice, ctx->stream));
   k_lincomb<T><<<nb, BLOCK, 0, ctx->stream>>>(n, kk, Z, ctx->kry_y, x);
@@ -1392,6 +1487,10 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
     return UFE_BAD_INPUT;
   }
   if (mr > 0) {
+    // GMRES works on the Neumann-consistent subspace (edge rows follow the SOR
+    // rows), which the corrections keep: start the velocities there as well
+    if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], nullptr);
+    k_neumann_scalar_pair_corners<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
     if ((r = krylov_alloc(ctx, mr, mixed))) return r;
     const int ks = o->precond_sweeps > 0 ? o->precond_sweeps : 2;
     if ((r = residual_fp64(ctx, &rel, true))) return r;
@@ -1432,6 +1531,297 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
   st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   st->status = rel <= o->rel_tol ? UFE_OK : UFE_SOR_NOT_CONVERGED;
   return UFE_OK;
+}
+
+
+// ---- adjoint of the SSA solve ------------------------------------------------
+// (no reference counterpart; the forward mirrors the Picard loop of
+// src/ice_dynamics_module.f90:498-545 with tight linear solves)
+
+static AaAcMesh aaac_mesh_T(ufe_ctx *ctx) {
+  AaAcMesh m = aaac_mesh(ctx);
+  m.cU = ctx->d_cUT;
+  m.cNx = ctx->d_cNxT;
+  m.cNy = ctx->d_cNyT;
+  return m;
+}
+static RevList rev_list(ufe_ctx *c) {
+  RevList R;
+  R.n = c->n_rev;
+  R.tgt = c->d_rev_tgt;
+  R.ptr = c->d_rev_ptr;
+  R.src = c->d_rev_src;
+  R.cnt = c->d_rev_cnt;
+  return R;
+}
+
+static int adj_alloc(ufe_ctx *ctx, int m, int K, int every) {
+  const size_t n = ctx->nA;
+  int r;
+  if (!ctx->adj_V || ctx->adj_m < m) {
+    if ((r = dalloc(ctx, &ctx->adj_V, n * (m + 1)))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_Z, n * m))) return r;
+    ctx->adj_m = m;
+  }
+  if (!ctx->adj_w) {
+    if ((r = dalloc(ctx, &ctx->adj_w, n))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_r, n))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_g, n))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_lam, n))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_ubar, n))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_uK, n))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_gs, 4 * n))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_gtau, n))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_gA, n))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_H1, 65 * 64))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_Hs, 65))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_y, 65))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_nrm, 8))) return r;
+    if (!ctx->kry_part && (r = dalloc(ctx, &ctx->kry_part, (size_t)64 * DOT_BLOCKS))) return r;
+  }
+  const int nck = (K + every - 1) / every;
+  while ((int)ctx->adj_ck.size() < nck) {
+    double2 *b;
+    if ((r = dalloc(ctx, &b, n))) return r;
+    ctx->adj_ck.push_back(b);
+  }
+  while ((int)ctx->adj_seg.size() < every - 1) {
+    double2 *b;
+    if ((r = dalloc(ctx, &b, n))) return r;
+    ctx->adj_seg.push_back(b);
+  }
+  ctx->adj_nck = nck;
+  return UFE_OK;
+}
+
+static int copy_pair(ufe_ctx *ctx, double2 *dst, const double2 *src) {
+  HIPCHK(ctx, hipMemcpyAsync(dst, src, sizeof(double2) * ctx->nA, hipMemcpyDeviceToDevice, ctx->stream));
+  return UFE_OK;
+}
+
+// one Picard step of the adjoint forward: eta, S and the system from W, then W := solution
+static int adj_forward_step(ufe_ctx *ctx, int *sweeps, double *rel, int *status) {
+  int r;
+  if ((r = ssa_viscosity_rn(ctx, nullptr))) return r;
+  if ((r = ssa_prepare_coeffs(ctx))) return r;
+  ufe_linsolve_stats ls;
+  if ((r = ufe_ssa_linear_solve(ctx, &ctx->adj_lin, &ls))) return r;
+  *sweeps += ls.sweeps;
+  *rel = std::max(*rel, ls.rel_residual);
+  if (ls.status != UFE_OK) *status = ls.status;
+  return UFE_OK;
+}
+
+// P^T: fold the edge rows of y back onto the SOR rows that define them, then clear them
+static void fold_T(ufe_ctx *ctx, double2 *y) {
+  k_fold_corners<<<1, 1, 0, ctx->stream>>>(cor_list(ctx), y);
+  if (ctx->n_rev) k_fold_edges<<<nblk(ctx->n_rev), BLOCK, 0, ctx->stream>>>(rev_list(ctx), y);
+  if (ctx->n_neu) k_zero_rows<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), y);
+}
+// y = P^T A^T R^T x
+static void apply_AT(ufe_ctx *ctx, const AaAcMesh &mT, const ActiveRanges &ar, const double2 *x, double2 *y) {
+  k_matvec_T<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(mT, ctx->d_pair[1], ar, x, y);
+  fold_T(ctx, y);
+}
+// z = ks transposed SOR sweeps from zero on A^T z = v (edge rows stay zero)
+static void precond_T(ufe_ctx *ctx, const AaAcMesh &mT, double omega, int ks, const double2 *v, double2 *z) {
+  k_fill<double2><<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(z, ctx->nA, make_double2(0.0, 0.0));
+  for (int k = 0; k < ks; k++)
+    for (int i = 0; i < 5; i++) {
+      const int c = (k & 1) ? 4 - i : i;
+      const int n = ctx->colour_active[c];
+      if (n == 0) continue;
+      k_sor_colour<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(mT, z, ctx->d_pair[1], v, ctx->colour_start[c], n,
+                                                         omega, nullptr);
+    }
+}
+
+// restarted GMRES(m) on (P^T A^T R^T) lam = g, FP64, right-preconditioned by
+// transposed SOR; lam (ctx->adj_lam) is the initial guess on entry
+static int adj_solve(ufe_ctx *ctx, double tol, int m, int ks, double omega, int max_sweeps, int *sweeps,
+                     int *matvecs, double *rel_out) {
+  const int n = ctx->nA, nb = nblk(n);
+  const AaAcMesh mT = aaac_mesh_T(ctx);
+  const ActiveRanges ar = active_ranges(ctx);
+  double2 *V = ctx->adj_V, *Z = ctx->adj_Z, *w = ctx->adj_w, *rr = ctx->adj_r;
+  int used = 0;
+  double rel = 0.0;
+  std::vector<double> H1(65 * 64), Hs(65), sc(4);
+  while (true) {
+    apply_AT(ctx, mT, ar, ctx->adj_lam, w);
+    k_adj_residual<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->adj_g, w, rr, ctx->d_part);
+    k_adj_norms<<<1, 1024, 0, ctx->stream>>>(nb, ctx->d_part, ctx->adj_nrm);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, hipMemcpyAsync(sc.data(), ctx->adj_nrm, sizeof(double) * 3, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const double gn = std::sqrt(sc[2]);
+    rel = gn > 0.0 ? sc[0] / gn : 0.0;
+    if (!std::isfinite(rel)) {
+      ctx->err = "adjoint solve diverged";
+      return UFE_SOR_NOT_CONVERGED;
+    }
+    if (rel <= tol || gn == 0.0 || used >= max_sweeps) break;
+    k_normalize<double><<<nb, BLOCK, 0, ctx->stream>>>(n, rr, ctx->adj_nrm + 1, V, ctx->adj_nrm + 4);
+    for (int j = 0; j < m; j++) {
+      double2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;
+      precond_T(ctx, mT, omega, ks, vj, zj);
+      used += ks;
+      apply_AT(ctx, mT, ar, zj, w);
+      (*matvecs)++;
+      double *h1 = ctx->adj_H1 + (int64_t)j * 65;
+      k_multi_dot<double><<<DOT_BLOCKS, BLOCK, 0, ctx->stream>>>(n, j + 1, ar, V, w, ctx->kry_part);
+      k_multi_dot_reduce<<<j + 1, 256, 0, ctx->stream>>>(DOT_BLOCKS, j + 1, ctx->kry_part, h1, 0);
+      k_multi_axpy<double><<<nb, BLOCK, 0, ctx->stream>>>(n, j + 1, V, h1, w);
+      k_multi_dot<double><<<DOT_BLOCKS, BLOCK, 0, ctx->stream>>>(n, 1, ar, w, w, ctx->kry_part);
+      k_multi_dot_reduce<<<1, 256, 0, ctx->stream>>>(DOT_BLOCKS, 1, ctx->kry_part, ctx->adj_nrm + 5, 0);
+      k_normalize<double><<<nb, BLOCK, 0, ctx->stream>>>(n, w, ctx->adj_nrm + 5, V + (int64_t)(j + 1) * n,
+                                                        ctx->adj_Hs + j);
+    }
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, hipMemcpyAsync(H1.data(), ctx->adj_H1, sizeof(double) * 65 * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(Hs.data(), ctx->adj_Hs, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<double> y;
+    const int kk = hessenberg_lsq(m, H1.data(), nullptr, Hs.data(), sc[0], y);
+    if (kk == 0) break;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->adj_y, y.data(), sizeof(double) * kk, hipMemcpyHostToDevice, ctx->stream));
+    k_lincomb<double><<<nb, BLOCK, 0, ctx->stream>>>(n, kk, Z, ctx->adj_y, w);
+    k_add_pair<<<nb, BLOCK, 0, ctx->stream>>>(n, w, ctx->adj_lam);
+    LAUNCHCHK(ctx);
+  }
+  *sweeps += used;
+  *rel_out = rel;
+  return rel <= tol ? UFE_OK : UFE_SOR_NOT_CONVERGED;
+}
+
+int ufe_ssa_adjoint_forward(ufe_ctx *ctx, const ufe_adjoint_opts *o, ufe_adjoint_stats *st) {
+  if (!ctx || !ctx->have_mesh || !o || !st) return UFE_BAD_INPUT;
+  std::memset(st, 0, sizeof(*st));
+  if (o->picard_iterations < 1 || o->picard_iterations > 100000 || o->checkpoint_every < 1 ||
+      o->gmres_restart < 1 || o->gmres_restart > 64 || !(o->lin_rel_tol > 0.0) ||
+      (o->precision != UFE_FP64 && o->precision != UFE_MIXED)) {
+    ctx->err = "ufe_ssa_adjoint_forward: picard_iterations >= 1, checkpoint_every >= 1, gmres_restart in 1..64, "
+               "lin_rel_tol > 0 and precision UFE_FP64 or UFE_MIXED are required";
+    return UFE_BAD_INPUT;
+  }
+  if (ctx->params.use_analytical_GL_flux) {
+    ctx->err = "use_analytical_GL_flux is not supported";
+    return st->status = UFE_NOT_SUPPORTED;
+  }
+  const int K = o->picard_iterations, every = o->checkpoint_every;
+  int r;
+  if ((r = adj_alloc(ctx, o->gmres_restart, K, every))) return r;
+  ufe_linsolve_opts &L = ctx->adj_lin;
+  L = ufe_linsolve_opts{};
+  L.rel_tol = o->lin_rel_tol;
+  L.max_sweeps = o->max_sweeps > 0 ? o->max_sweeps : 1000000;
+  L.precision = o->precision;
+  L.omega = o->omega;
+  L.gmres_restart = o->gmres_restart;
+  L.precond_sweeps = o->precond_sweeps;
+  L.orth_passes = 1;
+  ctx->adj_ready = false;
+  auto t0 = std::chrono::steady_clock::now();
+  if ((r = ssa_gather(ctx))) return r;
+  for (int k = 1; k <= K; k++) {
+    if ((k - 1) % every == 0) {
+      if ((r = copy_pair(ctx, ctx->adj_ck[(k - 1) / every], ctx->d_pair[0]))) return r;
+      st->checkpoints++;
+    }
+    if ((r = adj_forward_step(ctx, &st->forward_sweeps, &st->max_forward_residual, &st->status))) return r;
+  }
+  if ((r = copy_pair(ctx, ctx->adj_uK, ctx->d_pair[0]))) return r;
+  if ((r = ssa_finish(ctx))) return r;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  st->picard_iterations = K;
+  st->forward_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  ctx->adj_K = K;
+  ctx->adj_every = every;
+  ctx->adj_ready = true;
+  return st->status;
+}
+
+int ufe_ssa_adjoint_backward(ufe_ctx *ctx, const double *dJ_dU, const double *dJ_dV, double *g_logtau,
+                             double *g_A, ufe_adjoint_stats *st) {
+  if (!ctx || !ctx->have_mesh || !dJ_dU || !dJ_dV || !g_logtau || !g_A || !st) return UFE_BAD_INPUT;
+  if (!ctx->adj_ready) {
+    ctx->err = "ufe_ssa_adjoint_backward needs a preceding ufe_ssa_adjoint_forward";
+    return UFE_BAD_INPUT;
+  }
+  const int n = ctx->nA, nb = nblk(n), K = ctx->adj_K, every = ctx->adj_every;
+  const ufe_linsolve_opts &L = ctx->adj_lin;
+  const double omega = L.omega > 0.0 ? L.omega : ctx->params.SSA_SOR_omega;
+  const int ks = L.precond_sweeps > 0 ? L.precond_sweeps : 2;
+  const AaAcMesh am = aaac_mesh(ctx), mT = aaac_mesh_T(ctx);
+  const ActiveRanges ar = active_ranges(ctx);
+  int r;
+  auto t0 = std::chrono::steady_clock::now();
+  // g_K = P^T dJ/du_K
+  k_gather_perm_pair<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, dJ_dU, ctx->adj_ubar, 0);
+  k_gather_perm_pair<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, dJ_dV, ctx->adj_ubar, 1);
+  fold_T(ctx, ctx->adj_ubar);
+  if ((r = copy_pair(ctx, ctx->adj_g, ctx->adj_ubar))) return r;
+  k_fill<double><<<nb, BLOCK, 0, ctx->stream>>>(ctx->adj_gtau, n, 0.0);
+  k_fill<double><<<nb, BLOCK, 0, ctx->stream>>>(ctx->adj_gA, n, 0.0);
+  k_fill<double2><<<nb, BLOCK, 0, ctx->stream>>>(ctx->adj_lam, n, make_double2(0.0, 0.0));
+  LAUNCHCHK(ctx);
+  AdjPoint ap;
+  ap.lam = ctx->adj_lam;
+  ap.gtau = ctx->adj_gtau;
+  ap.gA = ctx->adj_gA;
+  ap.gux = ctx->adj_gs;
+  ap.guy = ctx->adj_gs + n;
+  ap.gvx = ctx->adj_gs + 2 * (size_t)n;
+  ap.gvy = ctx->adj_gs + 3 * (size_t)n;
+  ap.ubar = ctx->adj_ubar;
+  const double2 *u_k = ctx->adj_uK;
+  int seg_loaded = -1;
+  for (int k = K; k >= 1; k--) {
+    const int j = k - 1, s = j / every, off = j % every;
+    const double2 *u_km1;
+    if (off == 0) {
+      u_km1 = ctx->adj_ck[s];
+    } else {
+      if (seg_loaded != s) {  // recompute the stored segment's inner iterates once
+        if ((r = copy_pair(ctx, ctx->d_pair[0], ctx->adj_ck[s]))) return r;
+        for (int i = 1; i < every && s * every + i <= K - 1; i++) {
+          double rel = 0.0;
+          int stt = 0;
+          if ((r = adj_forward_step(ctx, &st->recompute_sweeps, &rel, &stt))) return r;
+          if ((r = copy_pair(ctx, ctx->adj_seg[i - 1], ctx->d_pair[0]))) return r;
+        }
+        seg_loaded = s;
+      }
+      u_km1 = ctx->adj_seg[off - 1];
+    }
+    // eta, S, the system and the strain rates of step k, from u_{k-1}
+    if ((r = copy_pair(ctx, ctx->d_pair[0], u_km1))) return r;
+    if ((r = ssa_viscosity_rn(ctx, nullptr))) return r;
+    if ((r = ssa_prepare_coeffs(ctx))) return r;
+    double rel = 0.0;
+    r = adj_solve(ctx, L.rel_tol, L.gmres_restart, ks, omega, L.max_sweeps, &st->adjoint_sweeps, &st->adjoint_matvecs,
+                  &rel);
+    st->max_adjoint_residual = std::max(st->max_adjoint_residual, rel);
+    if (r == UFE_SOR_NOT_CONVERGED) st->status = UFE_SOR_NOT_CONVERGED;
+    else if (r) return r;
+    ap.uk = u_k;
+    ap.ukm1 = u_km1;
+    k_adj_point<<<nb, BLOCK, 0, ctx->stream>>>(am, ssa_fields(ctx), ar, ap);
+    k_adj_strain_T<<<nb, BLOCK, 0, ctx->stream>>>(mT, ar, ap.gux, ap.guy, ap.gvx, ap.gvy, ctx->adj_ubar);
+    fold_T(ctx, ctx->adj_ubar);
+    if ((r = copy_pair(ctx, ctx->adj_g, ctx->adj_ubar))) return r;
+    LAUNCHCHK(ctx);
+    u_k = u_km1;
+  }
+  // leave the forward's final velocities in place
+  if ((r = copy_pair(ctx, ctx->d_pair[0], ctx->adj_uK))) return r;
+  k_scatter_perm<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, ctx->adj_gtau, g_logtau);
+  k_scatter_perm<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, ctx->adj_gA, g_A);
+  LAUNCHCHK(ctx);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  st->picard_iterations = K;
+  st->backward_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return st->status;
 }
 
 int ufe_profile_enable(ufe_ctx *ctx, int enable) {
