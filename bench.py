@@ -39,6 +39,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sor_colour_marine1M_hbm_traffic.txt")
+
+
+def measured_traffic(kernel, nA):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (copy-calibrated), when it was taken on a mesh of this size."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                p = line.split()
+                if len(p) == 4 and p[0] == "traffic" and p[1] == kernel and int(p[2]) == nA:
+                    return int(p[3])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,14 +74,20 @@ def main():
     ap.add_argument("--orth-passes", type=int, default=1, help="Gram-Schmidt passes (1 or 2)")
     ap.add_argument("--mode", default="both",
                     choices=["both", "mixed_gmres", "fp64_gmres", "fp64_sor"])
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-nv", type=int, default=200_000, help="vertices of the adjoint mesh")
+    ap.add_argument("--adj-picard", type=int, default=30, help="Picard steps of the adjoint forward")
+    ap.add_argument("--adj-tol", type=float, default=1e-12, help="linear-solve tolerance of the adjoint")
+    ap.add_argument("--adj-fd", type=int, default=4, help="finite-difference directions to check")
+    ap.add_argument("--adj-every", type=int, default=1, help="checkpoint every c-th Picard iterate")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
-                                   "design-only (see DESIGN.md); run with --gpus 1"}))
+        print(json.dumps({"error": "this build runs on one GPU only (the sharded multi-GPU path is "
+                                   "not built); run with --gpus 1"}))
         sys.exit(2)
 
     import numpy as np
@@ -175,9 +198,12 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": round(R["kernel_GBps"], 2), "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": round(R["kernel_GBps"] * 1e9 / HBM_PEAK, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(R["kernel_GBps"] * 1e9 / HBM_PEAK, 4),
+            "traffic": measured_traffic(kname, nA),
             "kernel": kname, "bytes_per_launch": round(R["kernel_bytes_per_launch"]),
             "avg_launch_us": round(R["kernel_avg_us"], 3),
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per launch, scaled by a 1 GiB "
+                              "copy (read x2.0, write x1.0); null when not measured on this mesh size",
         },
         "paths": paths,
         "reference_ssa_seconds": round(ref_ssa_s, 3),
@@ -203,7 +229,8 @@ def main():
                                               "dHs_dx_shelf_AaAc", "dHs_dy_shelf_AaAc")}
             fields["U_SSA_AaAc"] = np.zeros(nA)
             fields["V_SSA_AaAc"] = np.zeros(nA)
-            dt1, n1 = pyoracle.time_sor_sweeps(om, fields, wl.params, 2, threads)
+            pyoracle.time_sor_sweeps(om, fields, wl.params, 2, threads)       # first touch
+            dt1, n1 = pyoracle.time_sor_sweeps(om, fields, wl.params, 8, threads)
             nsw = max(2, int(args.cpu_seconds / max(dt1 / max(n1, 1), 1e-6)))
             dt, n = pyoracle.time_sor_sweeps(om, fields, wl.params, nsw, threads)
             out["cpu_baseline"] = {
@@ -213,8 +240,82 @@ def main():
                           f"on the same {nA}-point linear system, {dt:.1f}s"}
         except Exception as e:  # the CPU baseline is reported, never required
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(args, ctx)
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def bench_adjoint(args, ctx):
+    """Adjoint of K Picard steps on the marine mesh: J = 1/2 |u - u_obs|^2 with
+    u_obs from a friction field perturbed log-normally (sigma 0.3); times the
+    forward and the backward and checks the gradient by central differences."""
+    import numpy as np
+    import torch
+    from ufemism_amd import geometry
+    from ufemism_amd.adjoint import AdjointOptions, SSAProblem, ssa_velocities
+    from ufemism_amd.dynamics import Step
+
+    t0 = time.time()
+    wl = geometry.marine(n_target=args.adj_nv, seed=args.seed + 1, picard=args.adj_picard)
+    step = Step(ctx, wl)
+    ctx.update_general_ice_model_data()
+    log(f"[adjoint] mesh nV={wl.mesh.nV} nVAaAc={ctx.sizes['nVAaAc']} set up in {time.time() - t0:.1f}s")
+    prob = SSAProblem(ctx, AdjointOptions(picard_iterations=args.adj_picard, lin_rel_tol=args.adj_tol,
+                                          checkpoint_every=args.adj_every))
+    dev = prob.dev
+    VA = step.ma["VAaAc"]
+    log_tau0 = torch.log(torch.tensor(wl.tau_c_override(VA), dtype=torch.float64, device=dev))
+    A0 = torch.full((prob.nA,), wl.A_flow, dtype=torch.float64, device=dev)
+    g = torch.tensor(geometry.gaussian_field(VA, 50e3, args.seed + 4242), dtype=torch.float64, device=dev)
+    U_obs, V_obs = prob.forward(log_tau0 + 0.3 * g, A0)
+    log(f"[adjoint] observations: {prob.forward_stats.forward_seconds:.2f}s")
+
+    def grad():
+        lt = log_tau0.clone().requires_grad_(True)
+        a = A0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        U, V = ssa_velocities(lt, a, prob)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        J = 0.5 * (((U - U_obs) ** 2).sum() + ((V - V_obs) ** 2).sum())
+        J.backward()
+        torch.cuda.synchronize()
+        tc = time.perf_counter()
+        return lt.grad, a.grad, tb - ta, tc - tb, float(J)
+
+    grad()                                          # warm-up (allocations)
+    g_tau, g_A, tf, tb, J = grad()
+    fs, bs = prob.forward_stats, prob.backward_stats
+    log(f"[adjoint] forward {tf:.3f}s backward {tb:.3f}s ratio {tb / tf:.2f}")
+    h = 1e-6
+    gen = torch.Generator(device="cpu").manual_seed(args.seed)
+    errs = []
+    for k in range(args.adj_fd):
+        d = torch.randn(prob.nA, generator=gen, dtype=torch.float64).to(dev)
+        ad = float((g_tau * d).sum())
+        Up, Vp = prob.forward(log_tau0 + h * d, A0)
+        Um, Vm = prob.forward(log_tau0 - h * d, A0)
+        fd = 0.5 * float(((Up - Um) * (Up + Um - 2 * U_obs)).sum() +
+                         ((Vp - Vm) * (Vp + Vm - 2 * V_obs)).sum()) / (2 * h)
+        errs.append(abs(fd - ad) / abs(ad))
+        log(f"[adjoint] FD direction {k}: adjoint {ad:+.9e} fd {fd:+.9e} rel {errs[-1]:.1e}")
+    return {
+        "workload": f"marine ice sheet nV={wl.mesh.nV} (nVAaAc={prob.nA}); J = 1/2 |u - u_obs|^2 over "
+                    f"AaAc (u, v) after {args.adj_picard} Picard steps, u_obs from friction perturbed "
+                    f"log-normally (sigma 0.3); gradient w.r.t. log tau_c and A",
+        "picard_iterations": args.adj_picard, "lin_rel_tol": args.adj_tol,
+        "checkpoint_every": args.adj_every, "checkpoints": fs.checkpoints,
+        "forward_seconds": round(tf, 4), "backward_seconds": round(tb, 4),
+        "adjoint_forward_ratio": round(tb / tf, 3),
+        "forward_sweeps": fs.forward_sweeps, "adjoint_sweeps": bs.adjoint_sweeps,
+        "recompute_sweeps": bs.recompute_sweeps,
+        "max_forward_residual": fs.max_forward_residual, "max_adjoint_residual": bs.max_adjoint_residual,
+        "J": J, "fd_directions": args.adj_fd, "fd_step": h,
+        "fd_max_rel_error": max(errs) if errs else None,
+        "fd_rel_errors": [float(f"{e:.3e}") for e in errs],
+    }
 
 
 if __name__ == "__main__":

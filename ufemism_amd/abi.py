@@ -120,6 +120,13 @@ def load():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"libufe_dyn.so not found at {LIB_PATH}; run "
                                "`python -m ufemism_amd.build` (or __graft_entry__.build())")
+        # torch ships its own HIP runtime.  Both runtimes bind the HSA runtime by
+        # soname, and torch's finds no device on a newer HSA runtime loaded before
+        # it, so a process that may use torch loads torch first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ct.CDLL(LIB_PATH)
         lib.ufe_last_error.restype = ct.c_char_p
         lib.ufe_field_name.restype = ct.c_char_p
