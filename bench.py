@@ -77,9 +77,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-step", action="store_true", help="skip the mixed-vs-FP64 dynamics steps")
+    ap.add_argument("--step-picard", type=int, default=3, help="Picard iterations of those steps")
+    ap.add_argument("--step-tol", type=float, default=1e-9, help="tighter tolerance of those steps")
     ap.add_argument("--adj-nv", type=int, default=200_000, help="vertices of the adjoint mesh")
-    ap.add_argument("--adj-picard", type=int, default=30, help="Picard steps of the adjoint forward")
-    ap.add_argument("--adj-tol", type=float, default=1e-12, help="linear-solve tolerance of the adjoint")
+    ap.add_argument("--adj-picard", type=int, default=10, help="Picard steps of the adjoint forward")
+    ap.add_argument("--adj-tol", type=float, default=1e-10, help="linear-solve tolerance of the adjoint")
     ap.add_argument("--adj-fd", type=int, default=4, help="finite-difference directions to check")
     ap.add_argument("--adj-every", type=int, default=1, help="checkpoint every c-th Picard iterate")
     args = ap.parse_args()
@@ -240,10 +243,52 @@ def main():
                           f"on the same {nA}-point linear system, {dt:.1f}s"}
         except Exception as e:  # the CPU baseline is reported, never required
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    if not args.no_step:
+        out["precision_error"] = bench_precision(args, ctx, step, wl)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(args, ctx)
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def bench_precision(args, ctx, step, wl):
+    """One full dynamics step (general update, SIA, SSA with K Picard
+    iterations, thickness update) with every linearised system solved by
+    GMRES-IR in mixed precision and in FP64 to the same tolerance: the relative
+    L2 error of the mixed step's u, v and H against the FP64 step."""
+    import copy
+    import numpy as np
+    from ufemism_amd import abi
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    p = copy.deepcopy(wl.params)
+    p.SSA_max_outer_loops = args.step_picard
+    res = {"picard_iterations": args.step_picard}
+    for tol in (args.tol, args.step_tol):
+        outs = {}
+        for name, mode in (("fp64", abi.UFE_FP64_KRYLOV), ("mixed", abi.UFE_MIXED)):
+            step.reset()
+            for f in ("U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac", "N_AaAc"):
+                ctx.fill(f, 0.0)
+            ctx.set_params(p, mode, linear_rel_tol=tol, gmres_restart=args.gmres,
+                           precond_sweeps=args.precond)
+            ctx.synchronize()
+            ta = time.perf_counter()
+            r = step.run()
+            el = time.perf_counter() - ta
+            outs[name] = (r, el)
+            log(f"[step] tol {tol:g} {name}: {el:.2f}s picard {r['picard']} sweeps {r['sweeps']}")
+        (a, ta_), (b, tb_) = outs["fp64"], outs["mixed"]
+        res[f"tol_{tol:g}"] = {
+            "seconds": {"fp64": round(ta_, 3), "mixed": round(tb_, 3)},
+            "sweeps": {"fp64": a["sweeps"], "mixed": b["sweeps"]},
+            "relL2_mixed_vs_fp64": {k: rel(b[k], a[k]) for k in
+                                   ("U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac", "Hi")},
+        }
+    ctx.set_params(wl.params, abi.UFE_FP64)
+    return res
 
 
 def bench_adjoint(args, ctx):

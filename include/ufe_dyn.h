@@ -38,7 +38,7 @@ enum {
   UFE_NOT_SUPPORTED = 8
 };
 
-enum { UFE_FP64 = 0, UFE_MIXED = 1 };
+enum { UFE_FP64 = 0, UFE_MIXED = 1, UFE_FP64_KRYLOV = 2 };
 
 /* Mesh in reference layout.  Primary arrays are inputs everywhere; the
  * secondary arrays are written by ufe_mesh_build and read by ufe_mesh_upload.
@@ -87,8 +87,17 @@ typedef struct ufe_params {
   int32_t ssa_forced_zero;          /* EISMINT/Halfar/Bueler: SSA velocities zero (:431-449) */
   int32_t zero_Hi_at_edges;         /* ice-thickness edge condition (:189-229) */
   int32_t use_tau_c_override;       /* tau_c := field "tau_c_override_AaAc" after basal_yield_stress */
-  int32_t precision;                /* UFE_FP64 (reference arithmetic) or UFE_MIXED */
+  int32_t precision;                /* solve_SSA linear solves: UFE_FP64 (the reference's SOR and
+                                       stopping rule), UFE_MIXED or UFE_FP64_KRYLOV (GMRES-IR below) */
   int32_t sweep_batch;              /* SOR sweeps queued per host convergence read; 0 = adaptive */
+  /* UFE_MIXED / UFE_FP64_KRYLOV: each Picard iteration's linearised system is
+   * solved to the FP64 relative residual SSA_linear_rel_tol by restarted
+   * GMRES(SSA_gmres_restart) preconditioned by SSA_precond_sweeps SOR sweeps,
+   * FP32 sweeps and Krylov vectors for UFE_MIXED, FP64 for UFE_FP64_KRYLOV,
+   * with FP64 residuals and iterative refinement either way. */
+  double SSA_linear_rel_tol;        /* <= 0: 1e-6 */
+  int32_t SSA_gmres_restart;        /* <= 0: 30 */
+  int32_t SSA_precond_sweeps;       /* <= 0: 4 */
 } ufe_params;
 
 /* Statistics of one solve_SSA call. */

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as ct
 import os
+import warnings
 
 import numpy as np
 
@@ -19,7 +20,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "lib
 UFE_OK, UFE_SOR_NOT_CONVERGED, UFE_SSA_RESET, UFE_SSA_UNSTABLE = 0, 1, 2, 3
 UFE_COLOURING_IDENTIFY, UFE_COLOURING_FAILED, UFE_BAD_INPUT = 4, 5, 6
 UFE_HIP_ERROR, UFE_NOT_SUPPORTED = 7, 8
-UFE_FP64, UFE_MIXED = 0, 1
+UFE_FP64, UFE_MIXED, UFE_FP64_KRYLOV = 0, 1, 2
 
 STATUS_NAMES = {0: "OK", 1: "SOR_NOT_CONVERGED", 2: "SSA_RESET", 3: "SSA_UNSTABLE",
                 4: "COLOURING_IDENTIFY", 5: "COLOURING_FAILED", 6: "BAD_INPUT",
@@ -66,7 +67,9 @@ class Params(ct.Structure):
                 ("SSA_SOR_omega", ct.c_double), ("SSA_max_inner_loops", ct.c_int32),
                 ("use_analytical_GL_flux", ct.c_int32), ("ssa_forced_zero", ct.c_int32),
                 ("zero_Hi_at_edges", ct.c_int32), ("use_tau_c_override", ct.c_int32),
-                ("precision", ct.c_int32), ("sweep_batch", ct.c_int32)]
+                ("precision", ct.c_int32), ("sweep_batch", ct.c_int32),
+                ("SSA_linear_rel_tol", ct.c_double), ("SSA_gmres_restart", ct.c_int32),
+                ("SSA_precond_sweeps", ct.c_int32)]
 
 
 class SSAStats(ct.Structure):
@@ -160,7 +163,8 @@ def _ptr(a):
     return None if a is None else ct.c_void_p(a.ctypes.data)
 
 
-def params_struct(p: DynParams, precision=UFE_FP64, sweep_batch=0) -> Params:
+def params_struct(p: DynParams, precision=UFE_FP64, sweep_batch=0, linear_rel_tol=1e-6,
+                  gmres_restart=30, precond_sweeps=4) -> Params:
     s = Params()
     s.nZ = p.nZ
     for k, z in enumerate(p.zeta):
@@ -175,6 +179,9 @@ def params_struct(p: DynParams, precision=UFE_FP64, sweep_batch=0) -> Params:
     s.use_tau_c_override = int(p.use_tau_c_override)
     s.precision = int(precision)
     s.sweep_batch = int(sweep_batch)
+    s.SSA_linear_rel_tol = float(linear_rel_tol)
+    s.SSA_gmres_restart = int(gmres_restart)
+    s.SSA_precond_sweeps = int(precond_sweeps)
     return s
 
 
@@ -239,8 +246,9 @@ class Context:
             raise UFEError(r, f"{what}: {self.lib.ufe_last_error(self.h).decode()}")
         return r
 
-    def set_params(self, p: DynParams, precision=UFE_FP64, sweep_batch=0):
-        self._params = params_struct(p, precision, sweep_batch)
+    def set_params(self, p: DynParams, precision=UFE_FP64, sweep_batch=0, **krylov):
+        """krylov: linear_rel_tol, gmres_restart, precond_sweeps (UFE_MIXED / UFE_FP64_KRYLOV)."""
+        self._params = params_struct(p, precision, sweep_batch, **krylov)
         self._chk(self.lib.ufe_set_params(self.h, ct.byref(self._params)), "ufe_set_params")
 
     def mesh_build(self, mesh) -> MeshArrays:
@@ -320,7 +328,7 @@ class Context:
 
     def adjoint_forward(self, picard_iterations, lin_rel_tol=1e-12, checkpoint_every=1,
                         precision=UFE_FP64, gmres_restart=30, precond_sweeps=4, omega=0.0,
-                        max_sweeps=1_000_000, allow=(UFE_OK,)) -> AdjointStats:
+                        max_sweeps=1_000_000, allow=(UFE_OK, UFE_SOR_NOT_CONVERGED)) -> AdjointStats:
         o = AdjointOpts(float(lin_rel_tol), int(picard_iterations), int(checkpoint_every),
                         int(precision), int(gmres_restart), int(precond_sweeps), float(omega),
                         int(max_sweeps))
@@ -328,9 +336,13 @@ class Context:
         r = self.lib.ufe_ssa_adjoint_forward(self.h, ct.byref(o), ct.byref(st))
         if r not in allow:
             raise UFEError(r, f"ssa_adjoint_forward: {self.lib.ufe_last_error(self.h).decode()}")
+        if r == UFE_SOR_NOT_CONVERGED:
+            warnings.warn(f"ssa_adjoint_forward: a linear solve stopped at relative residual "
+                          f"{st.max_forward_residual:.2e} > {lin_rel_tol:.1e}", RuntimeWarning)
         return st
 
-    def adjoint_backward(self, dJ_dU, dJ_dV, grad_log_tau_c, grad_A, allow=(UFE_OK,)) -> AdjointStats:
+    def adjoint_backward(self, dJ_dU, dJ_dV, grad_log_tau_c, grad_A,
+                         allow=(UFE_OK, UFE_SOR_NOT_CONVERGED)) -> AdjointStats:
         """Device pointers (ints) of nVAaAc doubles, reference AaAc order."""
         st = AdjointStats()
         r = self.lib.ufe_ssa_adjoint_backward(self.h, *(ct.c_void_p(int(x)) for x in
@@ -338,6 +350,9 @@ class Context:
                                               ct.byref(st))
         if r not in allow:
             raise UFEError(r, f"ssa_adjoint_backward: {self.lib.ufe_last_error(self.h).decode()}")
+        if r == UFE_SOR_NOT_CONVERGED:
+            warnings.warn(f"ssa_adjoint_backward: an adjoint solve stopped at relative residual "
+                          f"{st.max_adjoint_residual:.2e}", RuntimeWarning)
         return st
 
     def fill(self, name, value):

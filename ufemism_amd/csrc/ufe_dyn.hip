@@ -436,6 +436,14 @@ int ufe_set_params(ufe_ctx *ctx, const ufe_params *p) {
     ctx->err = "nZ cannot change after mesh upload";
     return UFE_BAD_INPUT;
   }
+  if (p->precision != UFE_FP64 && p->precision != UFE_MIXED && p->precision != UFE_FP64_KRYLOV) {
+    ctx->err = "precision must be UFE_FP64, UFE_MIXED or UFE_FP64_KRYLOV";
+    return UFE_BAD_INPUT;
+  }
+  if (p->SSA_gmres_restart > 64) {
+    ctx->err = "SSA_gmres_restart must be at most 64";
+    return UFE_BAD_INPUT;
+  }
   ctx->params = *p;
   ctx->nZ = p->nZ;
   return UFE_OK;
@@ -1220,10 +1228,6 @@ int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) {
     ctx->err = "use_analytical_GL_flux is not supported";
     return st->status = UFE_NOT_SUPPORTED;
   }
-  if (P.precision != UFE_FP64) {
-    ctx->err = "solve_SSA runs the reference arithmetic (FP64); mixed precision is ufe_ssa_linear_solve";
-    return st->status = UFE_NOT_SUPPORTED;
-  }
   auto t0 = std::chrono::steady_clock::now();
   // gating (:430-465)
   HIPCHK(ctx, hipMemsetAsync(ctx->d_count, 0, sizeof(unsigned long long), ctx->stream));
@@ -1252,6 +1256,24 @@ int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) {
       break;
     }
     if ((r = ssa_prepare_coeffs(ctx))) return r;
+    if (P.precision != UFE_FP64) {
+      // GMRES-IR to a relative residual, warm-started from the current velocities
+      ufe_linsolve_opts lo{};
+      lo.rel_tol = P.SSA_linear_rel_tol > 0.0 ? P.SSA_linear_rel_tol : 1e-6;
+      lo.max_sweeps = 1 << 30;
+      lo.precision = P.precision == UFE_MIXED ? UFE_MIXED : UFE_FP64;
+      lo.gmres_restart = P.SSA_gmres_restart > 0 ? P.SSA_gmres_restart : 30;
+      lo.precond_sweeps = P.SSA_precond_sweeps > 0 ? P.SSA_precond_sweeps : 4;
+      lo.orth_passes = 1;
+      ufe_linsolve_stats ls;
+      if ((r = ufe_ssa_linear_solve(ctx, &lo, &ls))) return r;
+      if (it <= 64) st->sweeps[it - 1] = ls.sweeps;
+      st->total_sweeps += ls.sweeps;
+      st->last_max_residual = ls.rel_residual;
+      st->sor_seconds += ls.seconds;
+      if (ls.status != UFE_OK) st->status = UFE_SOR_NOT_CONVERGED;
+      continue;
+    }
     SorFlags fl;
     if ((r = run_sor_reference(ctx, P.SSA_max_residual_UV, P.SSA_max_inner_loops, &fl, &st->sor_seconds))) return r;
     if (it <= 64) st->sweeps[it - 1] = fl.sweep;
@@ -1495,7 +1517,10 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
     const int ks = o->precond_sweeps > 0 ? o->precond_sweeps : 2;
     if ((r = residual_fp64(ctx, &rel, true))) return r;
     st->refinements = 1;
+    double prev = HUGE_VAL;
     while (rel > o->rel_tol && st->sweeps < o->max_sweeps) {
+      if (rel > 0.99 * prev) break;   // stagnated at the rounding floor of the residual
+      prev = rel;
       int mv = 0;
       const int passes = o->orth_passes > 0 ? o->orth_passes : 2;
       r = mixed ? gmres_cycle<float>(ctx, mr, ks, omega, passes, &st->sweeps, &mv)
@@ -1645,7 +1670,7 @@ static int adj_solve(ufe_ctx *ctx, double tol, int m, int ks, double omega, int 
   const ActiveRanges ar = active_ranges(ctx);
   double2 *V = ctx->adj_V, *Z = ctx->adj_Z, *w = ctx->adj_w, *rr = ctx->adj_r;
   int used = 0;
-  double rel = 0.0;
+  double rel = 0.0, prev = HUGE_VAL;
   std::vector<double> H1(65 * 64), Hs(65), sc(4);
   while (true) {
     apply_AT(ctx, mT, ar, ctx->adj_lam, w);
@@ -1661,6 +1686,8 @@ static int adj_solve(ufe_ctx *ctx, double tol, int m, int ks, double omega, int 
       return UFE_SOR_NOT_CONVERGED;
     }
     if (rel <= tol || gn == 0.0 || used >= max_sweeps) break;
+    if (rel > 0.99 * prev) break;   // stagnated at the rounding floor of the residual
+    prev = rel;
     k_normalize<double><<<nb, BLOCK, 0, ctx->stream>>>(n, rr, ctx->adj_nrm + 1, V, ctx->adj_nrm + 4);
     for (int j = 0; j < m; j++) {
       double2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;

@@ -42,8 +42,12 @@ class Step:
         self.reset()
 
     def reset(self):
+        """Back to the initial state: input fields, and the SSA at rest (the
+        velocities and the Picard viscosity that solve_SSA warm-starts from)."""
         for k, v in self.init.items():
             self.ctx.set(k, v)
+        for k in ("U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac", "N_AaAc"):
+            self.ctx.fill(k, 0.0)
 
     def run(self):
         c = self.ctx

@@ -36,6 +36,12 @@ MODULE ice_dynamics_module
     INTEGER  :: SSA_max_inner_loops = 10000
     LOGICAL  :: ssa_forced_zero = .FALSE.
     LOGICAL  :: zero_Hi_at_edges = .TRUE.
+    ! linear solves of solve_SSA: UFE_FP64 runs the reference SOR; UFE_MIXED and
+    ! UFE_FP64_KRYLOV solve each linearised system to SSA_linear_rel_tol by GMRES-IR
+    INTEGER  :: precision = 0
+    REAL(dp) :: SSA_linear_rel_tol = 1E-6_dp
+    INTEGER  :: SSA_gmres_restart = 30
+    INTEGER  :: SSA_precond_sweeps = 4
   END TYPE type_dynamics_config
 
   TYPE(type_dynamics_config), SAVE :: C
@@ -170,8 +176,11 @@ CONTAINS
     p%ssa_forced_zero = MERGE(1, 0, ncfg%ssa_forced_zero)
     p%zero_Hi_at_edges = MERGE(1, 0, ncfg%zero_Hi_at_edges)
     p%use_tau_c_override = 0
-    p%precision = UFE_FP64
+    p%precision = ncfg%precision
     p%sweep_batch = 0
+    p%SSA_linear_rel_tol = ncfg%SSA_linear_rel_tol
+    p%SSA_gmres_restart = ncfg%SSA_gmres_restart
+    p%SSA_precond_sweeps = ncfg%SSA_precond_sweeps
   END FUNCTION params_of
 
   ! ---- mesh set-up -----------------------------------------------------------

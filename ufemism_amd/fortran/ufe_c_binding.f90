@@ -12,7 +12,7 @@ MODULE ufe_c_binding
                                UFE_SSA_UNSTABLE = 3, UFE_COLOURING_IDENTIFY = 4, &
                                UFE_COLOURING_FAILED = 5, UFE_BAD_INPUT = 6, UFE_HIP_ERROR = 7, &
                                UFE_NOT_SUPPORTED = 8
-  INTEGER(C_INT), PARAMETER :: UFE_FP64 = 0, UFE_MIXED = 1
+  INTEGER(C_INT), PARAMETER :: UFE_FP64 = 0, UFE_MIXED = 1, UFE_FP64_KRYLOV = 2
 
   TYPE, BIND(C) :: ufe_mesh_desc
     INTEGER(C_INT32_T) :: nV, nTri, nC_mem, nAc
@@ -36,6 +36,8 @@ MODULE ufe_c_binding
     INTEGER(C_INT32_T) :: SSA_max_inner_loops
     INTEGER(C_INT32_T) :: use_analytical_GL_flux, ssa_forced_zero, zero_Hi_at_edges
     INTEGER(C_INT32_T) :: use_tau_c_override, precision, sweep_batch
+    REAL(C_DOUBLE)     :: SSA_linear_rel_tol
+    INTEGER(C_INT32_T) :: SSA_gmres_restart, SSA_precond_sweeps
   END TYPE ufe_params
 
   TYPE, BIND(C) :: ufe_ssa_stats
