@@ -78,6 +78,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-step", action="store_true", help="skip the mixed-vs-FP64 dynamics steps")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble")
+    ap.add_argument("--batch-n", type=int, default=64, help="members of the batched ensemble")
+    ap.add_argument("--batch-min", type=int, default=5000)
+    ap.add_argument("--batch-max", type=int, default=100_000)
+    ap.add_argument("--batch-picard", type=int, default=3)
     ap.add_argument("--step-picard", type=int, default=3, help="Picard iterations of those steps")
     ap.add_argument("--step-tol", type=float, default=1e-9, help="tighter tolerance of those steps")
     ap.add_argument("--adj-nv", type=int, default=200_000, help="vertices of the adjoint mesh")
@@ -97,6 +102,13 @@ def main():
     from ufemism_amd import abi, geometry
     from ufemism_amd.dynamics import Step
 
+    ens = None
+    if not args.no_batch:       # CPU-only mesh generation in forked workers, before any GPU context
+        from ufemism_amd import batch
+        tb = time.time()
+        ens = batch.ensemble(n=args.batch_n, nv_min=args.batch_min, nv_max=args.batch_max, seed=args.seed,
+                             picard=args.batch_picard, workers=min(16, os.cpu_count() or 1))
+        log(f"[batch] {len(ens)} meshes, {sum(w.mesh.nV for w in ens)} vertices, generated in {time.time() - tb:.1f}s")
     t0 = time.time()
     wl = geometry.marine(n_target=args.nv, seed=args.seed, picard=args.picard0)
     if args.omega > 0:
@@ -247,6 +259,8 @@ def main():
         out["precision_error"] = bench_precision(args, ctx, step, wl)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(args, ctx)
+    if not args.no_batch:
+        out["batch"] = bench_batch(args, ens)
     print(json.dumps(out), flush=True)
     ctx.close()
 
@@ -289,6 +303,68 @@ def bench_precision(args, ctx, step, wl):
         }
     ctx.set_params(wl.params, abi.UFE_FP64)
     return res
+
+
+def bench_batch(args, wls):
+    """Batched ensemble: independent SSA solves (K Picard steps each) on seeded
+    meshes with sizes log-uniform in [batch-min, batch-max], dome or marine,
+    A uniform in [1e-17, 1e-16], log-normal friction; every linearised system to
+    the relative residual --tol in mixed precision, one hipGraph launch per GMRES
+    cycle of all members, the sweep's launch shape autotuned; error per member
+    against the same batch in FP64."""
+    import numpy as np
+    from ufemism_amd import abi, batch
+
+    for wl in wls:
+        wl.params.SSA_RN_tol = 0.0
+    ctx = abi.Context(0)
+    t1 = time.time()
+    B = batch.Batch(ctx, wls)
+    log(f"[batch] union nVAaAc={B.nA} built and uploaded in {time.time() - t1:.1f}s")
+    B.reset()
+    ctx.update_general_ice_model_data()
+    ctx.ssa_prepare()
+    tune = ctx.batch_autotune(10)
+    cands = [{"threads": tune.cand_threads[i], "schedule": tune.cand_schedule[i],
+              "us_per_sweep": round(tune.cand_us_per_sweep[i], 2)} for i in range(tune.n_candidates)]
+    log(f"[batch] autotune: {tune.threads_per_block} threads, schedule {tune.schedule}; {cands}")
+
+    def run(prec, graph=True):
+        B.reset()
+        ctx.update_general_ice_model_data()
+        ctx.synchronize()
+        ta = time.perf_counter()
+        st, rel = ctx.batch_solve_SSA(args.batch_picard, rel_tol=args.tol, precision=prec,
+                                      gmres_restart=args.gmres, precond_sweeps=args.precond, use_graph=graph)
+        ctx.synchronize()
+        el = time.perf_counter() - ta
+        return st, rel, el, B.member_values("U_SSA_AaAc"), B.member_values("V_SSA_AaAc")
+
+    run(abi.UFE_MIXED)                                   # warm-up: capture and allocations
+    st, rel, el, U1, V1 = run(abi.UFE_MIXED)
+    log(f"[batch] mixed: {el:.3f}s cycles {st.cycles} sweeps/member {st.sweeps} converged {st.converged}")
+    st0, rel0, el0, U0, V0 = run(abi.UFE_FP64)
+    log(f"[batch] fp64: {el0:.3f}s cycles {st0.cycles} sweeps/member {st0.sweeps}")
+    errs = [float(np.sqrt((np.sum((U1[b] - U0[b]) ** 2) + np.sum((V1[b] - V0[b]) ** 2)) /
+                          max(np.sum(U0[b] ** 2 + V0[b] ** 2), 1e-300))) for b in range(len(wls))]
+    rows = B.nA
+    ctx.close()
+    return {
+        "workload": f"{len(wls)} independent SSA solves ({args.batch_picard} Picard steps each) on seeded "
+                    f"meshes of {args.batch_min}-{args.batch_max} vertices (log-uniform), dome or marine, "
+                    f"A ~ U[1e-17, 1e-16], log-normal friction; linear tolerance {args.tol:g}",
+        "members": len(wls), "vertices": int(sum(w.mesh.nV for w in wls)), "nVAaAc": int(rows),
+        "member_nV_min": int(min(w.mesh.nV for w in wls)), "member_nV_max": int(max(w.mesh.nV for w in wls)),
+        "seconds": round(el, 4), "fp64_seconds": round(el0, 4), "speedup_vs_fp64": round(el0 / el, 3),
+        "gmres_cycles": st.cycles, "sweeps_per_member": st.sweeps, "converged": st.converged,
+        "aggregate_Mvertex_sweeps_per_s": round(rows * st.sweeps / el / 1e6, 1),
+        "solves_per_s": round(len(wls) / el, 2),
+        "max_member_rel_residual": float(np.max(rel)),
+        "member_velocity_error_vs_fp64": {"max": max(errs), "median": float(np.median(errs))},
+        "tuning": {"threads_per_block": tune.threads_per_block, "schedule": tune.schedule,
+                   "candidates": cands},
+        "graph": "one hipGraph launch per GMRES cycle of all members",
+    }
 
 
 def bench_adjoint(args, ctx):

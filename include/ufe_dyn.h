@@ -255,6 +255,59 @@ int ufe_ssa_adjoint_forward(ufe_ctx *ctx, const ufe_adjoint_opts *opts, ufe_adjo
 int ufe_ssa_adjoint_backward(ufe_ctx *ctx, const double *dJ_dU, const double *dJ_dV,
                              double *grad_log_tau_c, double *grad_A, ufe_adjoint_stats *st);
 
+/* ---- batched ensembles --------------------------------------------------- */
+/* No counterpart in the reference.  Many independent SSA systems solved
+ * together on one GPU: the caller joins the member meshes into one mesh whose
+ * Aa vertices and Ac points come in member-major blocks (member 0's vertices,
+ * member 1's, ..., then member 0's Ac points, ...; connectivity, colour lists
+ * and index arrays renumbered accordingly; every member keeps its own corner
+ * vertices, edge_index 2/4/6/8), declares the member sizes with
+ * ufe_batch_define BEFORE ufe_mesh_upload, and uploads the union.  Each
+ * colour's SOR rows of all members then form one range, so one launch sweeps
+ * the colour in every member; reductions, Krylov coefficients and
+ * convergence are per member.  nmembers = 0 clears the declaration. */
+typedef struct ufe_batch_opts {
+  double rel_tol;                   /* per-member FP64 relative residual */
+  int32_t precision;                /* UFE_MIXED (FP32 sweeps and Krylov vectors) or UFE_FP64 */
+  int32_t gmres_restart;            /* GMRES(m), 1..64, every member */
+  int32_t precond_sweeps;           /* SOR sweeps per preconditioner application (<= 0: 4) */
+  double omega;                     /* <= 0: params.SSA_SOR_omega */
+  int32_t max_cycles;               /* GMRES cycles (all members in lockstep; <= 0: unbounded) */
+  int32_t use_graph;                /* 1: each cycle of all members is one hipGraph launch */
+} ufe_batch_opts;
+
+typedef struct ufe_batch_stats {
+  int32_t cycles;                   /* GMRES cycles run */
+  int32_t sweeps;                   /* SOR sweeps per member */
+  int32_t converged;                /* members at the tolerance */
+  double max_rel_residual;          /* worst member at exit */
+  double seconds;
+  int32_t status;                   /* UFE_OK or UFE_SOR_NOT_CONVERGED */
+} ufe_batch_stats;
+
+/* Result of the autotuner: threads per block and block order of the FP32
+ * colour sweep (schedule 0: XCD-major bands of rows, 1: round-robin). */
+typedef struct ufe_batch_tuning {
+  int32_t threads_per_block;
+  int32_t schedule;
+  int32_t n_candidates;
+  int32_t cand_threads[16];
+  int32_t cand_schedule[16];
+  double cand_us_per_sweep[16];
+} ufe_batch_tuning;
+
+int ufe_batch_define(ufe_ctx *ctx, int nmembers, const int32_t *nV, const int32_t *nAc);
+/* Time `sweeps` sweeps of the uploaded batch per launch shape, keep the fastest. */
+int ufe_batch_autotune(ufe_ctx *ctx, int sweeps, ufe_batch_tuning *out);
+/* Solve every member's current linearised system (after ufe_ssa_prepare) from
+ * the current velocities; member_rel (nmembers doubles, may be NULL) receives
+ * each member's final relative residual. */
+int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *opts, double *member_rel, ufe_batch_stats *st);
+/* picard_iterations Picard steps of solve_SSA (src/ice_dynamics_module.f90:
+ * 498-545) for every member, each linearised system solved as above. */
+int ufe_batch_solve_SSA(ufe_ctx *ctx, int picard_iterations, const ufe_batch_opts *opts, double *member_rel,
+                        ufe_batch_stats *st);
+
 /* ---- operators (device pointers, original index order) ----------------- */
 /* src/mesh_derivatives_module.f90:315-432, 485-597 */
 int ufe_op_derivatives_Aa(ufe_ctx *ctx, const double *d, double *ddx, double *ddy);

@@ -38,7 +38,9 @@ def test_field_table_matches_reference_names():
 STRUCTS = {"ufe_mesh_desc": abi.MeshDesc, "ufe_params": abi.Params, "ufe_ssa_stats": abi.SSAStats,
            "ufe_linsolve_opts": abi.LinSolveOpts, "ufe_linsolve_stats": abi.LinSolveStats,
            "ufe_kernel_profile": abi.KernelProfile,
-           "ufe_adjoint_opts": abi.AdjointOpts, "ufe_adjoint_stats": abi.AdjointStats}
+           "ufe_adjoint_opts": abi.AdjointOpts, "ufe_adjoint_stats": abi.AdjointStats,
+           "ufe_batch_opts": abi.BatchOpts, "ufe_batch_stats": abi.BatchStats,
+           "ufe_batch_tuning": abi.BatchTuning}
 
 
 def test_struct_layouts_match_header(tmp_path):

@@ -1,0 +1,94 @@
+"""Batched ensembles on the GPU: every member of a batch solve must match the
+same member solved alone, mixed precision must match FP64, hipGraph replays
+must equal eager launches bit for bit, and the autotuned launch shape must
+not change a single bit either (the shape only reorders independent rows)."""
+import numpy as np
+import pytest
+
+from ufemism_amd import abi, batch
+from ufemism_amd.dynamics import Step
+
+pytestmark = pytest.mark.gpu
+
+PICARD = 3
+TOL = 1e-10
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def ens():
+    print("[batch] building 6 members", flush=True)
+    wls = batch.ensemble(n=6, nv_min=1500, nv_max=6000, seed=7, picard=PICARD)
+    for wl in wls:
+        wl.params.SSA_RN_tol = 0.0           # exactly PICARD iterations everywhere
+    ctx = abi.Context(0)
+    B = batch.Batch(ctx, wls)
+    yield ctx, B, wls
+    ctx.close()
+
+
+def _solve(ctx, B, **kw):
+    B.reset()
+    ctx.update_general_ice_model_data()
+    st, rel = ctx.batch_solve_SSA(PICARD, rel_tol=TOL, **kw)
+    assert st.status == 0 and st.converged == len(B.members), (st.status, st.converged, rel)
+    return B.member_values("U_SSA_AaAc"), B.member_values("V_SSA_AaAc"), st
+
+
+def test_batch_members_match_single_solves(ens):
+    ctx, B, wls = ens
+    U, V, st = _solve(ctx, B, precision=abi.UFE_FP64)
+    print(f"  fp64 batch: cycles {st.cycles} sweeps {st.sweeps} {st.seconds:.3f}s", flush=True)
+    one = abi.Context(0)
+    try:
+        for b, wl in enumerate(wls):
+            step = Step(one, wl)
+            one.set_params(wl.params, abi.UFE_FP64_KRYLOV, linear_rel_tol=TOL)
+            one.update_general_ice_model_data()
+            s = one.solve_SSA()
+            assert s.picard_iterations == PICARD
+            u, v = one.get("U_SSA_AaAc"), one.get("V_SSA_AaAc")
+            e = max(_rel(U[b], u), _rel(V[b], v))
+            print(f"  member {b} nV={wl.mesh.nV}: batch vs alone rel L2 {e:.2e}", flush=True)
+            assert e <= 1e-7, (b, e)
+    finally:
+        one.close()
+
+
+def test_batch_mixed_matches_fp64(ens):
+    ctx, B, wls = ens
+    U0, V0, _ = _solve(ctx, B, precision=abi.UFE_FP64)
+    U1, V1, st = _solve(ctx, B, precision=abi.UFE_MIXED)
+    print(f"  mixed batch: cycles {st.cycles} sweeps {st.sweeps} {st.seconds:.3f}s", flush=True)
+    for b in range(len(wls)):
+        e = max(_rel(U1[b], U0[b]), _rel(V1[b], V0[b]))
+        print(f"  member {b}: mixed vs fp64 rel L2 {e:.2e}", flush=True)
+        assert e <= 1e-7, (b, e)
+
+
+def test_graph_replays_equal_eager_launches(ens):
+    ctx, B, wls = ens
+    Ug, Vg, sg = _solve(ctx, B, precision=abi.UFE_MIXED, use_graph=True)
+    Ue, Ve, se = _solve(ctx, B, precision=abi.UFE_MIXED, use_graph=False)
+    assert sg.cycles == se.cycles and sg.cycles > PICARD
+    for b in range(len(wls)):
+        assert np.array_equal(Ug[b], Ue[b]) and np.array_equal(Vg[b], Ve[b]), b
+
+
+def test_autotuned_shape_changes_no_bits(ens):
+    ctx, B, wls = ens
+    U0, V0, _ = _solve(ctx, B, precision=abi.UFE_MIXED)
+    B.reset()
+    ctx.update_general_ice_model_data()
+    ctx.ssa_prepare()
+    t = ctx.batch_autotune(5)
+    cands = [(t.cand_threads[i], t.cand_schedule[i], round(t.cand_us_per_sweep[i], 2))
+             for i in range(t.n_candidates)]
+    print(f"  autotune: chose {t.threads_per_block} threads, schedule {t.schedule}; {cands}", flush=True)
+    assert t.n_candidates == 8 and (t.threads_per_block, t.schedule) in [(c[0], c[1]) for c in cands]
+    U1, V1, _ = _solve(ctx, B, precision=abi.UFE_MIXED)
+    for b in range(len(wls)):
+        assert np.array_equal(U0[b], U1[b]) and np.array_equal(V0[b], V1[b]), b

@@ -43,6 +43,7 @@ EXPORTS = [
     "ufe_device_alloc", "ufe_device_free", "ufe_memcpy_h2d", "ufe_memcpy_d2h", "ufe_synchronize",
     "ufe_field_fill", "ufe_profile_enable", "ufe_profile_read", "ufe_sweep_bytes",
     "ufe_ssa_adjoint_forward", "ufe_ssa_adjoint_backward",
+    "ufe_batch_define", "ufe_batch_autotune", "ufe_batch_linear_solve", "ufe_batch_solve_SSA",
 ]
 
 _P = ct.c_void_p
@@ -111,6 +112,23 @@ class AdjointStats(ct.Structure):
                 ("max_forward_residual", ct.c_double), ("max_adjoint_residual", ct.c_double),
                 ("forward_seconds", ct.c_double), ("backward_seconds", ct.c_double),
                 ("status", ct.c_int32)]
+
+
+class BatchOpts(ct.Structure):
+    _fields_ = [("rel_tol", ct.c_double), ("precision", ct.c_int32), ("gmres_restart", ct.c_int32),
+                ("precond_sweeps", ct.c_int32), ("omega", ct.c_double), ("max_cycles", ct.c_int32),
+                ("use_graph", ct.c_int32)]
+
+
+class BatchStats(ct.Structure):
+    _fields_ = [("cycles", ct.c_int32), ("sweeps", ct.c_int32), ("converged", ct.c_int32),
+                ("max_rel_residual", ct.c_double), ("seconds", ct.c_double), ("status", ct.c_int32)]
+
+
+class BatchTuning(ct.Structure):
+    _fields_ = [("threads_per_block", ct.c_int32), ("schedule", ct.c_int32), ("n_candidates", ct.c_int32),
+                ("cand_threads", ct.c_int32 * 16), ("cand_schedule", ct.c_int32 * 16),
+                ("cand_us_per_sweep", ct.c_double * 16)]
 
 
 _lib = None
@@ -354,6 +372,40 @@ class Context:
             warnings.warn(f"ssa_adjoint_backward: an adjoint solve stopped at relative residual "
                           f"{st.max_adjoint_residual:.2e}", RuntimeWarning)
         return st
+
+    def batch_define(self, nV, nAc):
+        """Declare the member sizes of the union mesh uploaded next (empty: clear)."""
+        nV = np.ascontiguousarray(nV, dtype=np.int32)
+        nAc = np.ascontiguousarray(nAc, dtype=np.int32)
+        self._batch_sizes = (nV, nAc)
+        self._chk(self.lib.ufe_batch_define(self.h, int(len(nV)), _ptr(nV), _ptr(nAc)), "batch_define")
+
+    def batch_autotune(self, sweeps=20) -> BatchTuning:
+        t = BatchTuning()
+        self._chk(self.lib.ufe_batch_autotune(self.h, int(sweeps), ct.byref(t)), "batch_autotune")
+        return t
+
+    def _batch_opts(self, rel_tol, precision, gmres_restart, precond_sweeps, omega, max_cycles, use_graph):
+        return BatchOpts(float(rel_tol), int(precision), int(gmres_restart), int(precond_sweeps),
+                         float(omega), int(max_cycles), int(bool(use_graph)))
+
+    def batch_linear_solve(self, rel_tol=1e-6, precision=UFE_MIXED, gmres_restart=30, precond_sweeps=4,
+                           omega=0.0, max_cycles=0, use_graph=True):
+        o = self._batch_opts(rel_tol, precision, gmres_restart, precond_sweeps, omega, max_cycles, use_graph)
+        st = BatchStats()
+        rel = np.zeros(len(self._batch_sizes[0]))
+        self._chk(self.lib.ufe_batch_linear_solve(self.h, ct.byref(o), _ptr(rel), ct.byref(st)),
+                  "batch_linear_solve")
+        return st, rel
+
+    def batch_solve_SSA(self, picard_iterations, rel_tol=1e-6, precision=UFE_MIXED, gmres_restart=30,
+                        precond_sweeps=4, omega=0.0, max_cycles=0, use_graph=True):
+        o = self._batch_opts(rel_tol, precision, gmres_restart, precond_sweeps, omega, max_cycles, use_graph)
+        st = BatchStats()
+        rel = np.zeros(len(self._batch_sizes[0]))
+        self._chk(self.lib.ufe_batch_solve_SSA(self.h, int(picard_iterations), ct.byref(o), _ptr(rel),
+                                               ct.byref(st)), "batch_solve_SSA")
+        return st, rel
 
     def fill(self, name, value):
         fid = self.fields[name][0]

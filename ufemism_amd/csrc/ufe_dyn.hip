@@ -187,6 +187,25 @@ struct ufe_ctx {
   std::vector<double2 *> adj_ck;    // stored iterates u_0, u_c, u_2c, ...
   std::vector<double2 *> adj_seg;   // recomputed iterates of one segment
   ufe_linsolve_opts adj_lin{};
+  // batch: the uploaded mesh is a union of independent member meshes (Aa
+  // vertices and Ac points in member-major blocks); declared before upload
+  std::vector<int32_t> b_nV, b_nAc;  // per member (host, kept across uploads)
+  int nb = 0;                        // members of the uploaded mesh (0: single mesh)
+  uint16_t *d_mid = nullptr;         // member of each colour-order row
+  int n_chunk = 0;                   // SOR-row chunks, each inside one member
+  int32_t *d_ch_start = nullptr, *d_ch_count = nullptr, *d_ch_member = nullptr, *d_mch_ptr = nullptr;
+  std::vector<int32_t> h_mch_ptr;
+  std::vector<int64_t> b_rows;       // SOR rows per member
+  // batch Krylov work space and the captured cycle
+  int bk_m = 0, bk_ks = 0, bk_prec = -1, bk_bs = 0, bk_sched = -1;
+  double bk_omega = 0.0;
+  void *bk_V = nullptr, *bk_Z = nullptr, *bk_w = nullptr, *bk_x = nullptr, *bk_R = nullptr;
+  double *bk_part = nullptr, *bk_H = nullptr, *bk_Hs = nullptr, *bk_y = nullptr, *bk_nrm = nullptr,
+         *bk_scale = nullptr, *bk_rel = nullptr, *h_bk_rel = nullptr;
+  double bk_tol = 0.0;
+  hipGraphExec_t bk_graph = nullptr;
+  // autotuned SOR schedule of the batch: threads per block and block order
+  int tune_bs = 256, tune_sched = 0;
   SorFlags *h_flags = nullptr;      // pinned
   double *h_scalar = nullptr;       // pinned
   std::vector<void *> allocs;       // everything to free with the mesh
@@ -265,6 +284,18 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->adj_gs = ctx->adj_gtau = ctx->adj_gA = ctx->adj_H1 = ctx->adj_Hs = ctx->adj_y = ctx->adj_nrm = nullptr;
   ctx->adj_ck.clear();
   ctx->adj_seg.clear();
+  ctx->nb = 0;
+  ctx->d_mid = nullptr;
+  ctx->n_chunk = 0;
+  ctx->d_ch_start = ctx->d_ch_count = ctx->d_ch_member = ctx->d_mch_ptr = nullptr;
+  ctx->h_mch_ptr.clear();
+  ctx->b_rows.clear();
+  ctx->bk_m = ctx->bk_ks = ctx->bk_bs = 0;
+  ctx->bk_prec = ctx->bk_sched = -1;
+  ctx->bk_V = ctx->bk_Z = ctx->bk_w = ctx->bk_x = ctx->bk_R = nullptr;
+  ctx->bk_part = ctx->bk_H = ctx->bk_Hs = ctx->bk_y = ctx->bk_nrm = ctx->bk_scale = ctx->bk_rel = nullptr;
+  if (ctx->bk_graph) (void)hipGraphExecDestroy(ctx->bk_graph);
+  ctx->bk_graph = nullptr;
 }
 
 static size_t grid_len(const ufe_ctx *ctx, int grid) {
@@ -678,6 +709,25 @@ int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
       hkey[a] = hilbert_index(gx, gy);
     }
   }
+  // member of each AaAc point when a batch was declared for this mesh
+  std::vector<int32_t> member;
+  if (!ctx->b_nV.empty()) {
+    int64_t sv = 0, sa = 0;
+    for (size_t b = 0; b < ctx->b_nV.size(); b++) {
+      sv += ctx->b_nV[b];
+      sa += ctx->b_nAc[b];
+    }
+    if (sv != nV || sa != nAc) {
+      ctx->err = "the declared batch does not add up to the uploaded mesh";
+      return UFE_BAD_INPUT;
+    }
+    member.resize(nA);
+    int a = 0;
+    for (size_t b = 0; b < ctx->b_nV.size(); b++)
+      for (int k = 0; k < ctx->b_nV[b]; k++) member[a++] = (int32_t)b;
+    for (size_t b = 0; b < ctx->b_nAc.size(); b++)
+      for (int k = 0; k < ctx->b_nAc[b]; k++) member[a++] = (int32_t)b;
+  }
   std::vector<int32_t> perm;
   perm.reserve(nA);
   for (int c = 1; c <= 5; c++) {
@@ -691,7 +741,10 @@ int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
       }
       (is_edge(ai) ? edg : act).push_back(ai - 1);
     }
-    auto by_key = [&](int32_t a, int32_t b) { return hkey[a] < hkey[b] || (hkey[a] == hkey[b] && a < b); };
+    auto by_key = [&](int32_t a, int32_t b) {
+      if (!member.empty() && member[a] != member[b]) return member[a] < member[b];
+      return hkey[a] < hkey[b] || (hkey[a] == hkey[b] && a < b);
+    };
     std::sort(act.begin(), act.end(), by_key);
     perm.insert(perm.end(), act.begin(), act.end());
     perm.insert(perm.end(), edg.begin(), edg.end());
@@ -757,11 +810,23 @@ int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
     Sxy[p] = b;
     Syy[p] = c;
   }
-  // Neumann lists: edge points with index >= 5 (1-based) average their
-  // non-edge neighbours; corners 1..4 average all neighbours, afterwards.
+  // Neumann lists: edge points average their non-edge neighbours; the domain
+  // corners (vertices 1..4 of a reference mesh, the only points with the
+  // corner codes 2/4/6/8 in edge_index) then average all their neighbours.
+  auto is_corner = [&](int ai) {
+    if (ai > nV) return false;
+    const int e = m->edge_index[ai - 1];
+    return e == 2 || e == 4 || e == 6 || e == 8;
+  };
+  if (member.empty())
+    for (int ai = 1; ai <= nV; ai++)
+      if (is_corner(ai) != (ai <= 4)) {
+        ctx->err = "vertices 1..4 must be the domain corners (edge_index 2/4/6/8), and only they";
+        return UFE_BAD_INPUT;
+      }
   std::vector<int32_t> nrow, nptr{0}, ncol, crow, cptr2{0}, ccol;
-  for (int ai = 5; ai <= nA; ai++) {
-    if (!is_edge(ai)) continue;
+  for (int ai = 1; ai <= nA; ai++) {
+    if (!is_edge(ai) || is_corner(ai)) continue;
     nrow.push_back(iperm[ai - 1]);
     int cnt = 0;
     for (int s = 1; s <= m->nCAaAc[ai - 1]; s++) {
@@ -776,7 +841,8 @@ int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
     }
     nptr.push_back((int32_t)ncol.size());
   }
-  for (int ai = 1; ai <= 4; ai++) {
+  for (int ai = 1; ai <= nV; ai++) {
+    if (!is_corner(ai)) continue;
     crow.push_back(iperm[ai - 1]);
     for (int s = 1; s <= m->nCAaAc[ai - 1]; s++) ccol.push_back(iperm[at(m->CAaAc, ldAA, ai, s) - 1]);
     cptr2.push_back((int32_t)ccol.size());
@@ -916,6 +982,48 @@ int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
   if ((r = dalloc(ctx, &ctx->d_ViSMB, nV))) return r;
   if ((r = dalloc(ctx, &ctx->d_count, 4))) return r;
   if ((r = dalloc(ctx, &ctx->d_flags, 1))) return r;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  // --- batch tables: member of each row and SOR-row chunks inside one member
+  if (!member.empty()) {
+    const int nbm = (int)ctx->b_nV.size();
+    std::vector<uint16_t> mid(nA);
+    for (int p = 0; p < nA; p++) mid[p] = (uint16_t)member[perm[p]];
+    constexpr int CH = 4096;
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> per(nbm);
+    ctx->b_rows.assign(nbm, 0);
+    for (int c = 0; c < 5; c++) {
+      const int s0 = ctx->colour_start[c], e0 = s0 + ctx->colour_active[c];
+      for (int p = s0; p < e0;) {
+        const int b = mid[p];
+        int q = p;
+        while (q < e0 && mid[q] == b && q - p < CH) q++;
+        per[b].push_back({p, q - p});
+        ctx->b_rows[b] += q - p;
+        p = q;
+      }
+    }
+    std::vector<int32_t> cs, cc, cm, mptr{0};
+    for (int b = 0; b < nbm; b++) {
+      if (per[b].empty()) {
+        ctx->err = "a batch member has no SOR rows";
+        return UFE_BAD_INPUT;
+      }
+      for (auto &x : per[b]) {
+        cs.push_back(x.first);
+        cc.push_back(x.second);
+        cm.push_back(b);
+      }
+      mptr.push_back((int32_t)cs.size());
+    }
+    ctx->n_chunk = (int)cs.size();
+    ctx->h_mch_ptr = mptr;
+    if ((r = dupload(ctx, &ctx->d_mid, mid.data(), mid.size()))) return r;
+    if ((r = dupload(ctx, &ctx->d_ch_start, cs.data(), cs.size()))) return r;
+    if ((r = dupload(ctx, &ctx->d_ch_count, cc.data(), cc.size()))) return r;
+    if ((r = dupload(ctx, &ctx->d_ch_member, cm.data(), cm.size()))) return r;
+    if ((r = dupload(ctx, &ctx->d_mch_ptr, mptr.data(), mptr.size()))) return r;
+    ctx->nb = nbm;
+  }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->have_mesh = true;
   return UFE_OK;
@@ -1106,7 +1214,7 @@ static void enqueue_sweep_fp64(ufe_ctx *ctx, const AaAcMesh &m, double omega, do
   if (with_flags)
     k_sor_finalize<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0], ctx->d_flags, tol, max_sweeps);
   else
-    k_neumann_scalar_pair_corners<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
+    k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
 }
 
 static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, float2 *D = nullptr,
@@ -1125,7 +1233,7 @@ static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, flo
                  ctx->colour_start[c], n, omega);
   }
   if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D);
-  k_neumann_corners_f32<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), D);
+  k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), D);
 }
 
 // FP64 sweep on an arbitrary vector pair (Krylov preconditioner)
@@ -1139,7 +1247,7 @@ static void enqueue_sweep_fp64_vec(ufe_ctx *ctx, const AaAcMesh &m, double omega
                  (const double2 *)ctx->d_pair[1], R, ctx->colour_start[c], n, omega, (SorFlags *)nullptr);
   }
   if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D, nullptr);
-  k_neumann_scalar_pair_corners<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), D);
+  k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), D);
 }
 
 // solve_SSA_linearised sweep loop (:601-691): returns the flags at exit.
@@ -1431,7 +1539,6 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
   const ActiveRanges ar = active_ranges(ctx);
   T2 *V = (T2 *)ctx->kry_V, *Z = (T2 *)ctx->kry_Z, *w = (T2 *)ctx->kry_w, *x = (T2 *)ctx->kry_x;
   const int nb = nblk(n);
-  constexpr int KMAX = 64;
   if (f32)
     k_r32_to_v<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_R32, ctx->d_scalar + 1, (float2 *)V);
   else
@@ -1512,7 +1619,7 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
     // GMRES works on the Neumann-consistent subspace (edge rows follow the SOR
     // rows), which the corrections keep: start the velocities there as well
     if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], nullptr);
-    k_neumann_scalar_pair_corners<<<1, 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
+    k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
     if ((r = krylov_alloc(ctx, mr, mixed))) return r;
     const int ks = o->precond_sweeps > 0 ? o->precond_sweeps : 2;
     if ((r = residual_fp64(ctx, &rel, true))) return r;
@@ -1849,6 +1956,341 @@ int ufe_ssa_adjoint_backward(ufe_ctx *ctx, const double *dJ_dU, const double *dJ
   st->picard_iterations = K;
   st->backward_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st->status;
+}
+
+
+// ---- batched ensembles -------------------------------------------------------
+// (no reference counterpart: many independent linearised SSA systems, each
+// src/ice_dynamics_module.f90:558-694, solved together)
+
+int ufe_batch_define(ufe_ctx *ctx, int nmembers, const int32_t *nV, const int32_t *nAc) {
+  if (!ctx || nmembers < 0 || nmembers > 4096 || (nmembers > 0 && (!nV || !nAc))) return UFE_BAD_INPUT;
+  ctx->b_nV.clear();
+  ctx->b_nAc.clear();
+  for (int b = 0; b < nmembers; b++) {
+    if (nV[b] < 5 || nAc[b] < 1) {
+      ctx->err = "every batch member needs at least 5 vertices and 1 Ac point";
+      return UFE_BAD_INPUT;
+    }
+    ctx->b_nV.push_back(nV[b]);
+    ctx->b_nAc.push_back(nAc[b]);
+  }
+  return UFE_OK;
+}
+
+static Chunks chunk_list(ufe_ctx *c) {
+  Chunks ch;
+  ch.n = c->n_chunk;
+  ch.start = c->d_ch_start;
+  ch.count = c->d_ch_count;
+  ch.member = c->d_ch_member;
+  return ch;
+}
+
+static int batch_alloc(ufe_ctx *ctx, int m, bool f32) {
+  const int prec = f32 ? 1 : 0;
+  if (ctx->bk_V && ctx->bk_m >= m && ctx->bk_prec == prec) return UFE_OK;
+  const size_t n = ctx->nA, es = f32 ? sizeof(float2) : sizeof(double2);
+  const size_t nb = ctx->nb;
+  char *V, *Z, *w, *x, *R;
+  int r;
+  if ((r = dalloc(ctx, &V, es * n * (m + 1)))) return r;
+  if ((r = dalloc(ctx, &Z, es * n * m))) return r;
+  if ((r = dalloc(ctx, &w, es * n))) return r;
+  if ((r = dalloc(ctx, &x, es * n))) return r;
+  if ((r = dalloc(ctx, &R, es * n))) return r;
+  ctx->bk_V = V;
+  ctx->bk_Z = Z;
+  ctx->bk_w = w;
+  ctx->bk_x = x;
+  ctx->bk_R = R;
+  // zero the residual (its edge rows are never written) and the basis
+  const int64_t nv = (int64_t)n * (m + 1);
+  if (f32) {
+    k_fill<float2><<<nblk(n), BLOCK, 0, ctx->stream>>>((float2 *)R, (int64_t)n, make_float2(0.0f, 0.0f));
+    k_fill<float2><<<nblk(nv), BLOCK, 0, ctx->stream>>>((float2 *)V, nv, make_float2(0.0f, 0.0f));
+  } else {
+    k_fill<double2><<<nblk(n), BLOCK, 0, ctx->stream>>>((double2 *)R, (int64_t)n, make_double2(0.0, 0.0));
+    k_fill<double2><<<nblk(nv), BLOCK, 0, ctx->stream>>>((double2 *)V, nv, make_double2(0.0, 0.0));
+  }
+  LAUNCHCHK(ctx);
+  if (!ctx->bk_part) {
+    if ((r = dalloc(ctx, &ctx->bk_part, (size_t)64 * ctx->n_chunk))) return r;
+    if ((r = dalloc(ctx, &ctx->bk_H, nb * BHS))) return r;
+    if ((r = dalloc(ctx, &ctx->bk_Hs, nb * BKS))) return r;
+    if ((r = dalloc(ctx, &ctx->bk_y, nb * BKS))) return r;
+    if ((r = dalloc(ctx, &ctx->bk_nrm, nb * BKS))) return r;
+    if ((r = dalloc(ctx, &ctx->bk_scale, 2 * nb))) return r;
+    if ((r = dalloc(ctx, &ctx->bk_rel, nb))) return r;
+    if (!ctx->h_bk_rel) HIPCHK(ctx, hipHostMalloc((void **)&ctx->h_bk_rel, sizeof(double) * 4096));
+  }
+  ctx->bk_m = m;
+  ctx->bk_prec = prec;
+  if (ctx->bk_graph) (void)hipGraphExecDestroy(ctx->bk_graph);
+  ctx->bk_graph = nullptr;
+  return UFE_OK;
+}
+
+// FP32 preconditioner sweep with the tuned launch shape (all members at once)
+extern "C++" template <int BS, int SCHED>
+static void sweep_f32_shape(ufe_ctx *ctx, const AaAcMesh &m, const Mesh32 &q, float omega, float2 *D, const float2 *R,
+                            bool backward) {
+  for (int k = 0; k < 5; k++) {
+    const int c = backward ? 4 - k : k;
+    const int n = ctx->colour_active[c];
+    if (n == 0) continue;
+    int g = (n + BS - 1) / BS;
+    if (SCHED == 0) g = (g + 7) / 8 * 8;
+    k_sor_colour_f32_tuned<BS, SCHED><<<g, BS, 0, ctx->stream>>>(m, q, D, R, ctx->colour_start[c], n, omega);
+  }
+}
+static void enqueue_sweep_fp32_tuned(ufe_ctx *ctx, const AaAcMesh &m, float omega, float2 *D, const float2 *R,
+                                     bool backward, int bs, int sched) {
+  Mesh32 q;
+  q.cU = ctx->d_cU32;
+  q.Sxy = ctx->d_Sxy32;
+  q.EV = ctx->d_EV32;
+  switch (bs * 2 + sched) {
+    case 64 * 2 + 0: sweep_f32_shape<64, 0>(ctx, m, q, omega, D, R, backward); break;
+    case 64 * 2 + 1: sweep_f32_shape<64, 1>(ctx, m, q, omega, D, R, backward); break;
+    case 128 * 2 + 0: sweep_f32_shape<128, 0>(ctx, m, q, omega, D, R, backward); break;
+    case 128 * 2 + 1: sweep_f32_shape<128, 1>(ctx, m, q, omega, D, R, backward); break;
+    case 512 * 2 + 0: sweep_f32_shape<512, 0>(ctx, m, q, omega, D, R, backward); break;
+    case 512 * 2 + 1: sweep_f32_shape<512, 1>(ctx, m, q, omega, D, R, backward); break;
+    case 256 * 2 + 1: sweep_f32_shape<256, 1>(ctx, m, q, omega, D, R, backward); break;
+    default: sweep_f32_shape<256, 0>(ctx, m, q, omega, D, R, backward); break;
+  }
+  if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D);
+  k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), D);
+}
+
+// residual of every member, its relative norm, and the scale of v0 = r/|r|
+extern "C++" template <typename T>
+static void batch_residual(ufe_ctx *ctx, double tol) {
+  using T2 = typename V2T<T>::t;
+  const AaAcMesh am = aaac_mesh(ctx);
+  k_bresidual<T><<<ctx->n_chunk, BLOCK, 0, ctx->stream>>>(am, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
+                                                         chunk_list(ctx), (T2 *)ctx->bk_R, ctx->bk_part);
+  k_bdot_reduce<<<dim3(2, ctx->nb), 256, 0, ctx->stream>>>(ctx->d_mch_ptr, 2, ctx->bk_part, ctx->bk_nrm, BKS);
+  k_bres_finish<<<nblk(ctx->nb), BLOCK, 0, ctx->stream>>>(ctx->nb, ctx->bk_nrm, tol, ctx->bk_rel,
+                                                          ctx->bk_scale + ctx->nb, ctx->bk_scale);
+}
+
+// one GMRES(m) cycle of every member, then w += correction and the new residual
+extern "C++" template <typename T>
+static void batch_enqueue_cycle(ufe_ctx *ctx, int m, int ks, double omega, double tol) {
+  using T2 = typename V2T<T>::t;
+  const bool f32 = sizeof(T) == 4;
+  const int n = ctx->nA, nb = nblk(n), NB = ctx->nb;
+  const AaAcMesh am = aaac_mesh(ctx);
+  const ActiveRanges ar = active_ranges(ctx);
+  const Chunks ch = chunk_list(ctx);
+  T2 *V = (T2 *)ctx->bk_V, *Z = (T2 *)ctx->bk_Z, *w = (T2 *)ctx->bk_w, *x = (T2 *)ctx->bk_x;
+  double *beta = ctx->bk_scale + NB;
+  k_bscale<T, T><<<nb, BLOCK, 0, ctx->stream>>>(n, (const T2 *)ctx->bk_R, ctx->bk_scale, ctx->d_mid, V);
+  for (int j = 0; j < m; j++) {
+    T2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;
+    k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
+    for (int k = 0; k < ks; k++) {
+      if (f32) enqueue_sweep_fp32_tuned(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1,
+                                        ctx->tune_bs, ctx->tune_sched);
+      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1);
+    }
+    if (f32)
+      k_matvec<float><<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32, ar,
+                                                     (const float2 *)zj, (float2 *)w);
+    else
+      k_matvec<double><<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_cU, ctx->d_Sxy, ctx->d_pair[1], ar,
+                                                      (const double2 *)zj, (double2 *)w);
+    double *hj = ctx->bk_H + (int64_t)j * 65;
+    k_bdot<T><<<ctx->n_chunk, BLOCK, 0, ctx->stream>>>(ch, n, j + 1, V, w, ctx->bk_part);
+    k_bdot_reduce<<<dim3(j + 1, NB), 256, 0, ctx->stream>>>(ctx->d_mch_ptr, j + 1, ctx->bk_part, hj, BHS);
+    k_baxpy<T><<<nb, BLOCK, 0, ctx->stream>>>(n, j + 1, V, hj, BHS, ctx->d_mid, w);
+    k_bdot<T><<<ctx->n_chunk, BLOCK, 0, ctx->stream>>>(ch, n, 1, w, w, ctx->bk_part);
+    k_bdot_reduce<<<dim3(1, NB), 256, 0, ctx->stream>>>(ctx->d_mch_ptr, 1, ctx->bk_part, ctx->bk_nrm, BKS);
+    k_bnorm_finish<<<nblk(NB), BLOCK, 0, ctx->stream>>>(NB, ctx->bk_nrm, ctx->bk_Hs + j, BKS, ctx->bk_rel, tol,
+                                                        ctx->bk_scale);
+    k_bscale<T, T><<<nb, BLOCK, 0, ctx->stream>>>(n, w, ctx->bk_scale, ctx->d_mid, V + (int64_t)(j + 1) * n);
+  }
+  k_blsq<<<NB, 64, 0, ctx->stream>>>(m, ctx->bk_H, ctx->bk_Hs, beta, ctx->bk_y);
+  k_blincomb<T><<<nb, BLOCK, 0, ctx->stream>>>(n, m, Z, ctx->bk_y, ctx->d_mid, x);
+  k_add_to_w<T><<<nb, BLOCK, 0, ctx->stream>>>(n, x, ctx->d_pair[0]);
+  batch_residual<T>(ctx, tol);
+}
+
+static int batch_capture(ufe_ctx *ctx, bool f32, int m, int ks, double omega, double tol) {
+  if (ctx->bk_graph && ctx->bk_ks == ks && ctx->bk_omega == omega && ctx->bk_tol == tol &&
+      ctx->bk_bs == ctx->tune_bs && ctx->bk_sched == ctx->tune_sched && ctx->bk_m == m)
+    return UFE_OK;
+  if (ctx->bk_graph) (void)hipGraphExecDestroy(ctx->bk_graph);
+  ctx->bk_graph = nullptr;
+  const bool prof = ctx->prof;
+  ctx->prof = false;              // no event records inside the captured graph
+  hipGraph_t g = nullptr;
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  if (f32) batch_enqueue_cycle<float>(ctx, m, ks, omega, tol);
+  else batch_enqueue_cycle<double>(ctx, m, ks, omega, tol);
+  const hipError_t e1 = hipGetLastError();
+  const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+  ctx->prof = prof;
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    ctx->err = std::string("graph capture: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2);
+    if (g) (void)hipGraphDestroy(g);
+    return UFE_HIP_ERROR;
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e3 != hipSuccess) {
+    ctx->err = std::string("graph instantiate: ") + hipGetErrorString(e3);
+    return UFE_HIP_ERROR;
+  }
+  ctx->bk_graph = ex;
+  ctx->bk_ks = ks;
+  ctx->bk_omega = omega;
+  ctx->bk_tol = tol;
+  ctx->bk_bs = ctx->tune_bs;
+  ctx->bk_sched = ctx->tune_sched;
+  return UFE_OK;
+}
+
+int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *o, double *member_rel, ufe_batch_stats *st) {
+  if (!ctx || !ctx->have_mesh || !o || !st) return UFE_BAD_INPUT;
+  std::memset(st, 0, sizeof(*st));
+  if (ctx->nb == 0) {
+    ctx->err = "no batch: call ufe_batch_define before ufe_mesh_upload";
+    return UFE_BAD_INPUT;
+  }
+  if (o->gmres_restart < 1 || o->gmres_restart > 64 || !(o->rel_tol > 0.0) ||
+      (o->precision != UFE_MIXED && o->precision != UFE_FP64)) {
+    ctx->err = "ufe_batch_linear_solve: gmres_restart 1..64, rel_tol > 0, precision UFE_MIXED or UFE_FP64";
+    return UFE_BAD_INPUT;
+  }
+  const bool f32 = o->precision == UFE_MIXED;
+  const int m = o->gmres_restart, ks = o->precond_sweeps > 0 ? o->precond_sweeps : 4;
+  const double omega = o->omega > 0.0 ? o->omega : ctx->params.SSA_SOR_omega, tol = o->rel_tol;
+  const int NB = ctx->nb;
+  int r;
+  if ((r = batch_alloc(ctx, m, f32))) return r;
+  auto t0 = std::chrono::steady_clock::now();
+  if (f32) k_to_f32_pair<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_pair[1], ctx->d_EV32);
+  if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], nullptr);
+  k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
+  if (f32) batch_residual<float>(ctx, tol);
+  else batch_residual<double>(ctx, tol);
+  LAUNCHCHK(ctx);
+  if (o->use_graph && (r = batch_capture(ctx, f32, m, ks, omega, tol))) return r;
+  const int max_cycles = o->max_cycles > 0 ? o->max_cycles : 100000;
+  auto all_done = [&](double *worst, int *conv) {
+    *worst = 0.0;
+    *conv = 0;
+    for (int b = 0; b < NB; b++) {
+      *worst = std::max(*worst, ctx->h_bk_rel[b]);
+      *conv += ctx->h_bk_rel[b] <= tol;
+    }
+    return *conv == NB;
+  };
+  double worst = 0.0;
+  int conv = 0;
+  while (true) {
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_bk_rel, ctx->bk_rel, sizeof(double) * NB, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (all_done(&worst, &conv) || st->cycles >= max_cycles || !std::isfinite(worst)) break;
+    if (o->use_graph) {
+      HIPCHK(ctx, hipGraphLaunch(ctx->bk_graph, ctx->stream));
+    } else {
+      if (f32) batch_enqueue_cycle<float>(ctx, m, ks, omega, tol);
+      else batch_enqueue_cycle<double>(ctx, m, ks, omega, tol);
+      LAUNCHCHK(ctx);
+    }
+    st->cycles++;
+    st->sweeps += m * ks;
+  }
+  st->converged = conv;
+  st->max_rel_residual = worst;
+  st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  st->status = conv == NB ? UFE_OK : UFE_SOR_NOT_CONVERGED;
+  if (member_rel) std::memcpy(member_rel, ctx->h_bk_rel, sizeof(double) * NB);
+  return UFE_OK;
+}
+
+int ufe_batch_solve_SSA(ufe_ctx *ctx, int picard_iterations, const ufe_batch_opts *o, double *member_rel,
+                        ufe_batch_stats *st) {
+  if (!ctx || !ctx->have_mesh || !o || !st || picard_iterations < 1) return UFE_BAD_INPUT;
+  if (ctx->nb == 0) {
+    ctx->err = "no batch: call ufe_batch_define before ufe_mesh_upload";
+    return UFE_BAD_INPUT;
+  }
+  ufe_batch_stats tot{};
+  auto t0 = std::chrono::steady_clock::now();
+  int r;
+  if ((r = ssa_gather(ctx))) return r;
+  for (int k = 0; k < picard_iterations; k++) {
+    if ((r = ssa_viscosity_rn(ctx, nullptr))) return r;
+    if ((r = ssa_prepare_coeffs(ctx))) return r;
+    ufe_batch_stats s1;
+    if ((r = ufe_batch_linear_solve(ctx, o, member_rel, &s1))) return r;
+    tot.cycles += s1.cycles;
+    tot.sweeps += s1.sweeps;
+    tot.converged = s1.converged;
+    tot.max_rel_residual = s1.max_rel_residual;
+    if (s1.status != UFE_OK) tot.status = s1.status;
+  }
+  if ((r = ssa_finish(ctx))) return r;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  tot.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *st = tot;
+  return UFE_OK;
+}
+
+// Time `sweeps` FP32 preconditioner sweeps (colour passes plus Neumann) of the
+// whole batch for each launch shape and keep the fastest.
+int ufe_batch_autotune(ufe_ctx *ctx, int sweeps, ufe_batch_tuning *out) {
+  if (!ctx || !ctx->have_mesh || !out || sweeps < 1) return UFE_BAD_INPUT;
+  std::memset(out, 0, sizeof(*out));
+  const AaAcMesh am = aaac_mesh(ctx);
+  k_to_f32_pair<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_pair[1], ctx->d_EV32);
+  LAUNCHCHK(ctx);
+  const int bss[4] = {64, 128, 256, 512};
+  hipEvent_t a, b;
+  HIPCHK(ctx, hipEventCreate(&a));
+  HIPCHK(ctx, hipEventCreate(&b));
+  double best = HUGE_VAL;
+  int nbest = 256, sbest = 0, k = 0;
+  for (int bs : bss)
+    for (int sched = 0; sched < 2; sched++) {
+      enqueue_sweep_fp32_tuned(ctx, am, 1.0f, ctx->d_D32, ctx->d_R32, false, bs, sched);  // warm-up
+      (void)hipEventRecord(a, ctx->stream);
+      for (int i = 0; i < sweeps; i++)
+        enqueue_sweep_fp32_tuned(ctx, am, 1.0f, ctx->d_D32, ctx->d_R32, i & 1, bs, sched);
+      (void)hipEventRecord(b, ctx->stream);
+      LAUNCHCHK(ctx);
+      HIPCHK(ctx, hipEventSynchronize(b));
+      float ms = 0.0f;
+      HIPCHK(ctx, hipEventElapsedTime(&ms, a, b));
+      const double us = 1e3 * ms / sweeps;
+      out->cand_threads[k] = bs;
+      out->cand_schedule[k] = sched;
+      out->cand_us_per_sweep[k] = us;
+      k++;
+      if (us < best) {
+        best = us;
+        nbest = bs;
+        sbest = sched;
+      }
+    }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  // the sweeps above ran on scratch vectors: clear the correction again
+  k_fill<float2><<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->d_D32, ctx->nA, make_float2(0.0f, 0.0f));
+  LAUNCHCHK(ctx);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  out->n_candidates = k;
+  out->threads_per_block = nbest;
+  out->schedule = sbest;
+  ctx->tune_bs = nbest;
+  ctx->tune_sched = sbest;
+  return UFE_OK;
 }
 
 int ufe_profile_enable(ufe_ctx *ctx, int enable) {
