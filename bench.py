@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--gmres", type=int, default=30, help="GMRES restart length")
     ap.add_argument("--precond", type=int, default=4, help="SOR sweeps per preconditioner")
     ap.add_argument("--orth-passes", type=int, default=1, help="Gram-Schmidt passes (1 or 2)")
+    ap.add_argument("--krylov", default="gmres", choices=["gmres", "bicgstab"],
+                    help="Krylov method of the mixed_gmres / fp64_gmres paths")
     ap.add_argument("--mode", default="both",
                     choices=["both", "mixed_gmres", "fp64_gmres", "fp64_sor"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -79,6 +81,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-step", action="store_true", help="skip the mixed-vs-FP64 dynamics steps")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default sweep launch shape")
     ap.add_argument("--batch-n", type=int, default=64, help="members of the batched ensemble")
     ap.add_argument("--batch-min", type=int, default=5000)
     ap.add_argument("--batch-max", type=int, default=100_000)
@@ -127,6 +130,12 @@ def main():
     log(f"[bench] reference SSA ({st0.picard_iterations} Picard, sweeps "
         f"{list(st0.sweeps)[:st0.picard_iterations]}) {ref_ssa_s:.2f}s")
     ctx.ssa_prepare()
+    tuning = None
+    if not args.no_tune:
+        t = ctx.sweep_autotune(10)
+        tuning = {"threads_per_block": t.threads_per_block, "schedule": t.schedule, "loads": t.loads,
+                  "candidates": t.candidates()}
+        log(f"[bench] sweep autotune: {t.threads_per_block} threads, schedule {t.schedule}, loads {t.loads}")
     nA = sizes["nVAaAc"]
     sb = ctx.sweep_bytes()
     omega = wl.params.SSA_SOR_omega
@@ -141,7 +150,8 @@ def main():
         prec = abi.UFE_MIXED if path == "mixed_gmres" else abi.UFE_FP64
         return ctx.ssa_linear_solve(args.tol, args.max_sweeps, precision=prec, omega=omega_mixed,
                                     gmres=args.gmres, precond_sweeps=args.precond,
-                                    orth_passes=args.orth_passes)
+                                    orth_passes=args.orth_passes,
+                                    method=1 if args.krylov == "bicgstab" else 0)
 
     def run(path, steps, warmup):
         for _ in range(warmup):
@@ -192,6 +202,8 @@ def main():
         log("[bench] the reference SOR path did not converge; value falls back to the raw sweep rate")
     R = paths[head]
     kname = {"mixed_gmres": "k_sor_colour_f32"}.get(head, "k_sor_colour")
+    if head == "mixed_gmres" and tuning:
+        kname = f"k_sor_colour_f32_tuned<{tuning['threads_per_block']},{tuning['schedule']},{tuning['loads']}>"
     out = {
         "metric": METRIC,
         "value": round(R["raw_Mvertex_sweeps_per_s"] if R_value_fallback else R["effective_Mvertex_sweeps_per_s"], 3),
@@ -206,7 +218,7 @@ def main():
                         f"FP64 relative residual {args.tol:g}",
             "nV": sizes["nV"], "nVAaAc": nA, "mean_degree": round(sb["mean_degree"], 4),
             "path": head, "omega": omega, "omega_inner": omega_mixed,
-            "gmres_restart": args.gmres, "precond_sweeps": args.precond,
+            "krylov": args.krylov, "gmres_restart": args.gmres, "precond_sweeps": args.precond,
             "orth_passes": args.orth_passes,
             "value_definition": "nVAaAc * (FP64 reference-SOR sweeps to the tolerance) / "
                                 "(this path's seconds to the tolerance) / 1e6",
@@ -221,6 +233,7 @@ def main():
                               "copy (read x2.0, write x1.0); null when not measured on this mesh size",
         },
         "paths": paths,
+        "sweep_tuning": tuning,
         "reference_ssa_seconds": round(ref_ssa_s, 3),
     }
     if sor_ok:
@@ -324,10 +337,9 @@ def bench_batch(args, wls):
     B.reset()
     ctx.update_general_ice_model_data()
     ctx.ssa_prepare()
-    tune = ctx.batch_autotune(10)
-    cands = [{"threads": tune.cand_threads[i], "schedule": tune.cand_schedule[i],
-              "us_per_sweep": round(tune.cand_us_per_sweep[i], 2)} for i in range(tune.n_candidates)]
-    log(f"[batch] autotune: {tune.threads_per_block} threads, schedule {tune.schedule}; {cands}")
+    tune = ctx.sweep_autotune(10)
+    cands = tune.candidates()
+    log(f"[batch] autotune: {tune.threads_per_block} threads, schedule {tune.schedule}, loads {tune.loads}")
 
     def run(prec, graph=True):
         B.reset()
@@ -362,7 +374,7 @@ def bench_batch(args, wls):
         "max_member_rel_residual": float(np.max(rel)),
         "member_velocity_error_vs_fp64": {"max": max(errs), "median": float(np.median(errs))},
         "tuning": {"threads_per_block": tune.threads_per_block, "schedule": tune.schedule,
-                   "candidates": cands},
+                   "loads": tune.loads, "candidates": cands},
         "graph": "one hipGraph launch per GMRES cycle of all members",
     }
 

@@ -132,6 +132,9 @@ typedef struct ufe_linsolve_opts {
                                        by precond_sweeps SOR sweeps, one FP64 refinement per cycle */
   int32_t precond_sweeps;           /* SOR sweeps per preconditioner application (<= 0: 2) */
   int32_t orth_passes;              /* classical Gram-Schmidt passes, 1 or 2 (<= 0: 2) */
+  int32_t method;                   /* 0: GMRES(gmres_restart) (plain SOR if that is 0);
+                                       1: BiCGStab preconditioned by precond_sweeps SOR sweeps,
+                                       scalars kept on the device, FP64 refinement around it */
 } ufe_linsolve_opts;
 
 /* Statistics of one linearised solve. */
@@ -255,6 +258,28 @@ int ufe_ssa_adjoint_forward(ufe_ctx *ctx, const ufe_adjoint_opts *opts, ufe_adjo
 int ufe_ssa_adjoint_backward(ufe_ctx *ctx, const double *dJ_dU, const double *dJ_dV,
                              double *grad_log_tau_c, double *grad_A, ufe_adjoint_stats *st);
 
+/* ---- autotuned launch shape of the FP32 sweep ----------------------------- */
+/* No counterpart in the reference.  The FP32 colour pass has 24 launch
+ * shapes: threads per block 64/128/256/512, block order (schedule 0: XCD-major
+ * bands of rows, 1: round-robin) and load variant (0: plain, 1: non-temporal
+ * loads for the once-streamed coefficients, 2: as 1 with eight slots' loads in
+ * flight).  All give identical results.  The autotuner times `sweeps` sweeps
+ * of the uploaded mesh (single or batch) per shape and keeps the fastest for
+ * every later FP32 sweep of the context (default: 256, 0, 0). */
+typedef struct ufe_sweep_tuning {
+  int32_t threads_per_block;
+  int32_t schedule;
+  int32_t loads;
+  int32_t n_candidates;
+  int32_t cand_threads[32];
+  int32_t cand_schedule[32];
+  int32_t cand_loads[32];
+  double cand_us_per_sweep[32];
+} ufe_sweep_tuning;
+
+int ufe_sweep_autotune(ufe_ctx *ctx, int sweeps, ufe_sweep_tuning *out);
+int ufe_sweep_tuning_set(ufe_ctx *ctx, int threads_per_block, int schedule, int loads);
+
 /* ---- batched ensembles --------------------------------------------------- */
 /* No counterpart in the reference.  Many independent SSA systems solved
  * together on one GPU: the caller joins the member meshes into one mesh whose
@@ -285,20 +310,7 @@ typedef struct ufe_batch_stats {
   int32_t status;                   /* UFE_OK or UFE_SOR_NOT_CONVERGED */
 } ufe_batch_stats;
 
-/* Result of the autotuner: threads per block and block order of the FP32
- * colour sweep (schedule 0: XCD-major bands of rows, 1: round-robin). */
-typedef struct ufe_batch_tuning {
-  int32_t threads_per_block;
-  int32_t schedule;
-  int32_t n_candidates;
-  int32_t cand_threads[16];
-  int32_t cand_schedule[16];
-  double cand_us_per_sweep[16];
-} ufe_batch_tuning;
-
 int ufe_batch_define(ufe_ctx *ctx, int nmembers, const int32_t *nV, const int32_t *nAc);
-/* Time `sweeps` sweeps of the uploaded batch per launch shape, keep the fastest. */
-int ufe_batch_autotune(ufe_ctx *ctx, int sweeps, ufe_batch_tuning *out);
 /* Solve every member's current linearised system (after ufe_ssa_prepare) from
  * the current velocities; member_rel (nmembers doubles, may be NULL) receives
  * each member's final relative residual. */

@@ -84,11 +84,20 @@ def test_autotuned_shape_changes_no_bits(ens):
     B.reset()
     ctx.update_general_ice_model_data()
     ctx.ssa_prepare()
-    t = ctx.batch_autotune(5)
-    cands = [(t.cand_threads[i], t.cand_schedule[i], round(t.cand_us_per_sweep[i], 2))
-             for i in range(t.n_candidates)]
-    print(f"  autotune: chose {t.threads_per_block} threads, schedule {t.schedule}; {cands}", flush=True)
-    assert t.n_candidates == 8 and (t.threads_per_block, t.schedule) in [(c[0], c[1]) for c in cands]
+    t = ctx.sweep_autotune(5)
+    cands = t.candidates()
+    print(f"  autotune: chose {t.threads_per_block} threads, schedule {t.schedule}, loads {t.loads}; {cands}",
+          flush=True)
+    assert t.n_candidates == 24
+    assert {"threads": t.threads_per_block, "schedule": t.schedule, "loads": t.loads} in \
+        [{k: c[k] for k in ("threads", "schedule", "loads")} for c in cands]
     U1, V1, _ = _solve(ctx, B, precision=abi.UFE_MIXED)
     for b in range(len(wls)):
         assert np.array_equal(U0[b], U1[b]) and np.array_equal(V0[b], V1[b]), b
+    # every other shape gives the same bits too
+    for shape in ((64, 1, 2), (512, 0, 1)):
+        ctx.sweep_tuning_set(*shape)
+        U2, V2, _ = _solve(ctx, B, precision=abi.UFE_MIXED)
+        for b in range(len(wls)):
+            assert np.array_equal(U0[b], U2[b]) and np.array_equal(V0[b], V2[b]), (shape, b)
+    ctx.sweep_tuning_set(t.threads_per_block, t.schedule, t.loads)

@@ -178,3 +178,25 @@ def test_krylov_ssa_step_mixed_vs_fp64():
             assert e <= 1e-7, (k, e)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("precision", [abi.UFE_FP64, abi.UFE_MIXED])
+def test_linear_solve_bicgstab_matches_gmres(ctx, precision):
+    """BiCGStab-IR reaches the FP64 tolerance and lands on the GMRES-IR solution."""
+    print(f"[linear] bicgstab vs gmres, precision {precision}", flush=True)
+    _linear_system(ctx)
+    sols = {}
+    for method in (0, 1):
+        ctx.fill("U_SSA_AaAc", 0.0)
+        ctx.fill("V_SSA_AaAc", 0.0)
+        st = ctx.ssa_linear_solve(1e-9, 400000, precision=precision, omega=1.3, gmres=30,
+                                  precond_sweeps=4, orth_passes=1, method=method)
+        assert st.status == 0 and st.rel_residual <= 1e-9, (method, st.rel_residual)
+        assert abs(ctx.ssa_residual() - st.rel_residual) <= 1e-12
+        sols[method] = (ctx.get("U_SSA_AaAc"), ctx.get("V_SSA_AaAc"))
+        print(f"  method {method}: sweeps {st.sweeps} matvecs {st.matvecs} refinements {st.refinements}",
+              flush=True)
+    (u0, v0), (u1, v1) = sols[0], sols[1]
+    err = np.sqrt(np.sum((u1 - u0) ** 2 + (v1 - v0) ** 2) / np.sum(u0 ** 2 + v0 ** 2))
+    print(f"  bicgstab vs gmres rel L2 {err:.2e}", flush=True)
+    assert err <= 1e-6

@@ -43,7 +43,8 @@ EXPORTS = [
     "ufe_device_alloc", "ufe_device_free", "ufe_memcpy_h2d", "ufe_memcpy_d2h", "ufe_synchronize",
     "ufe_field_fill", "ufe_profile_enable", "ufe_profile_read", "ufe_sweep_bytes",
     "ufe_ssa_adjoint_forward", "ufe_ssa_adjoint_backward",
-    "ufe_batch_define", "ufe_batch_autotune", "ufe_batch_linear_solve", "ufe_batch_solve_SSA",
+    "ufe_batch_define", "ufe_batch_linear_solve", "ufe_batch_solve_SSA",
+    "ufe_sweep_autotune", "ufe_sweep_tuning_set",
 ]
 
 _P = ct.c_void_p
@@ -84,7 +85,7 @@ class LinSolveOpts(ct.Structure):
     _fields_ = [("rel_tol", ct.c_double), ("max_sweeps", ct.c_int32), ("precision", ct.c_int32),
                 ("omega", ct.c_double), ("check_interval", ct.c_int32), ("use_graph", ct.c_int32),
                 ("gmres_restart", ct.c_int32), ("precond_sweeps", ct.c_int32),
-                ("orth_passes", ct.c_int32)]
+                ("orth_passes", ct.c_int32), ("method", ct.c_int32)]
 
 
 class KernelProfile(ct.Structure):
@@ -125,10 +126,16 @@ class BatchStats(ct.Structure):
                 ("max_rel_residual", ct.c_double), ("seconds", ct.c_double), ("status", ct.c_int32)]
 
 
-class BatchTuning(ct.Structure):
-    _fields_ = [("threads_per_block", ct.c_int32), ("schedule", ct.c_int32), ("n_candidates", ct.c_int32),
-                ("cand_threads", ct.c_int32 * 16), ("cand_schedule", ct.c_int32 * 16),
-                ("cand_us_per_sweep", ct.c_double * 16)]
+class SweepTuning(ct.Structure):
+    _fields_ = [("threads_per_block", ct.c_int32), ("schedule", ct.c_int32), ("loads", ct.c_int32),
+                ("n_candidates", ct.c_int32), ("cand_threads", ct.c_int32 * 32),
+                ("cand_schedule", ct.c_int32 * 32), ("cand_loads", ct.c_int32 * 32),
+                ("cand_us_per_sweep", ct.c_double * 32)]
+
+    def candidates(self):
+        return [{"threads": self.cand_threads[i], "schedule": self.cand_schedule[i],
+                 "loads": self.cand_loads[i], "us_per_sweep": round(self.cand_us_per_sweep[i], 2)}
+                for i in range(self.n_candidates)]
 
 
 _lib = None
@@ -326,9 +333,11 @@ class Context:
 
     def ssa_linear_solve(self, rel_tol, max_sweeps, precision=UFE_FP64, omega=0.0,
                          check_interval=16, gmres=0, precond_sweeps=2,
-                         orth_passes=2) -> LinSolveStats:
+                         orth_passes=2, method=0) -> LinSolveStats:
+        """method 0: GMRES(gmres) (plain SOR if gmres == 0); 1: BiCGStab."""
         o = LinSolveOpts(float(rel_tol), int(max_sweeps), int(precision), float(omega),
-                         int(check_interval), 0, int(gmres), int(precond_sweeps), int(orth_passes))
+                         int(check_interval), 0, int(gmres), int(precond_sweeps), int(orth_passes),
+                         int(method))
         st = LinSolveStats()
         self._chk(self.lib.ufe_ssa_linear_solve(self.h, ct.byref(o), ct.byref(st)),
                   "ssa_linear_solve")
@@ -380,10 +389,15 @@ class Context:
         self._batch_sizes = (nV, nAc)
         self._chk(self.lib.ufe_batch_define(self.h, int(len(nV)), _ptr(nV), _ptr(nAc)), "batch_define")
 
-    def batch_autotune(self, sweeps=20) -> BatchTuning:
-        t = BatchTuning()
-        self._chk(self.lib.ufe_batch_autotune(self.h, int(sweeps), ct.byref(t)), "batch_autotune")
+    def sweep_autotune(self, sweeps=20) -> SweepTuning:
+        """Time every launch shape of the FP32 sweep on the uploaded mesh; keep the fastest."""
+        t = SweepTuning()
+        self._chk(self.lib.ufe_sweep_autotune(self.h, int(sweeps), ct.byref(t)), "sweep_autotune")
         return t
+
+    def sweep_tuning_set(self, threads_per_block=256, schedule=0, loads=0):
+        self._chk(self.lib.ufe_sweep_tuning_set(self.h, int(threads_per_block), int(schedule), int(loads)),
+                  "sweep_tuning_set")
 
     def _batch_opts(self, rel_tol, precision, gmres_restart, precond_sweeps, omega, max_cycles, use_graph):
         return BatchOpts(float(rel_tol), int(precision), int(gmres_restart), int(precond_sweeps),

@@ -166,6 +166,7 @@ struct ufe_ctx {
   bool kry_f32 = true;
   void *kry_V = nullptr, *kry_Z = nullptr, *kry_w = nullptr, *kry_x = nullptr;
   double2 *kry_R64 = nullptr;
+  double *bic_S = nullptr;          // BiCGStab scalars on the device
   double *kry_part = nullptr, *kry_H1 = nullptr, *kry_H2 = nullptr, *kry_Hs = nullptr, *kry_y = nullptr,
          *kry_nrm = nullptr;
   // algorithmic bytes per colour (SOR rows only)
@@ -197,7 +198,7 @@ struct ufe_ctx {
   std::vector<int32_t> h_mch_ptr;
   std::vector<int64_t> b_rows;       // SOR rows per member
   // batch Krylov work space and the captured cycle
-  int bk_m = 0, bk_ks = 0, bk_prec = -1, bk_bs = 0, bk_sched = -1;
+  int bk_m = 0, bk_ks = 0, bk_prec = -1, bk_bs = 0, bk_sched = -1, bk_loads = -1;
   double bk_omega = 0.0;
   void *bk_V = nullptr, *bk_Z = nullptr, *bk_w = nullptr, *bk_x = nullptr, *bk_R = nullptr;
   double *bk_part = nullptr, *bk_H = nullptr, *bk_Hs = nullptr, *bk_y = nullptr, *bk_nrm = nullptr,
@@ -205,7 +206,7 @@ struct ufe_ctx {
   double bk_tol = 0.0;
   hipGraphExec_t bk_graph = nullptr;
   // autotuned SOR schedule of the batch: threads per block and block order
-  int tune_bs = 256, tune_sched = 0;
+  int tune_bs = 256, tune_sched = 0, tune_loads = 0;
   SorFlags *h_flags = nullptr;      // pinned
   double *h_scalar = nullptr;       // pinned
   std::vector<void *> allocs;       // everything to free with the mesh
@@ -276,6 +277,7 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->kry_m = 0;
   ctx->kry_V = ctx->kry_Z = ctx->kry_w = ctx->kry_x = nullptr;
   ctx->kry_R64 = nullptr;
+  ctx->bic_S = nullptr;
   ctx->kry_part = ctx->kry_H1 = ctx->kry_H2 = ctx->kry_Hs = ctx->kry_y = ctx->kry_nrm = nullptr;
   ctx->adj_m = ctx->adj_K = ctx->adj_nck = 0;
   ctx->adj_ready = false;
@@ -1217,6 +1219,48 @@ static void enqueue_sweep_fp64(ufe_ctx *ctx, const AaAcMesh &m, double omega, do
     k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
 }
 
+// One FP32 colour pass with the tuned launch shape (threads per block, block
+// order, load variant), timed like the fixed-shape passes.
+extern "C++" template <int BS, int SCHED, int LOADS>
+static void launch_sor_f32_shape(ufe_ctx *ctx, const AaAcMesh &m, const Mesh32 &q, float2 *D, const float2 *R, int c,
+                                 float omega) {
+  const int n = ctx->colour_active[c];
+  int g = (n + BS - 1) / BS;
+  if (SCHED == 0) g = (g + 7) / 8 * 8;
+  timed_kernel(ctx, ctx->bytes32[c], n, k_sor_colour_f32_tuned<BS, SCHED, LOADS>, dim3(g), dim3(BS), m, q, D, R,
+               ctx->colour_start[c], n, omega);
+}
+static void launch_sor_f32(ufe_ctx *ctx, int bs, int sched, int loads, const AaAcMesh &m, const Mesh32 &q, float2 *D,
+                           const float2 *R, int c, float omega) {
+  switch (bs * 8 + sched * 4 + loads) {
+    case 512: launch_sor_f32_shape<64, 0, 0>(ctx, m, q, D, R, c, omega); break;
+    case 513: launch_sor_f32_shape<64, 0, 1>(ctx, m, q, D, R, c, omega); break;
+    case 514: launch_sor_f32_shape<64, 0, 2>(ctx, m, q, D, R, c, omega); break;
+    case 516: launch_sor_f32_shape<64, 1, 0>(ctx, m, q, D, R, c, omega); break;
+    case 517: launch_sor_f32_shape<64, 1, 1>(ctx, m, q, D, R, c, omega); break;
+    case 518: launch_sor_f32_shape<64, 1, 2>(ctx, m, q, D, R, c, omega); break;
+    case 1024: launch_sor_f32_shape<128, 0, 0>(ctx, m, q, D, R, c, omega); break;
+    case 1025: launch_sor_f32_shape<128, 0, 1>(ctx, m, q, D, R, c, omega); break;
+    case 1026: launch_sor_f32_shape<128, 0, 2>(ctx, m, q, D, R, c, omega); break;
+    case 1028: launch_sor_f32_shape<128, 1, 0>(ctx, m, q, D, R, c, omega); break;
+    case 1029: launch_sor_f32_shape<128, 1, 1>(ctx, m, q, D, R, c, omega); break;
+    case 1030: launch_sor_f32_shape<128, 1, 2>(ctx, m, q, D, R, c, omega); break;
+    case 2048: launch_sor_f32_shape<256, 0, 0>(ctx, m, q, D, R, c, omega); break;
+    case 2049: launch_sor_f32_shape<256, 0, 1>(ctx, m, q, D, R, c, omega); break;
+    case 2050: launch_sor_f32_shape<256, 0, 2>(ctx, m, q, D, R, c, omega); break;
+    case 2052: launch_sor_f32_shape<256, 1, 0>(ctx, m, q, D, R, c, omega); break;
+    case 2053: launch_sor_f32_shape<256, 1, 1>(ctx, m, q, D, R, c, omega); break;
+    case 2054: launch_sor_f32_shape<256, 1, 2>(ctx, m, q, D, R, c, omega); break;
+    case 4096: launch_sor_f32_shape<512, 0, 0>(ctx, m, q, D, R, c, omega); break;
+    case 4097: launch_sor_f32_shape<512, 0, 1>(ctx, m, q, D, R, c, omega); break;
+    case 4098: launch_sor_f32_shape<512, 0, 2>(ctx, m, q, D, R, c, omega); break;
+    case 4100: launch_sor_f32_shape<512, 1, 0>(ctx, m, q, D, R, c, omega); break;
+    case 4101: launch_sor_f32_shape<512, 1, 1>(ctx, m, q, D, R, c, omega); break;
+    case 4102: launch_sor_f32_shape<512, 1, 2>(ctx, m, q, D, R, c, omega); break;
+    default: launch_sor_f32_shape<256, 0, 0>(ctx, m, q, D, R, c, omega); break;
+  }
+}
+
 static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, float2 *D = nullptr,
                                const float2 *R = nullptr, bool backward = false) {
   Mesh32 q;
@@ -1229,8 +1273,7 @@ static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, flo
     const int c = backward ? 4 - k : k;
     const int n = ctx->colour_active[c];
     if (n == 0) continue;
-    timed_kernel(ctx, ctx->bytes32[c], n, k_sor_colour_f32, dim3(xcd_grid(n)), dim3(BLOCK), m, q, D, R,
-                 ctx->colour_start[c], n, omega);
+    launch_sor_f32(ctx, ctx->tune_bs, ctx->tune_sched, ctx->tune_loads, m, q, D, R, c, omega);
   }
   if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D);
   k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), D);
@@ -1596,6 +1639,86 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
   return UFE_OK;
 }
 
+// BiCGStab on A d = r (the FP64 residual, copied to precision T) until the
+// recurrence residual falls by `reduce` or `max_sweeps` is reached; then
+// w += d in FP64.  Vectors live in the Krylov work space, scalars in bic_S.
+extern "C++" template <typename T>
+static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max_sweeps, int *sweeps,
+                      int *matvecs) {
+  using T2 = typename V2T<T>::t;
+  const bool f32 = sizeof(T) == 4;
+  const int n = ctx->nA, nb = nblk(n);
+  const AaAcMesh am = aaac_mesh(ctx);
+  const ActiveRanges ar = active_ranges(ctx);
+  T2 *V = (T2 *)ctx->kry_V;
+  T2 *rh = V, *r = V + (int64_t)n, *s = V + 2 * (int64_t)n, *t = V + 3 * (int64_t)n, *p = V + 4 * (int64_t)n,
+     *v = V + 5 * (int64_t)n, *ph = V + 6 * (int64_t)n, *sh = V + 7 * (int64_t)n, *x = V + 8 * (int64_t)n;
+  double *S = ctx->bic_S;
+  if (f32) {
+    k_bicg_init<float><<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_R32, (float2 *)rh, (float2 *)r, (float2 *)x,
+                                                      (float2 *)p, (float2 *)v);
+  } else {
+    k_residual_store64<<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2], ar,
+                                                       ctx->kry_R64);
+    k_bicg_init<double><<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->kry_R64, (double2 *)rh, (double2 *)r,
+                                                       (double2 *)x, (double2 *)p, (double2 *)v);
+  }
+  auto dot = [&](const T2 *a, int k, const T2 *b, double *out) {
+    k_multi_dot<T><<<DOT_BLOCKS, BLOCK, 0, ctx->stream>>>(n, k, ar, a, b, ctx->kry_part);
+    k_multi_dot_reduce<<<k, 256, 0, ctx->stream>>>(DOT_BLOCKS, k, ctx->kry_part, out, 0);
+  };
+  auto precond = [&](const T2 *in, T2 *out) {
+    k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(out, n, T2{T(0), T(0)});
+    for (int k = 0; k < ks; k++) {
+      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)out, (const float2 *)in, k & 1);
+      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)out, (const double2 *)in, k & 1);
+    }
+  };
+  auto matvec = [&](const T2 *in, T2 *out) {
+    if (f32)
+      k_matvec<float><<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32, ar,
+                                                     (const float2 *)in, (float2 *)out);
+    else
+      k_matvec<double><<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_cU, ctx->d_Sxy, ctx->d_pair[1], ar,
+                                                      (const double2 *)in, (double2 *)out);
+  };
+  dot(rh, 2, r, S + 1);                       // rho = (r^, r), |r|^2
+  k_bicg_scalars_init<<<1, 1, 0, ctx->stream>>>(S);
+  LAUNCHCHK(ctx);
+  double h[3];
+  HIPCHK(ctx, hipMemcpyAsync(h, S + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const double rr0 = h[1];
+  if (!(rr0 > 0.0)) return UFE_OK;
+  int it = 0;
+  while (*sweeps < max_sweeps) {
+    k_bicg_beta<<<1, 1, 0, ctx->stream>>>(S);
+    k_bicg_p<T><<<nb, BLOCK, 0, ctx->stream>>>(n, r, v, p, S);
+    precond(p, ph);
+    matvec(ph, v);
+    dot(rh, 1, v, S + 5);
+    k_bicg_alpha<<<1, 1, 0, ctx->stream>>>(S);
+    k_bicg_s<T><<<nb, BLOCK, 0, ctx->stream>>>(n, r, v, s, S);
+    precond(s, sh);
+    matvec(sh, t);
+    dot(s, 2, t, S + 6);                      // (s, t), (t, t): s and t are adjacent
+    k_bicg_omega<<<1, 1, 0, ctx->stream>>>(S);
+    k_bicg_update<T><<<nb, BLOCK, 0, ctx->stream>>>(n, ph, sh, s, t, x, r, S);
+    dot(rh, 2, r, S + 1);                     // new rho and |r|^2: r^ and r are adjacent
+    LAUNCHCHK(ctx);
+    *sweeps += 2 * ks;
+    *matvecs += 2;
+    if (++it % 2 == 0) {
+      HIPCHK(ctx, hipMemcpyAsync(h, S + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      if (!std::isfinite(h[1]) || h[1] <= reduce * reduce * rr0) break;
+    }
+  }
+  k_add_to_w<T><<<nb, BLOCK, 0, ctx->stream>>>(n, x, ctx->d_pair[0]);
+  LAUNCHCHK(ctx);
+  return UFE_OK;
+}
+
 int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_stats *st) {
   if (!ctx || !ctx->have_mesh || !st || !o) return UFE_BAD_INPUT;
   std::memset(st, 0, sizeof(*st));
@@ -1614,6 +1737,32 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
   if (mr < 0 || mr > 64) {
     ctx->err = "gmres_restart must be in 0..64";
     return UFE_BAD_INPUT;
+  }
+  if (o->method == 1) {
+    // BiCGStab-IR: FP64 residual, inner BiCGStab in FP32 (mixed) or FP64
+    if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], nullptr);
+    k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
+    if ((r = krylov_alloc(ctx, std::max(mr, 8), mixed))) return r;
+    if (!ctx->bic_S && (r = dalloc(ctx, &ctx->bic_S, 16))) return r;
+    const int ks = o->precond_sweeps > 0 ? o->precond_sweeps : 2;
+    if ((r = residual_fp64(ctx, &rel, mixed))) return r;
+    st->refinements = 1;
+    double prev = HUGE_VAL;
+    while (rel > o->rel_tol && st->sweeps < o->max_sweeps) {
+      if (rel > 0.99 * prev) break;
+      prev = rel;
+      const double reduce = std::max(0.1 * o->rel_tol / rel, mixed ? 1e-4 : 1e-13);
+      r = mixed ? bicg_inner<float>(ctx, ks, omega, reduce, o->max_sweeps, &st->sweeps, &st->matvecs)
+                : bicg_inner<double>(ctx, ks, omega, reduce, o->max_sweeps, &st->sweeps, &st->matvecs);
+      if (r) return r;
+      if ((r = residual_fp64(ctx, &rel, mixed))) return r;
+      st->refinements++;
+      if (!std::isfinite(rel)) break;
+    }
+    st->rel_residual = rel;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    st->status = rel <= o->rel_tol ? UFE_OK : UFE_SOR_NOT_CONVERGED;
+    return UFE_OK;
   }
   if (mr > 0) {
     // GMRES works on the Neumann-consistent subspace (edge rows follow the SOR
@@ -2031,39 +2180,6 @@ static int batch_alloc(ufe_ctx *ctx, int m, bool f32) {
   return UFE_OK;
 }
 
-// FP32 preconditioner sweep with the tuned launch shape (all members at once)
-extern "C++" template <int BS, int SCHED>
-static void sweep_f32_shape(ufe_ctx *ctx, const AaAcMesh &m, const Mesh32 &q, float omega, float2 *D, const float2 *R,
-                            bool backward) {
-  for (int k = 0; k < 5; k++) {
-    const int c = backward ? 4 - k : k;
-    const int n = ctx->colour_active[c];
-    if (n == 0) continue;
-    int g = (n + BS - 1) / BS;
-    if (SCHED == 0) g = (g + 7) / 8 * 8;
-    k_sor_colour_f32_tuned<BS, SCHED><<<g, BS, 0, ctx->stream>>>(m, q, D, R, ctx->colour_start[c], n, omega);
-  }
-}
-static void enqueue_sweep_fp32_tuned(ufe_ctx *ctx, const AaAcMesh &m, float omega, float2 *D, const float2 *R,
-                                     bool backward, int bs, int sched) {
-  Mesh32 q;
-  q.cU = ctx->d_cU32;
-  q.Sxy = ctx->d_Sxy32;
-  q.EV = ctx->d_EV32;
-  switch (bs * 2 + sched) {
-    case 64 * 2 + 0: sweep_f32_shape<64, 0>(ctx, m, q, omega, D, R, backward); break;
-    case 64 * 2 + 1: sweep_f32_shape<64, 1>(ctx, m, q, omega, D, R, backward); break;
-    case 128 * 2 + 0: sweep_f32_shape<128, 0>(ctx, m, q, omega, D, R, backward); break;
-    case 128 * 2 + 1: sweep_f32_shape<128, 1>(ctx, m, q, omega, D, R, backward); break;
-    case 512 * 2 + 0: sweep_f32_shape<512, 0>(ctx, m, q, omega, D, R, backward); break;
-    case 512 * 2 + 1: sweep_f32_shape<512, 1>(ctx, m, q, omega, D, R, backward); break;
-    case 256 * 2 + 1: sweep_f32_shape<256, 1>(ctx, m, q, omega, D, R, backward); break;
-    default: sweep_f32_shape<256, 0>(ctx, m, q, omega, D, R, backward); break;
-  }
-  if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D);
-  k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), D);
-}
-
 // residual of every member, its relative norm, and the scale of v0 = r/|r|
 extern "C++" template <typename T>
 static void batch_residual(ufe_ctx *ctx, double tol) {
@@ -2092,8 +2208,7 @@ static void batch_enqueue_cycle(ufe_ctx *ctx, int m, int ks, double omega, doubl
     T2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;
     k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
     for (int k = 0; k < ks; k++) {
-      if (f32) enqueue_sweep_fp32_tuned(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1,
-                                        ctx->tune_bs, ctx->tune_sched);
+      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1);
       else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1);
     }
     if (f32)
@@ -2120,7 +2235,8 @@ static void batch_enqueue_cycle(ufe_ctx *ctx, int m, int ks, double omega, doubl
 
 static int batch_capture(ufe_ctx *ctx, bool f32, int m, int ks, double omega, double tol) {
   if (ctx->bk_graph && ctx->bk_ks == ks && ctx->bk_omega == omega && ctx->bk_tol == tol &&
-      ctx->bk_bs == ctx->tune_bs && ctx->bk_sched == ctx->tune_sched && ctx->bk_m == m)
+      ctx->bk_bs == ctx->tune_bs && ctx->bk_sched == ctx->tune_sched && ctx->bk_loads == ctx->tune_loads &&
+      ctx->bk_m == m)
     return UFE_OK;
   if (ctx->bk_graph) (void)hipGraphExecDestroy(ctx->bk_graph);
   ctx->bk_graph = nullptr;
@@ -2151,6 +2267,7 @@ static int batch_capture(ufe_ctx *ctx, bool f32, int m, int ks, double omega, do
   ctx->bk_tol = tol;
   ctx->bk_bs = ctx->tune_bs;
   ctx->bk_sched = ctx->tune_sched;
+  ctx->bk_loads = ctx->tune_loads;
   return UFE_OK;
 }
 
@@ -2243,44 +2360,52 @@ int ufe_batch_solve_SSA(ufe_ctx *ctx, int picard_iterations, const ufe_batch_opt
   return UFE_OK;
 }
 
-// Time `sweeps` FP32 preconditioner sweeps (colour passes plus Neumann) of the
-// whole batch for each launch shape and keep the fastest.
-int ufe_batch_autotune(ufe_ctx *ctx, int sweeps, ufe_batch_tuning *out) {
+// Time `sweeps` FP32 sweeps (colour passes plus Neumann) of the uploaded mesh
+// (a single mesh or a batch) for every launch shape and keep the fastest.
+int ufe_sweep_autotune(ufe_ctx *ctx, int sweeps, ufe_sweep_tuning *out) {
   if (!ctx || !ctx->have_mesh || !out || sweeps < 1) return UFE_BAD_INPUT;
   std::memset(out, 0, sizeof(*out));
   const AaAcMesh am = aaac_mesh(ctx);
   k_to_f32_pair<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_pair[1], ctx->d_EV32);
   LAUNCHCHK(ctx);
+  const bool prof = ctx->prof;
+  ctx->prof = false;
   const int bss[4] = {64, 128, 256, 512};
   hipEvent_t a, b;
   HIPCHK(ctx, hipEventCreate(&a));
   HIPCHK(ctx, hipEventCreate(&b));
   double best = HUGE_VAL;
-  int nbest = 256, sbest = 0, k = 0;
-  for (int bs : bss)
-    for (int sched = 0; sched < 2; sched++) {
-      enqueue_sweep_fp32_tuned(ctx, am, 1.0f, ctx->d_D32, ctx->d_R32, false, bs, sched);  // warm-up
-      (void)hipEventRecord(a, ctx->stream);
-      for (int i = 0; i < sweeps; i++)
-        enqueue_sweep_fp32_tuned(ctx, am, 1.0f, ctx->d_D32, ctx->d_R32, i & 1, bs, sched);
-      (void)hipEventRecord(b, ctx->stream);
-      LAUNCHCHK(ctx);
-      HIPCHK(ctx, hipEventSynchronize(b));
-      float ms = 0.0f;
-      HIPCHK(ctx, hipEventElapsedTime(&ms, a, b));
-      const double us = 1e3 * ms / sweeps;
-      out->cand_threads[k] = bs;
-      out->cand_schedule[k] = sched;
-      out->cand_us_per_sweep[k] = us;
-      k++;
-      if (us < best) {
-        best = us;
-        nbest = bs;
-        sbest = sched;
+  int nbest = 256, sbest = 0, lbest = 0, k = 0;
+  for (int loads = 0; loads < 3; loads++)
+    for (int bs : bss)
+      for (int sched = 0; sched < 2; sched++) {
+        ctx->tune_bs = bs;
+        ctx->tune_sched = sched;
+        ctx->tune_loads = loads;
+        enqueue_sweep_fp32(ctx, am, 1.0f, ctx->d_D32, ctx->d_R32, false);  // warm-up
+        (void)hipEventRecord(a, ctx->stream);
+        for (int i = 0; i < sweeps; i++) enqueue_sweep_fp32(ctx, am, 1.0f, ctx->d_D32, ctx->d_R32, i & 1);
+        (void)hipEventRecord(b, ctx->stream);
+        LAUNCHCHK(ctx);
+        HIPCHK(ctx, hipEventSynchronize(b));
+        float ms = 0.0f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, a, b));
+        const double us = 1e3 * ms / sweeps;
+        out->cand_threads[k] = bs;
+        out->cand_schedule[k] = sched;
+        out->cand_loads[k] = loads;
+        out->cand_us_per_sweep[k] = us;
+        k++;
+        if (us < best) {
+          best = us;
+          nbest = bs;
+          sbest = sched;
+          lbest = loads;
+        }
       }
-    }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  ctx->prof = prof;
   // the sweeps above ran on scratch vectors: clear the correction again
   k_fill<float2><<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->d_D32, ctx->nA, make_float2(0.0f, 0.0f));
   LAUNCHCHK(ctx);
@@ -2288,8 +2413,20 @@ int ufe_batch_autotune(ufe_ctx *ctx, int sweeps, ufe_batch_tuning *out) {
   out->n_candidates = k;
   out->threads_per_block = nbest;
   out->schedule = sbest;
+  out->loads = lbest;
   ctx->tune_bs = nbest;
   ctx->tune_sched = sbest;
+  ctx->tune_loads = lbest;
+  return UFE_OK;
+}
+
+int ufe_sweep_tuning_set(ufe_ctx *ctx, int threads_per_block, int schedule, int loads) {
+  if (!ctx || (threads_per_block != 64 && threads_per_block != 128 && threads_per_block != 256 &&
+               threads_per_block != 512) || schedule < 0 || schedule > 1 || loads < 0 || loads > 2)
+    return UFE_BAD_INPUT;
+  ctx->tune_bs = threads_per_block;
+  ctx->tune_sched = schedule;
+  ctx->tune_loads = loads;
   return UFE_OK;
 }
 
