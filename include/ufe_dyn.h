@@ -96,8 +96,8 @@ typedef struct ufe_params {
    * FP32 sweeps and Krylov vectors for UFE_MIXED, FP64 for UFE_FP64_KRYLOV,
    * with FP64 residuals and iterative refinement either way. */
   double SSA_linear_rel_tol;        /* <= 0: 1e-6 */
-  int32_t SSA_gmres_restart;        /* <= 0: 30 */
-  int32_t SSA_precond_sweeps;       /* <= 0: 4 */
+  int32_t SSA_gmres_restart;        /* <= 0: 40 */
+  int32_t SSA_precond_sweeps;       /* <= 0: 5 */
 } ufe_params;
 
 /* Statistics of one solve_SSA call. */
@@ -135,11 +135,14 @@ typedef struct ufe_linsolve_opts {
   int32_t method;                   /* 0: GMRES(gmres_restart) (plain SOR if that is 0);
                                        1: BiCGStab preconditioned by precond_sweeps SOR sweeps,
                                        scalars kept on the device, FP64 refinement around it */
+  int32_t precond_full_sweeps;      /* 0: successive preconditioner sweeps alternate direction and
+                                       skip the colour the previous sweep ended on; 1: full sweeps */
 } ufe_linsolve_opts;
 
 /* Statistics of one linearised solve. */
 typedef struct ufe_linsolve_stats {
-  int32_t sweeps;                   /* SOR sweeps performed (FP32 sweeps in mixed mode) */
+  int32_t sweeps;                   /* SOR sweeps performed (FP32 sweeps in mixed mode), counted as
+                                       colour passes / 5 */
   int32_t refinements;              /* FP64 residual evaluations */
   int32_t matvecs;                  /* operator applications outside the SOR sweeps (GMRES) */
   double rel_residual;              /* ||b - A w||_2 / ||b||_2 in FP64 at exit */
@@ -147,12 +150,15 @@ typedef struct ufe_linsolve_stats {
   int32_t status;
 } ufe_linsolve_stats;
 
-/* Live kernel timing (HIP events around every SOR colour-kernel launch). */
+/* Live kernel timing: start/stop events attached to the dispatch of every
+ * 8th SOR colour-kernel launch (hipExtLaunchKernelGGL), i.e. the kernel's own
+ * execution time on the library's stream. */
 typedef struct ufe_kernel_profile {
-  int64_t launches;                 /* colour-kernel launches timed */
+  int64_t launches;                 /* colour-kernel launches timed (the sample) */
   double seconds;                   /* summed event time of those launches */
   double algorithmic_bytes;         /* summed algorithmic bytes of those launches */
   int64_t rows;                     /* summed rows (vertex-sweeps) of those launches */
+  int64_t total_launches;           /* colour-kernel launches while profiling was on */
 } ufe_kernel_profile;
 
 typedef struct ufe_ctx ufe_ctx;
@@ -303,7 +309,7 @@ typedef struct ufe_batch_opts {
 
 typedef struct ufe_batch_stats {
   int32_t cycles;                   /* GMRES cycles run */
-  int32_t sweeps;                   /* SOR sweeps per member */
+  int32_t sweeps;                   /* SOR sweeps per member (colour passes / 5) */
   int32_t converged;                /* members at the tolerance */
   double max_rel_residual;          /* worst member at exit */
   double seconds;

@@ -85,12 +85,13 @@ class LinSolveOpts(ct.Structure):
     _fields_ = [("rel_tol", ct.c_double), ("max_sweeps", ct.c_int32), ("precision", ct.c_int32),
                 ("omega", ct.c_double), ("check_interval", ct.c_int32), ("use_graph", ct.c_int32),
                 ("gmres_restart", ct.c_int32), ("precond_sweeps", ct.c_int32),
-                ("orth_passes", ct.c_int32), ("method", ct.c_int32)]
+                ("orth_passes", ct.c_int32), ("method", ct.c_int32),
+                ("precond_full_sweeps", ct.c_int32)]
 
 
 class KernelProfile(ct.Structure):
     _fields_ = [("launches", ct.c_int64), ("seconds", ct.c_double),
-                ("algorithmic_bytes", ct.c_double), ("rows", ct.c_int64)]
+                ("algorithmic_bytes", ct.c_double), ("rows", ct.c_int64), ("total_launches", ct.c_int64)]
 
 
 class LinSolveStats(ct.Structure):
@@ -189,7 +190,7 @@ def _ptr(a):
 
 
 def params_struct(p: DynParams, precision=UFE_FP64, sweep_batch=0, linear_rel_tol=1e-6,
-                  gmres_restart=30, precond_sweeps=4) -> Params:
+                  gmres_restart=40, precond_sweeps=5) -> Params:
     s = Params()
     s.nZ = p.nZ
     for k, z in enumerate(p.zeta):
@@ -333,11 +334,11 @@ class Context:
 
     def ssa_linear_solve(self, rel_tol, max_sweeps, precision=UFE_FP64, omega=0.0,
                          check_interval=16, gmres=0, precond_sweeps=2,
-                         orth_passes=2, method=0) -> LinSolveStats:
+                         orth_passes=2, method=0, full_sweeps=False) -> LinSolveStats:
         """method 0: GMRES(gmres) (plain SOR if gmres == 0); 1: BiCGStab."""
         o = LinSolveOpts(float(rel_tol), int(max_sweeps), int(precision), float(omega),
                          int(check_interval), 0, int(gmres), int(precond_sweeps), int(orth_passes),
-                         int(method))
+                         int(method), int(bool(full_sweeps)))
         st = LinSolveStats()
         self._chk(self.lib.ufe_ssa_linear_solve(self.h, ct.byref(o), ct.byref(st)),
                   "ssa_linear_solve")
@@ -354,7 +355,7 @@ class Context:
         self._chk(self.lib.ufe_field_get_device(self.h, fid, ct.c_void_p(int(ptr))), f"get_device {name}")
 
     def adjoint_forward(self, picard_iterations, lin_rel_tol=1e-12, checkpoint_every=1,
-                        precision=UFE_FP64, gmres_restart=30, precond_sweeps=4, omega=0.0,
+                        precision=UFE_FP64, gmres_restart=40, precond_sweeps=5, omega=0.0,
                         max_sweeps=1_000_000, allow=(UFE_OK, UFE_SOR_NOT_CONVERGED)) -> AdjointStats:
         o = AdjointOpts(float(lin_rel_tol), int(picard_iterations), int(checkpoint_every),
                         int(precision), int(gmres_restart), int(precond_sweeps), float(omega),
@@ -403,7 +404,7 @@ class Context:
         return BatchOpts(float(rel_tol), int(precision), int(gmres_restart), int(precond_sweeps),
                          float(omega), int(max_cycles), int(bool(use_graph)))
 
-    def batch_linear_solve(self, rel_tol=1e-6, precision=UFE_MIXED, gmres_restart=30, precond_sweeps=4,
+    def batch_linear_solve(self, rel_tol=1e-6, precision=UFE_MIXED, gmres_restart=40, precond_sweeps=5,
                            omega=0.0, max_cycles=0, use_graph=True):
         o = self._batch_opts(rel_tol, precision, gmres_restart, precond_sweeps, omega, max_cycles, use_graph)
         st = BatchStats()
@@ -412,8 +413,8 @@ class Context:
                   "batch_linear_solve")
         return st, rel
 
-    def batch_solve_SSA(self, picard_iterations, rel_tol=1e-6, precision=UFE_MIXED, gmres_restart=30,
-                        precond_sweeps=4, omega=0.0, max_cycles=0, use_graph=True):
+    def batch_solve_SSA(self, picard_iterations, rel_tol=1e-6, precision=UFE_MIXED, gmres_restart=40,
+                        precond_sweeps=5, omega=0.0, max_cycles=0, use_graph=True):
         o = self._batch_opts(rel_tol, precision, gmres_restart, precond_sweeps, omega, max_cycles, use_graph)
         st = BatchStats()
         rel = np.zeros(len(self._batch_sizes[0]))

@@ -31,8 +31,8 @@ class AdjointOptions:
     lin_rel_tol: float = 1e-12
     checkpoint_every: int = 1
     precision: int = abi.UFE_FP64
-    gmres_restart: int = 30
-    precond_sweeps: int = 4
+    gmres_restart: int = 40
+    precond_sweeps: int = 5
     omega: float = 0.0
     max_sweeps: int = 1_000_000
 

@@ -174,6 +174,7 @@ struct ufe_ctx {
   double degsum = 0.0;
   // live kernel profile
   bool prof = false;
+  int64_t prof_count = 0;
   std::vector<hipEvent_t> ev_pool;
   struct Pending { hipEvent_t a, b; double bytes; int64_t rows; };
   std::vector<Pending> pending;
@@ -207,6 +208,7 @@ struct ufe_ctx {
   hipGraphExec_t bk_graph = nullptr;
   // autotuned SOR schedule of the batch: threads per block and block order
   int tune_bs = 256, tune_sched = 0, tune_loads = 0;
+  bool repeat_turn = false;          // Krylov preconditioner: repeat the turn colour (full sweeps)
   SorFlags *h_flags = nullptr;      // pinned
   double *h_scalar = nullptr;       // pinned
   std::vector<void *> allocs;       // everything to free with the mesh
@@ -248,6 +250,7 @@ static uint64_t hilbert_index(uint32_t x, uint32_t y) {
 }
 
 static inline int nblk(int64_t n, int b = BLOCK) { return (int)((n + b - 1) / b); }
+constexpr int PROF_EVERY = 8;     // colour launches per timed sample
 // grid for XCD-major kernels: a multiple of the 8 XCDs
 static inline int xcd_grid(int64_t n) { return (nblk(n) + 7) / 8 * 8; }
 
@@ -1170,7 +1173,10 @@ extern "C++" template <typename K, typename... Args>
 static void timed_kernel(ufe_ctx *ctx, double bytes, int64_t rows, K kernel, dim3 grid, dim3 block,
                          Args... args) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (ctx->prof) {
+  // events ride on every PROF_EVERY-th launch: the sample's mean duration is
+  // the kernel's, without an event pair on every launch of the timed region
+  if (ctx->prof) ctx->prof_acc.total_launches++;
+  if (ctx->prof && (ctx->prof_count++ % PROF_EVERY) == 0) {
     a = prof_event(ctx);
     b = prof_event(ctx);
     if (!a || !b) {
@@ -1261,15 +1267,18 @@ static void launch_sor_f32(ufe_ctx *ctx, int bs, int sched, int loads, const AaA
   }
 }
 
+// skip_first: leave out the first colour of the pass (a preconditioner that
+// alternates direction would otherwise repeat the previous pass's last colour
+// with nothing changed in between)
 static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, float2 *D = nullptr,
-                               const float2 *R = nullptr, bool backward = false) {
+                               const float2 *R = nullptr, bool backward = false, bool skip_first = false) {
   Mesh32 q;
   q.cU = ctx->d_cU32;
   q.Sxy = ctx->d_Sxy32;
   q.EV = ctx->d_EV32;
   if (!D) D = ctx->d_D32;
   if (!R) R = ctx->d_R32;
-  for (int k = 0; k < 5; k++) {
+  for (int k = skip_first ? 1 : 0; k < 5; k++) {
     const int c = backward ? 4 - k : k;
     const int n = ctx->colour_active[c];
     if (n == 0) continue;
@@ -1281,8 +1290,8 @@ static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, flo
 
 // FP64 sweep on an arbitrary vector pair (Krylov preconditioner)
 static void enqueue_sweep_fp64_vec(ufe_ctx *ctx, const AaAcMesh &m, double omega, double2 *D, const double2 *R,
-                                   bool backward) {
-  for (int k = 0; k < 5; k++) {
+                                   bool backward, bool skip_first = false) {
+  for (int k = skip_first ? 1 : 0; k < 5; k++) {
     const int c = backward ? 4 - k : k;
     const int n = ctx->colour_active[c];
     if (n == 0) continue;
@@ -1413,8 +1422,8 @@ int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) {
       lo.rel_tol = P.SSA_linear_rel_tol > 0.0 ? P.SSA_linear_rel_tol : 1e-6;
       lo.max_sweeps = 1 << 30;
       lo.precision = P.precision == UFE_MIXED ? UFE_MIXED : UFE_FP64;
-      lo.gmres_restart = P.SSA_gmres_restart > 0 ? P.SSA_gmres_restart : 30;
-      lo.precond_sweeps = P.SSA_precond_sweeps > 0 ? P.SSA_precond_sweeps : 4;
+      lo.gmres_restart = P.SSA_gmres_restart > 0 ? P.SSA_gmres_restart : 40;
+      lo.precond_sweeps = P.SSA_precond_sweeps > 0 ? P.SSA_precond_sweeps : 5;
       lo.orth_passes = 1;
       ufe_linsolve_stats ls;
       if ((r = ufe_ssa_linear_solve(ctx, &lo, &ls))) return r;
@@ -1571,10 +1580,16 @@ static int hessenberg_lsq(int m, const double *H1, const double *H2, const doubl
   return kk;
 }
 
+// colour passes of one preconditioner application of ks sweeps (the turn
+// colour is skipped unless full sweeps were asked for)
+static int64_t precond_passes(const ufe_ctx *ctx, int ks) {
+  return 5 * (int64_t)ks - (ctx->repeat_turn ? 0 : (int64_t)(ks - 1));
+}
+
 // One GMRES(m) cycle on A d = r (r held in R32 or kry_R64 with |r|^2 in d_scalar[1]),
 // then w += d in FP64.  Templated on the Krylov precision.
 extern "C++" template <typename T>
-static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passes, int *sweeps, int *matvecs) {
+static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passes, int64_t *cpass, int *matvecs) {
   using T2 = typename V2T<T>::t;
   const bool f32 = sizeof(T) == 4;
   const int n = ctx->nA;
@@ -1595,10 +1610,11 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
     T2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;
     k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
     for (int k = 0; k < ksweeps; k++) {
-      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1);
-      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1);
+      const bool skip = k > 0 && !ctx->repeat_turn;
+      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1, skip);
+      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1, skip);
     }
-    *sweeps += ksweeps;
+    *cpass += precond_passes(ctx, ksweeps);
     if (f32)
       k_matvec<float><<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32, ar, (const float2 *)zj, (float2 *)w);
     else
@@ -1643,7 +1659,7 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
 // recurrence residual falls by `reduce` or `max_sweeps` is reached; then
 // w += d in FP64.  Vectors live in the Krylov work space, scalars in bic_S.
 extern "C++" template <typename T>
-static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max_sweeps, int *sweeps,
+static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max_sweeps, int64_t *cpass,
                       int *matvecs) {
   using T2 = typename V2T<T>::t;
   const bool f32 = sizeof(T) == 4;
@@ -1670,8 +1686,9 @@ static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max
   auto precond = [&](const T2 *in, T2 *out) {
     k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(out, n, T2{T(0), T(0)});
     for (int k = 0; k < ks; k++) {
-      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)out, (const float2 *)in, k & 1);
-      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)out, (const double2 *)in, k & 1);
+      const bool skip = k > 0 && !ctx->repeat_turn;
+      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)out, (const float2 *)in, k & 1, skip);
+      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)out, (const double2 *)in, k & 1, skip);
     }
   };
   auto matvec = [&](const T2 *in, T2 *out) {
@@ -1691,7 +1708,7 @@ static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max
   const double rr0 = h[1];
   if (!(rr0 > 0.0)) return UFE_OK;
   int it = 0;
-  while (*sweeps < max_sweeps) {
+  while (*cpass / 5 < max_sweeps) {
     k_bicg_beta<<<1, 1, 0, ctx->stream>>>(S);
     k_bicg_p<T><<<nb, BLOCK, 0, ctx->stream>>>(n, r, v, p, S);
     precond(p, ph);
@@ -1706,7 +1723,7 @@ static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max
     k_bicg_update<T><<<nb, BLOCK, 0, ctx->stream>>>(n, ph, sh, s, t, x, r, S);
     dot(rh, 2, r, S + 1);                     // new rho and |r|^2: r^ and r are adjacent
     LAUNCHCHK(ctx);
-    *sweeps += 2 * ks;
+    *cpass += 2 * precond_passes(ctx, ks);
     *matvecs += 2;
     if (++it % 2 == 0) {
       HIPCHK(ctx, hipMemcpyAsync(h, S + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1726,6 +1743,7 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
   const double omega = o->omega > 0.0 ? o->omega : ctx->params.SSA_SOR_omega;
   const int check = o->check_interval > 0 ? o->check_interval : 16;
   const bool mixed = o->precision == UFE_MIXED;
+  ctx->repeat_turn = o->precond_full_sweeps != 0;
   auto t0 = std::chrono::steady_clock::now();
   double rel = 0.0;
   int r;
@@ -1748,12 +1766,14 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
     if ((r = residual_fp64(ctx, &rel, mixed))) return r;
     st->refinements = 1;
     double prev = HUGE_VAL;
-    while (rel > o->rel_tol && st->sweeps < o->max_sweeps) {
+    int64_t cp = 0;
+    while (rel > o->rel_tol && cp / 5 < o->max_sweeps) {
       if (rel > 0.99 * prev) break;
       prev = rel;
       const double reduce = std::max(0.1 * o->rel_tol / rel, mixed ? 1e-4 : 1e-13);
-      r = mixed ? bicg_inner<float>(ctx, ks, omega, reduce, o->max_sweeps, &st->sweeps, &st->matvecs)
-                : bicg_inner<double>(ctx, ks, omega, reduce, o->max_sweeps, &st->sweeps, &st->matvecs);
+      r = mixed ? bicg_inner<float>(ctx, ks, omega, reduce, o->max_sweeps, &cp, &st->matvecs)
+                : bicg_inner<double>(ctx, ks, omega, reduce, o->max_sweeps, &cp, &st->matvecs);
+      st->sweeps = (int32_t)(cp / 5);
       if (r) return r;
       if ((r = residual_fp64(ctx, &rel, mixed))) return r;
       st->refinements++;
@@ -1774,13 +1794,15 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
     if ((r = residual_fp64(ctx, &rel, true))) return r;
     st->refinements = 1;
     double prev = HUGE_VAL;
-    while (rel > o->rel_tol && st->sweeps < o->max_sweeps) {
+    int64_t cp = 0;
+    while (rel > o->rel_tol && cp / 5 < o->max_sweeps) {
       if (rel > 0.99 * prev) break;   // stagnated at the rounding floor of the residual
       prev = rel;
       int mv = 0;
       const int passes = o->orth_passes > 0 ? o->orth_passes : 2;
-      r = mixed ? gmres_cycle<float>(ctx, mr, ks, omega, passes, &st->sweeps, &mv)
-                : gmres_cycle<double>(ctx, mr, ks, omega, passes, &st->sweeps, &mv);
+      r = mixed ? gmres_cycle<float>(ctx, mr, ks, omega, passes, &cp, &mv)
+                : gmres_cycle<double>(ctx, mr, ks, omega, passes, &cp, &mv);
+      st->sweeps = (int32_t)(cp / 5);
       if (r) return r;
       st->matvecs += mv;
       if ((r = residual_fp64(ctx, &rel, true))) return r;
@@ -1908,7 +1930,7 @@ static void apply_AT(ufe_ctx *ctx, const AaAcMesh &mT, const ActiveRanges &ar, c
 static void precond_T(ufe_ctx *ctx, const AaAcMesh &mT, double omega, int ks, const double2 *v, double2 *z) {
   k_fill<double2><<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(z, ctx->nA, make_double2(0.0, 0.0));
   for (int k = 0; k < ks; k++)
-    for (int i = 0; i < 5; i++) {
+    for (int i = k > 0 ? 1 : 0; i < 5; i++) {   // the turn colour once, as in the forward preconditioner
       const int c = (k & 1) ? 4 - i : i;
       const int n = ctx->colour_active[c];
       if (n == 0) continue;
@@ -2208,8 +2230,9 @@ static void batch_enqueue_cycle(ufe_ctx *ctx, int m, int ks, double omega, doubl
     T2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;
     k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
     for (int k = 0; k < ks; k++) {
-      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1);
-      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1);
+      const bool skip = k > 0 && !ctx->repeat_turn;
+      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1, skip);
+      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1, skip);
     }
     if (f32)
       k_matvec<float><<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32, ar,
@@ -2284,6 +2307,7 @@ int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *o, double *member
     return UFE_BAD_INPUT;
   }
   const bool f32 = o->precision == UFE_MIXED;
+  ctx->repeat_turn = false;          // batch preconditioner: turn colour once
   const int m = o->gmres_restart, ks = o->precond_sweeps > 0 ? o->precond_sweeps : 4;
   const double omega = o->omega > 0.0 ? o->omega : ctx->params.SSA_SOR_omega, tol = o->rel_tol;
   const int NB = ctx->nb;
@@ -2309,6 +2333,7 @@ int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *o, double *member
   };
   double worst = 0.0;
   int conv = 0;
+  int64_t cpass = 0;
   while (true) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->h_bk_rel, ctx->bk_rel, sizeof(double) * NB, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -2321,7 +2346,8 @@ int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *o, double *member
       LAUNCHCHK(ctx);
     }
     st->cycles++;
-    st->sweeps += m * ks;
+    cpass += (int64_t)m * precond_passes(ctx, ks);
+    st->sweeps = (int32_t)(cpass / 5);
   }
   st->converged = conv;
   st->max_rel_residual = worst;

@@ -40,8 +40,8 @@ MODULE ice_dynamics_module
     ! UFE_FP64_KRYLOV solve each linearised system to SSA_linear_rel_tol by GMRES-IR
     INTEGER  :: precision = 0
     REAL(dp) :: SSA_linear_rel_tol = 1E-6_dp
-    INTEGER  :: SSA_gmres_restart = 30
-    INTEGER  :: SSA_precond_sweeps = 4
+    INTEGER  :: SSA_gmres_restart = 40
+    INTEGER  :: SSA_precond_sweeps = 5
   END TYPE type_dynamics_config
 
   TYPE(type_dynamics_config), SAVE :: C
