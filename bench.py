@@ -79,7 +79,7 @@ def main():
                     help="Krylov method of the mixed_gmres / fp64_gmres paths")
     ap.add_argument("--mode", default="both",
                     choices=["both", "mixed_gmres", "fp64_gmres", "fp64_sor"])
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-step", action="store_true", help="skip the mixed-vs-FP64 dynamics steps")
