@@ -137,6 +137,9 @@ typedef struct ufe_linsolve_opts {
                                        scalars kept on the device, FP64 refinement around it */
   int32_t precond_full_sweeps;      /* 0: successive preconditioner sweeps alternate direction and
                                        skip the colour the previous sweep ended on; 1: full sweeps */
+  int32_t precond_neumann_every_sweep; /* 0: Neumann update of the edge rows after the last sweep of a
+                                       preconditioner application only (the vector handed to the
+                                       Krylov method satisfies it); 1: after every sweep */
 } ufe_linsolve_opts;
 
 /* Statistics of one linearised solve. */

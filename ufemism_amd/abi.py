@@ -86,7 +86,7 @@ class LinSolveOpts(ct.Structure):
                 ("omega", ct.c_double), ("check_interval", ct.c_int32), ("use_graph", ct.c_int32),
                 ("gmres_restart", ct.c_int32), ("precond_sweeps", ct.c_int32),
                 ("orth_passes", ct.c_int32), ("method", ct.c_int32),
-                ("precond_full_sweeps", ct.c_int32)]
+                ("precond_full_sweeps", ct.c_int32), ("precond_neumann_every_sweep", ct.c_int32)]
 
 
 class KernelProfile(ct.Structure):
@@ -334,11 +334,12 @@ class Context:
 
     def ssa_linear_solve(self, rel_tol, max_sweeps, precision=UFE_FP64, omega=0.0,
                          check_interval=16, gmres=0, precond_sweeps=2,
-                         orth_passes=2, method=0, full_sweeps=False) -> LinSolveStats:
+                         orth_passes=2, method=0, full_sweeps=False,
+                         neumann_every_sweep=False) -> LinSolveStats:
         """method 0: GMRES(gmres) (plain SOR if gmres == 0); 1: BiCGStab."""
         o = LinSolveOpts(float(rel_tol), int(max_sweeps), int(precision), float(omega),
                          int(check_interval), 0, int(gmres), int(precond_sweeps), int(orth_passes),
-                         int(method), int(bool(full_sweeps)))
+                         int(method), int(bool(full_sweeps)), int(bool(neumann_every_sweep)))
         st = LinSolveStats()
         self._chk(self.lib.ufe_ssa_linear_solve(self.h, ct.byref(o), ct.byref(st)),
                   "ssa_linear_solve")

@@ -209,6 +209,7 @@ struct ufe_ctx {
   // autotuned SOR schedule of the batch: threads per block and block order
   int tune_bs = 256, tune_sched = 0, tune_loads = 0;
   bool repeat_turn = false;          // Krylov preconditioner: repeat the turn colour (full sweeps)
+  bool neumann_last = true;          // Krylov preconditioner: Neumann update after the last sweep only
   SorFlags *h_flags = nullptr;      // pinned
   double *h_scalar = nullptr;       // pinned
   std::vector<void *> allocs;       // everything to free with the mesh
@@ -1271,7 +1272,8 @@ static void launch_sor_f32(ufe_ctx *ctx, int bs, int sched, int loads, const AaA
 // alternates direction would otherwise repeat the previous pass's last colour
 // with nothing changed in between)
 static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, float2 *D = nullptr,
-                               const float2 *R = nullptr, bool backward = false, bool skip_first = false) {
+                               const float2 *R = nullptr, bool backward = false, bool skip_first = false,
+                               bool neumann = true) {
   Mesh32 q;
   q.cU = ctx->d_cU32;
   q.Sxy = ctx->d_Sxy32;
@@ -1284,13 +1286,14 @@ static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, flo
     if (n == 0) continue;
     launch_sor_f32(ctx, ctx->tune_bs, ctx->tune_sched, ctx->tune_loads, m, q, D, R, c, omega);
   }
+  if (!neumann) return;
   if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D);
   k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), D);
 }
 
 // FP64 sweep on an arbitrary vector pair (Krylov preconditioner)
 static void enqueue_sweep_fp64_vec(ufe_ctx *ctx, const AaAcMesh &m, double omega, double2 *D, const double2 *R,
-                                   bool backward, bool skip_first = false) {
+                                   bool backward, bool skip_first = false, bool neumann = true) {
   for (int k = skip_first ? 1 : 0; k < 5; k++) {
     const int c = backward ? 4 - k : k;
     const int n = ctx->colour_active[c];
@@ -1298,6 +1301,7 @@ static void enqueue_sweep_fp64_vec(ufe_ctx *ctx, const AaAcMesh &m, double omega
     timed_kernel(ctx, ctx->bytes64[c], n, k_sor_colour, dim3(xcd_grid(n)), dim3(BLOCK), m, D,
                  (const double2 *)ctx->d_pair[1], R, ctx->colour_start[c], n, omega, (SorFlags *)nullptr);
   }
+  if (!neumann) return;
   if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D, nullptr);
   k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), D);
 }
@@ -1611,8 +1615,9 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
     k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
     for (int k = 0; k < ksweeps; k++) {
       const bool skip = k > 0 && !ctx->repeat_turn;
-      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1, skip);
-      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1, skip);
+      const bool neu = !ctx->neumann_last || k == ksweeps - 1;
+      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1, skip, neu);
+      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1, skip, neu);
     }
     *cpass += precond_passes(ctx, ksweeps);
     if (f32)
@@ -1687,8 +1692,9 @@ static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max
     k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(out, n, T2{T(0), T(0)});
     for (int k = 0; k < ks; k++) {
       const bool skip = k > 0 && !ctx->repeat_turn;
-      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)out, (const float2 *)in, k & 1, skip);
-      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)out, (const double2 *)in, k & 1, skip);
+      const bool neu = !ctx->neumann_last || k == ks - 1;
+      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)out, (const float2 *)in, k & 1, skip, neu);
+      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)out, (const double2 *)in, k & 1, skip, neu);
     }
   };
   auto matvec = [&](const T2 *in, T2 *out) {
@@ -1744,6 +1750,7 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
   const int check = o->check_interval > 0 ? o->check_interval : 16;
   const bool mixed = o->precision == UFE_MIXED;
   ctx->repeat_turn = o->precond_full_sweeps != 0;
+  ctx->neumann_last = o->precond_neumann_every_sweep == 0;
   auto t0 = std::chrono::steady_clock::now();
   double rel = 0.0;
   int r;
@@ -2231,8 +2238,9 @@ static void batch_enqueue_cycle(ufe_ctx *ctx, int m, int ks, double omega, doubl
     k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
     for (int k = 0; k < ks; k++) {
       const bool skip = k > 0 && !ctx->repeat_turn;
-      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1, skip);
-      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1, skip);
+      const bool neu = !ctx->neumann_last || k == ks - 1;
+      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1, skip, neu);
+      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1, skip, neu);
     }
     if (f32)
       k_matvec<float><<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32, ar,
@@ -2307,7 +2315,8 @@ int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *o, double *member
     return UFE_BAD_INPUT;
   }
   const bool f32 = o->precision == UFE_MIXED;
-  ctx->repeat_turn = false;          // batch preconditioner: turn colour once
+  ctx->repeat_turn = false;          // batch preconditioner: turn colour once, Neumann at the end
+  ctx->neumann_last = true;
   const int m = o->gmres_restart, ks = o->precond_sweeps > 0 ? o->precond_sweeps : 4;
   const double omega = o->omega > 0.0 ? o->omega : ctx->params.SSA_SOR_omega, tol = o->rel_tol;
   const int NB = ctx->nb;
