@@ -85,6 +85,7 @@ def main():
     ap.add_argument("--no-step", action="store_true", help="skip the mixed-vs-FP64 dynamics steps")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble")
     ap.add_argument("--no-tune", action="store_true", help="keep the default sweep launch shape")
+    ap.add_argument("--no-ref-steps", action="store_true", help="skip the FP64 reference-step configs")
     ap.add_argument("--batch-n", type=int, default=64, help="members of the batched ensemble")
     ap.add_argument("--batch-min", type=int, default=5000)
     ap.add_argument("--batch-max", type=int, default=100_000)
@@ -274,6 +275,8 @@ def main():
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
     if not args.no_step:
         out["precision_error"] = bench_precision(args, ctx, step, wl)
+    if not args.no_ref_steps:
+        out["reference_steps"] = bench_reference_steps(args, ctx)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(args, ctx)
     if not args.no_batch:
@@ -320,6 +323,83 @@ def bench_precision(args, ctx, step, wl):
         }
     ctx.set_params(wl.params, abi.UFE_FP64)
     return res
+
+
+def bench_reference_steps(args, ctx):
+    """The FP64 configurations that run the reference arithmetic and stopping
+    rules on the GPU: the 8k dome (one full dynamics step, 10 Picard) against
+    the CPU oracle's step, and the 200k marine SSA (30 Picard to RN 1e-6)
+    against the oracle's sweep rate on a bounded sample of the same system."""
+    import numpy as np
+    from ufemism_amd import geometry
+    from ufemism_amd.dynamics import Step
+
+    out = {}
+    threads = min(16, os.cpu_count() or 1)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    # C1: dome, one step, GPU vs oracle
+    wl = geometry.dome(8000, seed=1, picard=10)
+    st = Step(ctx, wl)
+    st.run()                                   # warm-up
+    st.reset()
+    ctx.synchronize()
+    ta = time.perf_counter()
+    g = st.run()
+    tg = time.perf_counter() - ta
+    nA = ctx.sizes["nVAaAc"]
+    c1 = {"workload": f"dome nV={wl.mesh.nV} (nVAaAc={nA}), general update + SIA + SSA (10 Picard, "
+                      f"reference SOR and stopping rules, omega 1.3) + thickness update, FP64",
+          "gpu_seconds": round(tg, 4), "picard": g["picard"], "sweeps": g["sweeps"]}
+    sw = int(sum(g["sweeps"]))
+    c1["gpu_Mvertex_sweeps_per_s"] = round(nA * sw / tg / 1e6, 1)
+    if not args.no_cpu:
+        from oracle import pyoracle
+        ta = time.perf_counter()
+        ref = pyoracle.run_step(wl, threads=threads)
+        tc = time.perf_counter() - ta
+        c1.update({"cpu_seconds": round(tc, 3), "cpu_threads": threads, "cpu_sweeps": ref["sweeps"],
+                   "cpu_Mvertex_sweeps_per_s": round(nA * int(sum(ref["sweeps"])) / tc / 1e6, 1),
+                   "speedup": round(tc / tg, 2),
+                   "relL2_vs_oracle": {k: rel(g[k], ref[k]) for k in ("U_SSA", "V_SSA", "Hi")},
+                   "sweeps_equal": list(g["sweeps"]) == list(ref["sweeps"])})
+    log(f"[ref] dome 8k: {c1}")
+    out["dome8k_step"] = c1
+    # C2: marine 200k, FP64 reference SSA
+    wl = geometry.marine(200_000, seed=2, picard=30, RN_tol=1e-6)
+    st = Step(ctx, wl)
+    ctx.update_general_ice_model_data()
+    ctx.solve_SIA()
+    ctx.synchronize()
+    ta = time.perf_counter()
+    s2 = ctx.solve_SSA()
+    tg = time.perf_counter() - ta
+    nA = ctx.sizes["nVAaAc"]
+    sweeps = list(s2.sweeps)[: s2.picard_iterations]
+    c2 = {"workload": f"marine nV={wl.mesh.nV} (nVAaAc={nA}), solve_SSA: Picard to RN 1e-6 (at most 30), "
+                      f"reference SOR and stopping rules, FP64",
+          "gpu_seconds": round(tg, 3), "picard": s2.picard_iterations, "picard_converged": bool(s2.picard_converged),
+          "sweeps_total": int(sum(sweeps)),
+          "gpu_Mvertex_sweeps_per_s": round(nA * sum(sweeps) / tg / 1e6, 1)}
+    log(f"[ref] marine 200k SSA: {c2}")
+    if not args.no_cpu:
+        from oracle import pyoracle
+        ctx.ssa_prepare()
+        om = pyoracle.mesh_from_arrays(wl.mesh, st.ma.arr, st.ma.colour_nV)
+        fields = {k: ctx.get(k) for k in ("Hi_AaAc", "Hb_AaAc", "SL_AaAc", "eta_AaAc", "S_AaAc",
+                                          "dHs_dx_shelf_AaAc", "dHs_dy_shelf_AaAc", "U_SSA_AaAc", "V_SSA_AaAc")}
+        pyoracle.time_sor_sweeps(om, fields, wl.params, 2, threads)
+        dt, n = pyoracle.time_sor_sweeps(om, fields, wl.params, 200, threads)
+        rate = nA * n / dt / 1e6
+        c2.update({"cpu_Mvertex_sweeps_per_s": round(rate, 1), "cpu_threads": threads,
+                   "cpu_sample": f"{n} oracle SOR sweeps of this system, {dt:.1f}s",
+                   "cpu_seconds_estimated": round(nA * sum(sweeps) / rate / 1e6, 1),
+                   "speedup_estimated": round(nA * sum(sweeps) / rate / 1e6 / tg, 1)})
+    log(f"[ref] marine 200k SSA cpu: {c2}")
+    out["marine200k_ssa"] = c2
+    return out
 
 
 def bench_batch(args, wls):
