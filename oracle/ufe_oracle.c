@@ -99,6 +99,7 @@ static const char *ice_d_names[] = {
   "dHs_dx_shelf_Ac", "dHs_dy_shelf_Ac", "A_flow_mean_Ac",
   "D_SIA_Ac", "Ux_SIA_Ac", "Uy_SIA_Ac", "Up_SIA_Ac", "Uo_SIA_Ac",
   "Ux_SSA_Ac", "Uy_SSA_Ac", "Up_SSA_Ac", "Uo_SSA_Ac",
+  "Qabs_GL_Ac", "Qp_GL_Ac",
   "A_flow_Ac", "D_SIA_3D_Ac",
   "Hi_AaAc", "Hb_AaAc", "SL_AaAc", "dHs_dx_shelf_AaAc", "dHs_dy_shelf_AaAc",
   "A_flow_mean_AaAc", "U_SSA_AaAc", "V_SSA_AaAc",
@@ -1322,6 +1323,53 @@ int om_solve_SSA_linearised(const om_mesh *m, om_ice *ice, const om_params *p, i
 }
 
 /* :408-557 */
+
+/* ------------------------------------------------------------------------ */
+/* Grounding-line flux, src/ice_dynamics_module.f90:848-949 (the regularised */
+/* Coulomb branch: Tsai et al. 2015).  At every GL Ac point the flux follows */
+/* the flotation-crossing thickness and points down the thickness-above-     */
+/* flotation gradient; it prescribes Ux/Uy_SSA_Ac there.                     */
+/* ------------------------------------------------------------------------ */
+void om_calculate_GL_flux(const om_mesh *m, om_ice *ice) {
+  const int nAc = nAc_;
+  const double Q0 = 0.61;
+  const double *Hi = IF(OI_Hi), *Hb = IF(OI_Hb), *SL = IF(OI_SL), *phi = IF(OI_phi_fric_AaAc),
+               *Am = IF(OI_A_flow_mean);
+  const int *glA = II(OI_mask_gl_Ac), *sheet = II(OI_mask_sheet);
+  double *Qabs = IF(OI_Qabs_GL_Ac), *Qp = IF(OI_Qp_GL_Ac), *Ux = IF(OI_Ux_SSA_Ac), *Uy = IF(OI_Uy_SSA_Ac);
+  for (int a = 0; a < nAc; a++) { Qabs[a] = 0.0; Qp[a] = 0.0; }       /* :870-871 */
+  for (int aci = 1; aci <= nAc; aci++) {                                /* :878 */
+    const int a = aci - 1;
+    if (glA[a] == 0) continue;
+    const int vi = MAci(aci, 1), vj = MAci(aci, 2);
+    const double TAFi = E1(Hi, vi) - ((E1(SL, vi) - E1(Hb, vi)) * (seawater_density / ice_density));
+    const double TAFj = E1(Hi, vj) - ((E1(SL, vj) - E1(Hb, vj)) * (seawater_density / ice_density));
+    const double lambda_GL = TAFi / (TAFi - TAFj);
+    const double Hi_GL = (E1(Hi, vi) * (1.0 - lambda_GL)) + (E1(Hi, vj) * lambda_GL);
+    double phi_GL, A_GL;                                               /* grounded side, :893-899 */
+    if (E1(sheet, vi) == 1) { phi_GL = E1(phi, vi); A_GL = E1(Am, vi); }
+    else { phi_GL = E1(phi, vj); A_GL = E1(Am, vj); }
+    const double factor_Tsai = (8.0 * Q0 * A_GL * pow(ice_density * grav, n_flow) *   /* :908-911 */
+                                pow(1.0 - (ice_density / seawater_density), n_flow - 1.0) /
+                                pow(4.0, n_flow));
+    Qabs[a] = factor_Tsai * pow(Hi_GL, n_flow + 2.0) / tan(phi_GL * (pi_c / 180.0));  /* :917 */
+    double Fx = -(IF(OI_dHi_dx_Ac)[a] - ((IF(OI_dSL_dx_Ac)[a] - IF(OI_dHb_dx_Ac)[a]) *
+                                         (seawater_density / ice_density)));
+    double Fy = -(IF(OI_dHi_dy_Ac)[a] - ((IF(OI_dSL_dy_Ac)[a] - IF(OI_dHb_dy_Ac)[a]) *
+                                         (seawater_density / ice_density)));
+    const double F = sqrt(Fx * Fx + Fy * Fy);
+    Fx = Fx / F;
+    Fy = Fy / F;
+    Ux[a] = Qabs[a] * Fx / Hi_GL;                                      /* :934-935 */
+    Uy[a] = Qabs[a] * Fy / Hi_GL;
+    double Dx = MV(vj, 1) - MV(vi, 1), Dy = MV(vj, 2) - MV(vi, 2);
+    const double D = sqrt(Dx * Dx + Dy * Dy);
+    Dx = Dx / D;
+    Dy = Dy / D;
+    Qp[a] = Qabs[a] * (Dx * Fx + Dy * Fy);                             /* :944 */
+  }
+}
+
 int om_solve_SSA(const om_mesh *m, om_ice *ice, const om_params *p, om_ssa_stats *st) {
   const int nV = nV_, nAc = nAc_, nA = nVAaAc_;
   memset(st, 0, sizeof(*st));
@@ -1340,8 +1388,8 @@ int om_solve_SSA(const om_mesh *m, om_ice *ice, const om_params *p, om_ssa_stats
     memset(IF(OI_Uo_SSA_Ac), 0, sizeof(double) * (size_t)nAc);
     return OM_OK;
   }
-  if (p->use_analytical_GL_flux) return OM_BAD_INPUT;  /* calculate_GL_flux not restated */
   om_basal_yield_stress(m, ice);                                       /* :468 */
+  if (p->use_analytical_GL_flux) om_calculate_GL_flux(m, ice);         /* :471 */
   if (p->use_tau_c_override)
     memcpy(IF(OI_tau_c_AaAc), IF(OI_tau_c_override_AaAc), sizeof(double) * (size_t)nA);
   /* gather, :478-495 */

@@ -79,6 +79,7 @@ enum om_ice_d {
   OI_dHs_dx_shelf_Ac, OI_dHs_dy_shelf_Ac, OI_A_flow_mean_Ac,
   OI_D_SIA_Ac, OI_Ux_SIA_Ac, OI_Uy_SIA_Ac, OI_Up_SIA_Ac, OI_Uo_SIA_Ac,
   OI_Ux_SSA_Ac, OI_Uy_SSA_Ac, OI_Up_SSA_Ac, OI_Uo_SSA_Ac,
+  OI_Qabs_GL_Ac, OI_Qp_GL_Ac,
   /* Ac (nAc,nZ) */
   OI_A_flow_Ac, OI_D_SIA_3D_Ac,
   /* AaAc (nVAaAc) */
@@ -116,7 +117,7 @@ typedef struct {
   double SSA_max_residual_UV;
   double SSA_SOR_omega;
   int SSA_max_inner_loops;
-  int use_analytical_GL_flux;   /* only the SOR skip of GL Ac points is honoured */
+  int use_analytical_GL_flux;   /* calculate_GL_flux (Tsai 2015) at the GL Ac points, skipped by SOR */
   int ssa_forced_zero;          /* EISMINT/Halfar/Bueler benchmarks: SSA velocities set to zero */
   int zero_Hi_at_edges;         /* calculate_ice_thickness_change edge condition */
   int use_tau_c_override;       /* replace basal_yield_stress tau_c by OI_tau_c_override_AaAc */
@@ -183,6 +184,7 @@ void om_update_general_ice_model_data(const om_mesh *m, om_ice *ice);
 void om_determine_masks(const om_mesh *m, om_ice *ice);
 void om_solve_SIA(const om_mesh *m, om_ice *ice, const om_params *p);
 void om_basal_yield_stress(const om_mesh *m, om_ice *ice);
+void om_calculate_GL_flux(const om_mesh *m, om_ice *ice);
 void om_SSA_effective_viscosity(const om_mesh *m, om_ice *ice, const om_params *p);
 void om_SSA_sliding_term(const om_mesh *m, om_ice *ice);
 int  om_solve_SSA_linearised(const om_mesh *m, om_ice *ice, const om_params *p, int *did_reset,

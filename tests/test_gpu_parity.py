@@ -200,3 +200,24 @@ def test_linear_solve_bicgstab_matches_gmres(ctx, precision):
     err = np.sqrt(np.sum((u1 - u0) ** 2 + (v1 - v0) ** 2) / np.sum(u0 ** 2 + v0 ** 2))
     print(f"  bicgstab vs gmres rel L2 {err:.2e}", flush=True)
     assert err <= 1e-6
+
+
+def test_dynamics_step_with_gl_flux_matches_oracle(ctx):
+    """use_analytical_GL_flux: calculate_GL_flux prescribes the grounding-line Ac
+    velocities and the reference SOR skips those rows; the GPU step equals the
+    oracle's (sweep counts, u, v, H and the GL fluxes)."""
+    print("[gl] marine20k step with the GL flux", flush=True)
+    wl = geometry.marine(20000, seed=5, picard=10)
+    wl.A_flow = 1e-18
+    wl.params.use_analytical_GL_flux = True
+    step = Step(ctx, wl)
+    g = step.run()
+    Qg = ctx.get("Qabs_GL_Ac")
+    ref = _oracle().run_step(wl)
+    print(f"  gpu sweeps {g['sweeps']} oracle {ref['sweeps']} resets {g['stats'].resets}", flush=True)
+    assert g["sweeps"] == ref["sweeps"]
+    assert (Qg > 0).sum() > 20
+    for k in ("U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac", "Hi"):
+        e = _rel(g[k], ref[k])
+        print(f"  {k}: rel L2 {e:.2e}", flush=True)
+        assert e <= TOL_FP64, (k, e)

@@ -264,3 +264,82 @@ def test_zero_timestep_and_edges():
     ice["Hi"][:] = 100.0
     ice.calculate_ice_thickness_change(np.zeros(m.nV), np.zeros(m.nV), 1.0, noice, DynParams())
     assert np.all(ice["Hi"][noice == 1] == 0.0)
+
+
+def _gl_setup(H_scale=1.0, A=1e-16):
+    """Marine sheet on a deep bed (Hb < -1000 m near the GL, so the friction angle
+    sits at its 5 degree floor): general update and yield stress on the oracle."""
+    from ufemism_amd.dynamics import initial_fields
+    wl = geometry.marine(3000, seed=21, picard=2)
+    r = np.hypot(wl.mesh.V[:, 0], wl.mesh.V[:, 1])
+    wl.Hb = -1500.0 - 1.5 * r / 1e3
+    om = po.OracleMesh(wl.mesh).build()
+    ice = po.OracleIce(om)
+    for k, v in initial_fields(wl, om["VAaAc"], wl.params.nZ).items():
+        if k in ice.arr:
+            ice[k][...] = v
+    ice["Hi"][...] = H_scale * ice["Hi"]
+    ice["Hb"][...] = H_scale * ice["Hb"]                 # same flotation crossing at every scale
+    ice["A_flow_mean"][...] = A
+    ice.update_general_ice_model_data()
+    lib = po.load()
+    lib.om_basal_yield_stress(ct.byref(om.s), ct.byref(ice.s))
+    lib.om_calculate_GL_flux(ct.byref(om.s), ct.byref(ice.s))
+    return wl, om, ice
+
+
+def test_gl_flux_scaling_and_direction():
+    """calculate_GL_flux (src/ice_dynamics_module.f90:848-949): the Tsai (2015)
+    flux scales as A H^(n+2) (doubling thickness and depth keeps the GL and the
+    5 degree friction angle, and multiplies the flux by 2^5), points down the
+    thickness-above-flotation gradient, and Qp is its projection on vi -> vj."""
+    wl, om, ice = _gl_setup()
+    gl = ice["mask_gl_Ac"] == 1
+    assert gl.sum() > 20
+    assert np.allclose(ice["phi_fric_AaAc"][: wl.mesh.nV][ice["mask_sheet"] == 1], 5.0)
+    Q = ice["Qabs_GL_Ac"].copy()
+    assert np.all(Q[gl] > 0) and np.all(Q[~gl] == 0)
+    _, _, ice2 = _gl_setup(H_scale=2.0)
+    assert np.allclose(ice2["Qabs_GL_Ac"][gl], 32.0 * Q[gl], rtol=1e-12)
+    _, _, ice3 = _gl_setup(A=3e-16)
+    assert np.allclose(ice3["Qabs_GL_Ac"][gl], 3.0 * Q[gl], rtol=1e-12)
+    # direction: antiparallel to grad(TAF) on the Ac mesh
+    k = 1028.0 / 910.0
+    gx = ice["dHi_dx_Ac"] - (ice["dSL_dx_Ac"] - ice["dHb_dx_Ac"]) * k
+    gy = ice["dHi_dy_Ac"] - (ice["dSL_dy_Ac"] - ice["dHb_dy_Ac"]) * k
+    ux, uy = ice["Ux_SSA_Ac"][gl], ice["Uy_SSA_Ac"][gl]
+    gn = np.hypot(gx[gl], gy[gl])
+    un = np.hypot(ux, uy)
+    assert np.allclose((ux * gx[gl] + uy * gy[gl]) / (un * gn), -1.0, atol=1e-12)
+    # Qp: flux component along the Ac connection vi -> vj
+    A = om["Aci"] - 1
+    d = wl.mesh.V[A[:, 1]] - wl.mesh.V[A[:, 0]]
+    d /= np.hypot(d[:, 0], d[:, 1])[:, None]
+    cosang = (d[gl, 0] * ux + d[gl, 1] * uy) / un
+    assert np.allclose(ice["Qp_GL_Ac"][gl], Q[gl] * cosang, rtol=1e-10, atol=1e-12 * Q.max())
+    # the thickness at the crossing times the speed is the flux magnitude
+    Hi, Hb, SL = ice["Hi"], ice["Hb"], ice["SL"]
+    ti = Hi[A[gl, 0]] - (SL[A[gl, 0]] - Hb[A[gl, 0]]) * k
+    tj = Hi[A[gl, 1]] - (SL[A[gl, 1]] - Hb[A[gl, 1]]) * k
+    lam = ti / (ti - tj)
+    assert np.all((lam > 0) & (lam < 1))
+    HGL = Hi[A[gl, 0]] * (1 - lam) + Hi[A[gl, 1]] * lam
+    assert np.allclose(un * HGL, Q[gl], rtol=1e-12)
+
+
+def test_gl_flux_rows_are_prescribed_in_solve_SSA():
+    """With the GL flux on, solve_SSA keeps the GL Ac velocities it prescribed
+    (src/ice_dynamics_module.f90:624-631 skips them in every sweep).  A small
+    rate factor keeps the prescribed speeds moderate: a reset (:679-684) would
+    zero them like every other velocity."""
+    wl, om, ice = _gl_setup(A=1e-21)
+    ice["A_flow_mean_Ac"][...] = 1e-21
+    gl = ice["mask_gl_Ac"] == 1
+    ux0, uy0 = ice["Ux_SSA_Ac"][gl].copy(), ice["Uy_SSA_Ac"][gl].copy()
+    assert 10.0 < np.abs(ux0).max() < 1e5
+    p = DynParams(m_enh_sia=1.0, m_enh_ssa=1.0, SSA_max_outer_loops=3, use_tau_c_override=True,
+                  use_analytical_GL_flux=True)
+    st = ice.solve_SSA(p)
+    assert st.picard_iterations == 3 and st.resets == 0
+    assert np.array_equal(ice["Ux_SSA_Ac"][gl], ux0) and np.array_equal(ice["Uy_SSA_Ac"][gl], uy0)
+    assert np.abs(ice["Ux_SSA_Ac"][~gl]).max() > 0

@@ -65,6 +65,7 @@ static const FieldDef FIELDS[] = {
     {"Uo_SIA_Ac", G_AC, D_F64, -1, 0}, {"Ux_SSA_Ac", G_AC, D_F64, -1, 0},
     {"Uy_SSA_Ac", G_AC, D_F64, -1, 0}, {"Up_SSA_Ac", G_AC, D_F64, -1, 0},
     {"Uo_SSA_Ac", G_AC, D_F64, -1, 0},
+    {"Qabs_GL_Ac", G_AC, D_F64, -1, 0}, {"Qp_GL_Ac", G_AC, D_F64, -1, 0},
     {"A_flow_Ac", G_ACZ, D_F64, -1, 0}, {"D_SIA_3D_Ac", G_ACZ, D_F64, -1, 0},
     // AaAc (stored in colour order)
     {"Hi_AaAc", G_AAAC, D_F64, -1, 0}, {"Hb_AaAc", G_AAAC, D_F64, -1, 0},
@@ -125,8 +126,10 @@ struct ufe_ctx {
   int32_t *d_Aci = nullptr, *d_edgeAc = nullptr, *d_slot_i = nullptr, *d_slot_j = nullptr;
   double *d_NxAc = nullptr, *d_NyAc = nullptr, *d_NoAc = nullptr, *d_NpAc = nullptr;
   // AaAc colour order
-  int32_t *d_perm = nullptr;
+  int32_t *d_perm = nullptr, *d_iperm = nullptr;
   std::vector<int32_t> h_perm;
+  uint8_t *d_skip = nullptr;        // SOR rows prescribed by the grounding-line flux
+  bool gl_active = false;
   int64_t *d_chunk_ptr = nullptr;
   int32_t *d_chunk_w = nullptr, *d_col = nullptr;
   uint8_t *d_deg = nullptr;
@@ -912,6 +915,11 @@ int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
   ctx->n_cor = (int)crow.size();
   ctx->h_perm = perm;
   if ((r = dupload(ctx, &ctx->d_perm, perm.data(), nA))) return r;
+  if ((r = dupload(ctx, &ctx->d_iperm, iperm.data(), nA))) return r;
+  {
+    std::vector<uint8_t> none(nA, 0);
+    if ((r = dupload(ctx, &ctx->d_skip, none.data(), nA))) return r;
+  }
   if ((r = dupload(ctx, &ctx->d_chunk_ptr, cptr.data(), cptr.size()))) return r;
   if ((r = dupload(ctx, &ctx->d_chunk_w, cw.data(), cw.size()))) return r;
   if ((r = dupload(ctx, &ctx->d_deg, deg.data(), deg.size()))) return r;
@@ -1217,7 +1225,7 @@ static void enqueue_sweep_fp64(ufe_ctx *ctx, const AaAcMesh &m, double omega, do
     if (n == 0) continue;
     timed_kernel(ctx, ctx->bytes64[c], n, k_sor_colour, dim3(xcd_grid(n)), dim3(BLOCK), m, ctx->d_pair[0],
                  (const double2 *)ctx->d_pair[1], (const double2 *)ctx->d_pair[2], ctx->colour_start[c], n, omega,
-                 fl);
+                 fl, (const uint8_t *)(ctx->gl_active ? ctx->d_skip : nullptr));
   }
   if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], fl);
   if (with_flags)
@@ -1299,7 +1307,8 @@ static void enqueue_sweep_fp64_vec(ufe_ctx *ctx, const AaAcMesh &m, double omega
     const int n = ctx->colour_active[c];
     if (n == 0) continue;
     timed_kernel(ctx, ctx->bytes64[c], n, k_sor_colour, dim3(xcd_grid(n)), dim3(BLOCK), m, D,
-                 (const double2 *)ctx->d_pair[1], R, ctx->colour_start[c], n, omega, (SorFlags *)nullptr);
+                 (const double2 *)ctx->d_pair[1], R, ctx->colour_start[c], n, omega, (SorFlags *)nullptr,
+                 (const uint8_t *)nullptr);
   }
   if (!neumann) return;
   if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D, nullptr);
@@ -1388,8 +1397,8 @@ int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) {
   if (!st) st = &local;
   std::memset(st, 0, sizeof(*st));
   const ufe_params &P = ctx->params;
-  if (P.use_analytical_GL_flux) {
-    ctx->err = "use_analytical_GL_flux is not supported";
+  if (P.use_analytical_GL_flux && P.precision != UFE_FP64) {
+    ctx->err = "use_analytical_GL_flux runs with the reference SOR (precision UFE_FP64)";
     return st->status = UFE_NOT_SUPPORTED;
   }
   auto t0 = std::chrono::steady_clock::now();
@@ -1408,6 +1417,25 @@ int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) {
     return UFE_OK;
   }
   if ((r = ssa_gather(ctx))) return r;
+  // calculate_GL_flux (:471) prescribes the GL Ac velocities; SOR then skips those rows
+  struct GLGuard {
+    ufe_ctx *c;
+    ~GLGuard() { c->gl_active = false; }
+  } gl_guard{ctx};
+  ctx->gl_active = P.use_analytical_GL_flux != 0;
+  if (ctx->gl_active) {
+    GLFields g;
+    g.Hi = F(ctx, "Hi"); g.Hb = F(ctx, "Hb"); g.SL = F(ctx, "SL"); g.Am = F(ctx, "A_flow_mean");
+    g.sheet = FI(ctx, "mask_sheet"); g.phi = F(ctx, "phi_fric_AaAc"); g.iperm = ctx->d_iperm;
+    g.glA = FI(ctx, "mask_gl_Ac");
+    g.dHi_dx = F(ctx, "dHi_dx_Ac"); g.dHi_dy = F(ctx, "dHi_dy_Ac"); g.dSL_dx = F(ctx, "dSL_dx_Ac");
+    g.dSL_dy = F(ctx, "dSL_dy_Ac"); g.dHb_dx = F(ctx, "dHb_dx_Ac"); g.dHb_dy = F(ctx, "dHb_dy_Ac");
+    g.Qabs = F(ctx, "Qabs_GL_Ac"); g.Qp = F(ctx, "Qp_GL_Ac"); g.Ux = F(ctx, "Ux_SSA_Ac"); g.Uy = F(ctx, "Uy_SSA_Ac");
+    g.W = ctx->d_pair[0];
+    g.skip = ctx->d_skip;
+    k_gl_flux<<<nblk(ctx->nAc), BLOCK, 0, ctx->stream>>>(ac_mesh(ctx), g);
+    LAUNCHCHK(ctx);
+  }
   int it = 0, did_reset_before = 0, did_reset_now = 0;
   while (it < P.SSA_max_outer_loops) {
     it++;
@@ -1942,7 +1970,7 @@ static void precond_T(ufe_ctx *ctx, const AaAcMesh &mT, double omega, int ks, co
       const int n = ctx->colour_active[c];
       if (n == 0) continue;
       k_sor_colour<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(mT, z, ctx->d_pair[1], v, ctx->colour_start[c], n,
-                                                         omega, nullptr);
+                                                         omega, nullptr, nullptr);
     }
 }
 

@@ -34,6 +34,7 @@ MODULE ice_dynamics_module
     REAL(dp) :: SSA_max_residual_UV = 2.5_dp
     REAL(dp) :: SSA_SOR_omega = 1.2_dp
     INTEGER  :: SSA_max_inner_loops = 10000
+    LOGICAL  :: use_analytical_GL_flux = .FALSE.
     LOGICAL  :: ssa_forced_zero = .FALSE.
     LOGICAL  :: zero_Hi_at_edges = .TRUE.
     ! linear solves of solve_SSA: UFE_FP64 runs the reference SOR; UFE_MIXED and
@@ -172,7 +173,7 @@ CONTAINS
     p%SSA_max_residual_UV = ncfg%SSA_max_residual_UV
     p%SSA_SOR_omega = ncfg%SSA_SOR_omega
     p%SSA_max_inner_loops = ncfg%SSA_max_inner_loops
-    p%use_analytical_GL_flux = 0
+    p%use_analytical_GL_flux = MERGE(1, 0, ncfg%use_analytical_GL_flux)
     p%ssa_forced_zero = MERGE(1, 0, ncfg%ssa_forced_zero)
     p%zero_Hi_at_edges = MERGE(1, 0, ncfg%zero_Hi_at_edges)
     p%use_tau_c_override = 0
