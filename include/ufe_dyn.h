@@ -99,7 +99,18 @@ typedef struct ufe_params {
   double SSA_linear_rel_tol;        /* <= 0: 1e-6 */
   int32_t SSA_gmres_restart;        /* <= 0: 40 */
   int32_t SSA_precond_sweeps;       /* <= 0: 5 */
+  /* ice_physical_properties (src/general_ice_model_data_module.f90:298-462),
+   * run at the end of update_general_ice_model_data (:91) when non-zero:
+   * UFE_PHYS_INPUT (0): A_flow, A_flow_Ac, A_flow_mean, A_flow_mean_Ac are inputs;
+   * UFE_PHYS_ARRHENIUS (1): A from Ti by the Arrhenius law, Ti_pmp, Cpi, Ki (:370-460);
+   * UFE_PHYS_BENCHMARK_FIXED (2): the fixed values of the EISMINT/Halfar/Bueler
+   * benchmarks (:323-348). */
+  int32_t physical_properties;
 } ufe_params;
+
+#define UFE_PHYS_INPUT 0
+#define UFE_PHYS_ARRHENIUS 1
+#define UFE_PHYS_BENCHMARK_FIXED 2
 
 /* Statistics of one solve_SSA call. */
 typedef struct ufe_ssa_stats {
@@ -203,9 +214,15 @@ int ufe_field_get_device(ufe_ctx *ctx, int id, void *dev);
 int ufe_field_fill(ufe_ctx *ctx, int id, double value);     /* every element := value */
 
 /* ---- the hot path (operate on the device-resident fields) --------------- */
-/* update_general_ice_model_data, src/general_ice_model_data_module.f90:23-96
- * (without ice_physical_properties: A_flow fields are inputs). */
+/* update_general_ice_model_data, src/general_ice_model_data_module.f90:23-96;
+ * it ends with ice_physical_properties when ufe_params.physical_properties is
+ * not UFE_PHYS_INPUT (otherwise the A_flow fields are inputs). */
 int ufe_update_general_ice_model_data(ufe_ctx *ctx);
+/* ice_physical_properties, src/general_ice_model_data_module.f90:298-462, on
+ * its own: mode UFE_PHYS_ARRHENIUS maps Ti to Ti_Ac (:40-43) and sets Ti_pmp,
+ * A_flow, Cpi, Ki, A_flow_mean, A_flow_Ac and A_flow_mean_Ac from Ti and the
+ * sheet masks; UFE_PHYS_BENCHMARK_FIXED sets the benchmark constants. */
+int ufe_ice_physical_properties(ufe_ctx *ctx, int mode);
 /* solve_SIA, src/ice_dynamics_module.f90:240-314 */
 int ufe_solve_SIA(ufe_ctx *ctx);
 /* solve_SSA, src/ice_dynamics_module.f90:408-557 (with basal_yield_stress

@@ -181,6 +181,10 @@ class OracleIce:
     def update_general_ice_model_data(self):
         self.lib.om_update_general_ice_model_data(self._m(), ct.byref(self.s))
 
+    def ice_physical_properties(self, p: DynParams, mode=1):
+        ps = params_struct(p)
+        self.lib.om_ice_physical_properties(self._m(), ct.byref(self.s), ct.byref(ps), ct.c_int(mode))
+
     def solve_SIA(self, p: DynParams):
         ps = params_struct(p)
         self.lib.om_solve_SIA(self._m(), ct.byref(self.s), ct.byref(ps))
@@ -228,6 +232,8 @@ def run_step(wl, threads=None):
         if k in ice.arr:
             ice[k][...] = v
     ice.update_general_ice_model_data()
+    if wl.params.physical_properties:
+        ice.ice_physical_properties(wl.params, wl.params.physical_properties)
     ice.solve_SIA(wl.params)
     st = ice.solve_SSA(wl.params)
     ice.calculate_ice_thickness_change(wl.SMB, wl.BMB, wl.dt,

@@ -90,7 +90,7 @@ static const char *ice_d_names[] = {
   "Hi", "Hb", "SL", "Hs", "dHb_dt", "dHi_dt", "dHs_dt", "Hi_prev",
   "A_flow_mean", "dHi_dx", "dHi_dy", "dHs_dx", "dHs_dy",
   "dHs_dx_shelf", "dHs_dy_shelf", "U_SIA", "V_SIA", "D_SIA", "U_SSA", "V_SSA",
-  "A_flow", "dVi_in",
+  "A_flow", "Ti", "Ti_pmp", "Cpi", "Ki", "dVi_in",
   "Hi_Ac", "Hb_Ac", "SL_Ac", "Hs_Ac",
   "dHi_dx_Ac", "dHi_dy_Ac", "dHi_dp_Ac", "dHi_do_Ac",
   "dHb_dx_Ac", "dHb_dy_Ac", "dHb_dp_Ac", "dHb_do_Ac",
@@ -100,7 +100,7 @@ static const char *ice_d_names[] = {
   "D_SIA_Ac", "Ux_SIA_Ac", "Uy_SIA_Ac", "Up_SIA_Ac", "Uo_SIA_Ac",
   "Ux_SSA_Ac", "Uy_SSA_Ac", "Up_SSA_Ac", "Uo_SSA_Ac",
   "Qabs_GL_Ac", "Qp_GL_Ac",
-  "A_flow_Ac", "D_SIA_3D_Ac",
+  "A_flow_Ac", "D_SIA_3D_Ac", "Ti_Ac",
   "Hi_AaAc", "Hb_AaAc", "SL_AaAc", "dHs_dx_shelf_AaAc", "dHs_dy_shelf_AaAc",
   "A_flow_mean_AaAc", "U_SSA_AaAc", "V_SSA_AaAc",
   "dU_SSA_dx_AaAc", "dU_SSA_dy_AaAc", "dV_SSA_dx_AaAc", "dV_SSA_dy_AaAc",
@@ -1106,6 +1106,66 @@ void om_update_general_ice_model_data(const om_mesh *m, om_ice *ice) {
       IF(OI_dHs_dx_shelf_Ac)[a] = (1.0 - ice_density / seawater_density) * IF(OI_dHi_dx_Ac)[a];
       IF(OI_dHs_dy_shelf_Ac)[a] = (1.0 - ice_density / seawater_density) * IF(OI_dHi_dy_Ac)[a];
     }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Ice physical properties, src/general_ice_model_data_module.f90:298-462    */
+/* (the non-benchmark branch), after Ti -> Ti_Ac (:40-43).                   */
+/* ------------------------------------------------------------------------ */
+static double arrhenius(double T) {                                    /* :379-383 */
+  const double A_low_temp = 1.14E-05, A_high_temp = 5.47E+10, Q_low_temp = 6.0E+04,
+               Q_high_temp = 13.9E+04, R_gas = 8.314;
+  if (T < 263.15) return A_low_temp * exp(-Q_low_temp / (R_gas * T));
+  return A_high_temp * exp(-Q_high_temp / (R_gas * T));
+}
+
+/* mode 1: the temperature-dependent branch (:370-460); mode 2: the fixed
+ * values of the EISMINT / Halfar / Bueler / mesh_generation_test benchmarks
+ * (:323-348), which set no Ac temperature and no Ti-dependent field. */
+void om_ice_physical_properties(const om_mesh *m, om_ice *ice, const om_params *p, int mode) {
+  const int nV = nV_, nAc = nAc_, nZ = p->nZ;
+  const double T0 = 273.16, CC = 8.7E-04, SMT = 271.15;               /* src/parameters_module.f90 */
+  const double sec_per_year = 31556943.36;
+  double *A = IF(OI_A_flow), *Am = IF(OI_A_flow_mean), *Tp = IF(OI_Ti_pmp), *Cp = IF(OI_Cpi),
+         *Ki = IF(OI_Ki);
+  const double *Ti = IF(OI_Ti), *Hi = IF(OI_Hi);
+  if (mode == 2) {
+    for (int k = 0; k < nZ; k++)
+      for (int v = 0; v < nV; v++) {
+        A[(size_t)k * nV + v] = 1.0E-16;
+        Ki[(size_t)k * nV + v] = 2.1 * sec_per_year;
+        Cp[(size_t)k * nV + v] = 2009.0;
+        Tp[(size_t)k * nV + v] = T0 - (p->zeta[k] * Hi[v] * 8.7E-04);
+      }
+    for (int v = 0; v < nV; v++) Am[v] = 1.0E-16;
+    for (size_t i = 0; i < (size_t)nAc * nZ; i++) IF(OI_A_flow_Ac)[i] = 1.0E-16;
+    for (int a = 0; a < nAc; a++) IF(OI_A_flow_mean_Ac)[a] = 1.0E-16;
+    return;
+  }
+  om_map_Aa_to_Ac_3D(m, IF(OI_Ti), IF(OI_Ti_Ac), nZ);                  /* :40-43 */
+  double prof[32];
+  for (int v = 0; v < nV; v++) {                                       /* :370-408 */
+    for (int k = 0; k < nZ; k++) {
+      const double T = Ti[(size_t)k * nV + v];
+      Tp[(size_t)k * nV + v] = T0 - CC * Hi[v] * p->zeta[k];
+      A[(size_t)k * nV + v] = arrhenius(T);
+      Cp[(size_t)k * nV + v] = 2115.3 + 7.79293 * (T - T0);
+      Ki[(size_t)k * nV + v] = 3.101E+08 * exp(-0.0057 * T);
+      prof[k] = A[(size_t)k * nV + v];
+    }
+    if (II(OI_mask_sheet)[v] == 1) Am[v] = om_vertical_average(p, prof);
+    else Am[v] = arrhenius((Ti[v] + SMT) / 2.0);
+  }
+  double *AA = IF(OI_A_flow_Ac), *AmA = IF(OI_A_flow_mean_Ac);
+  const double *TA = IF(OI_Ti_Ac);
+  for (int a = 0; a < nAc; a++) {                                      /* :413-460 */
+    for (int k = 0; k < nZ; k++) {
+      AA[(size_t)k * nAc + a] = arrhenius(TA[(size_t)k * nAc + a]);
+      prof[k] = AA[(size_t)k * nAc + a];
+    }
+    if (II(OI_mask_sheet_Ac)[a] == 1) AmA[a] = om_vertical_average(p, prof);
+    else AmA[a] = arrhenius((TA[a] + SMT) / 2.0);
   }
 }
 

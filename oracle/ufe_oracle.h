@@ -67,7 +67,7 @@ enum om_ice_d {
   OI_A_flow_mean, OI_dHi_dx, OI_dHi_dy, OI_dHs_dx, OI_dHs_dy,
   OI_dHs_dx_shelf, OI_dHs_dy_shelf, OI_U_SIA, OI_V_SIA, OI_D_SIA, OI_U_SSA, OI_V_SSA,
   /* Aa (nV,nZ) */
-  OI_A_flow,
+  OI_A_flow, OI_Ti, OI_Ti_pmp, OI_Cpi, OI_Ki,
   /* Aa (nV,nC_mem) */
   OI_dVi_in,
   /* Ac (nAc) */
@@ -81,7 +81,7 @@ enum om_ice_d {
   OI_Ux_SSA_Ac, OI_Uy_SSA_Ac, OI_Up_SSA_Ac, OI_Uo_SSA_Ac,
   OI_Qabs_GL_Ac, OI_Qp_GL_Ac,
   /* Ac (nAc,nZ) */
-  OI_A_flow_Ac, OI_D_SIA_3D_Ac,
+  OI_A_flow_Ac, OI_D_SIA_3D_Ac, OI_Ti_Ac,
   /* AaAc (nVAaAc) */
   OI_Hi_AaAc, OI_Hb_AaAc, OI_SL_AaAc, OI_dHs_dx_shelf_AaAc, OI_dHs_dy_shelf_AaAc,
   OI_A_flow_mean_AaAc, OI_U_SSA_AaAc, OI_V_SSA_AaAc,
@@ -181,6 +181,7 @@ void om_rotate_xy_to_po(const om_mesh *m, const double *x, const double *y, doub
 double om_vertical_average(const om_params *p, const double *f);
 void om_vertical_integrate(const om_params *p, const double *f, double *int_f);
 void om_update_general_ice_model_data(const om_mesh *m, om_ice *ice);
+void om_ice_physical_properties(const om_mesh *m, om_ice *ice, const om_params *p, int mode);
 void om_determine_masks(const om_mesh *m, om_ice *ice);
 void om_solve_SIA(const om_mesh *m, om_ice *ice, const om_params *p);
 void om_basal_yield_stress(const om_mesh *m, om_ice *ice);

@@ -32,7 +32,7 @@ EXPORTS = [
     "ufe_mesh_build", "ufe_mesh_upload", "ufe_mesh_sizes",
     "ufe_field_count", "ufe_field_name", "ufe_field_id", "ufe_field_info", "ufe_field_set",
     "ufe_field_get", "ufe_field_set_device", "ufe_field_get_device",
-    "ufe_update_general_ice_model_data", "ufe_solve_SIA", "ufe_solve_SSA",
+    "ufe_update_general_ice_model_data", "ufe_ice_physical_properties", "ufe_solve_SIA", "ufe_solve_SSA",
     "ufe_calculate_ice_thickness_change", "ufe_ssa_prepare", "ufe_ssa_linear_solve",
     "ufe_ssa_residual",
     "ufe_op_derivatives_Aa", "ufe_op_curvatures_Aa", "ufe_op_derivatives_Aa_3D",
@@ -71,7 +71,7 @@ class Params(ct.Structure):
                 ("zero_Hi_at_edges", ct.c_int32), ("use_tau_c_override", ct.c_int32),
                 ("precision", ct.c_int32), ("sweep_batch", ct.c_int32),
                 ("SSA_linear_rel_tol", ct.c_double), ("SSA_gmres_restart", ct.c_int32),
-                ("SSA_precond_sweeps", ct.c_int32)]
+                ("SSA_precond_sweeps", ct.c_int32), ("physical_properties", ct.c_int32)]
 
 
 class SSAStats(ct.Structure):
@@ -208,6 +208,7 @@ def params_struct(p: DynParams, precision=UFE_FP64, sweep_batch=0, linear_rel_to
     s.SSA_linear_rel_tol = float(linear_rel_tol)
     s.SSA_gmres_restart = int(gmres_restart)
     s.SSA_precond_sweeps = int(precond_sweeps)
+    s.physical_properties = int(p.physical_properties)
     return s
 
 
@@ -314,6 +315,10 @@ class Context:
 
     def update_general_ice_model_data(self):
         self._chk(self.lib.ufe_update_general_ice_model_data(self.h), "update_general_ice_model_data")
+
+    def ice_physical_properties(self, mode=1):
+        """ice_physical_properties on its own (mode 1 Arrhenius from Ti, 2 fixed benchmark values)."""
+        self._chk(self.lib.ufe_ice_physical_properties(self.h, int(mode)), "ice_physical_properties")
 
     def solve_SIA(self):
         self._chk(self.lib.ufe_solve_SIA(self.h), "solve_SIA")

@@ -31,6 +31,9 @@ class DynParams:
     ssa_forced_zero: bool = False
     zero_Hi_at_edges: bool = True
     use_tau_c_override: bool = False
+    # ice_physical_properties: 0 A fields are inputs, 1 Arrhenius from Ti,
+    # 2 the fixed benchmark values (src/general_ice_model_data_module.f90:298-462)
+    physical_properties: int = 0
 
     @property
     def nZ(self) -> int:

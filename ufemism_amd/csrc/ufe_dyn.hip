@@ -46,7 +46,8 @@ static const FieldDef FIELDS[] = {
     {"U_SIA", G_AA, D_F64, -1, 0}, {"V_SIA", G_AA, D_F64, -1, 0}, {"D_SIA", G_AA, D_F64, -1, 0},
     {"U_SSA", G_AA, D_F64, -1, 0}, {"V_SSA", G_AA, D_F64, -1, 0},
     {"SMB_year", G_AA, D_F64, -1, 0}, {"BMB", G_AA, D_F64, -1, 0},
-    {"A_flow", G_AAZ, D_F64, -1, 0}, {"dVi_in", G_AAC, D_F64, -1, 0},
+    {"A_flow", G_AAZ, D_F64, -1, 0}, {"Ti", G_AAZ, D_F64, -1, 0}, {"Ti_pmp", G_AAZ, D_F64, -1, 0},
+    {"Cpi", G_AAZ, D_F64, -1, 0}, {"Ki", G_AAZ, D_F64, -1, 0}, {"dVi_in", G_AAC, D_F64, -1, 0},
     // Ac
     {"Hi_Ac", G_AC, D_F64, -1, 0}, {"Hb_Ac", G_AC, D_F64, -1, 0}, {"SL_Ac", G_AC, D_F64, -1, 0},
     {"Hs_Ac", G_AC, D_F64, -1, 0},
@@ -67,6 +68,7 @@ static const FieldDef FIELDS[] = {
     {"Uo_SSA_Ac", G_AC, D_F64, -1, 0},
     {"Qabs_GL_Ac", G_AC, D_F64, -1, 0}, {"Qp_GL_Ac", G_AC, D_F64, -1, 0},
     {"A_flow_Ac", G_ACZ, D_F64, -1, 0}, {"D_SIA_3D_Ac", G_ACZ, D_F64, -1, 0},
+    {"Ti_Ac", G_ACZ, D_F64, -1, 0},
     // AaAc (stored in colour order)
     {"Hi_AaAc", G_AAAC, D_F64, -1, 0}, {"Hb_AaAc", G_AAAC, D_F64, -1, 0},
     {"SL_AaAc", G_AAAC, D_F64, -1, 0}, {"dHs_dx_shelf_AaAc", G_AAAC, D_F64, -1, 0},
@@ -1113,13 +1115,55 @@ static GenFields gen_fields(ufe_ctx *c) {
   return f;
 }
 
+static int enqueue_physical_properties(ufe_ctx *ctx, int mode) {
+  PhysParams p;
+  p.nZ = ctx->nZ;
+  p.mode = mode;
+  for (int k = 0; k < ctx->nZ; k++) p.zeta[k] = ctx->params.zeta[k];
+  k_phys_aa<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(ctx->nV, p, F(ctx, "Ti"), F(ctx, "Hi"),
+                                                       FI(ctx, "mask_sheet"), F(ctx, "A_flow"),
+                                                       F(ctx, "A_flow_mean"), F(ctx, "Ti_pmp"),
+                                                       F(ctx, "Cpi"), F(ctx, "Ki"));
+  k_phys_ac<<<nblk(ctx->nAc), BLOCK, 0, ctx->stream>>>(ac_mesh(ctx), p, F(ctx, "Ti"), FI(ctx, "mask_sheet_Ac"),
+                                                        F(ctx, "Ti_Ac"), F(ctx, "A_flow_Ac"),
+                                                        F(ctx, "A_flow_mean_Ac"));
+  LAUNCHCHK(ctx);
+  return UFE_OK;
+}
+
+static bool phys_mode_ok(int mode) {
+  return mode == UFE_PHYS_ARRHENIUS || mode == UFE_PHYS_BENCHMARK_FIXED;
+}
+
 int ufe_update_general_ice_model_data(ufe_ctx *ctx) {
   if (!ctx || !ctx->have_mesh) return UFE_BAD_INPUT;
+  const int mode = ctx->params.physical_properties;
+  if (mode != UFE_PHYS_INPUT && !phys_mode_ok(mode)) {
+    ctx->err = "physical_properties must be UFE_PHYS_INPUT, UFE_PHYS_ARRHENIUS or UFE_PHYS_BENCHMARK_FIXED";
+    return UFE_BAD_INPUT;
+  }
   GenFields f = gen_fields(ctx);
   k_gen_aa1<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(ctx->nV, f);
   k_gen_ac<<<nblk(ctx->nAc), BLOCK, 0, ctx->stream>>>(ac_mesh(ctx), f);
   k_gen_aa2<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(aa_mesh(ctx), f);
   LAUNCHCHK(ctx);
+  // ice_physical_properties closes the routine (src/general_ice_model_data_module.f90:91)
+  if (mode != UFE_PHYS_INPUT) {
+    const int st = enqueue_physical_properties(ctx, mode);
+    if (st != UFE_OK) return st;
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return UFE_OK;
+}
+
+int ufe_ice_physical_properties(ufe_ctx *ctx, int mode) {
+  if (!ctx || !ctx->have_mesh) return UFE_BAD_INPUT;
+  if (!phys_mode_ok(mode)) {
+    ctx->err = "ufe_ice_physical_properties: mode must be UFE_PHYS_ARRHENIUS or UFE_PHYS_BENCHMARK_FIXED";
+    return UFE_BAD_INPUT;
+  }
+  const int st = enqueue_physical_properties(ctx, mode);
+  if (st != UFE_OK) return st;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return UFE_OK;
 }

@@ -21,7 +21,8 @@ OUTPUTS = ("U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac", "Up_SSA_Ac", "Uo_SSA_Ac",
 def initial_fields(wl, VAaAc, nZ):
     """Initial ice state of a workload, keyed by reference field name."""
     nV = wl.mesh.nV
-    return {
+    extra = {"Ti": wl.temperature()} if wl.params.physical_properties == 1 else {}
+    return extra | {
         "Hi": wl.Hi, "Hb": wl.Hb, "SL": wl.SL,
         "A_flow": np.full((nV, nZ), wl.A_flow), "A_flow_mean": np.full(nV, wl.A_flow),
         "A_flow_Ac": wl.A_flow, "A_flow_mean_Ac": wl.A_flow,

@@ -59,11 +59,14 @@ ICE_FIELDS = {}
 for _n in _AA:
     ICE_FIELDS[_n] = ("Aa", F64)
 ICE_FIELDS["A_flow"] = ("Aaz", F64)
+for _n in ("Ti", "Ti_pmp", "Cpi", "Ki"):
+    ICE_FIELDS[_n] = ("Aaz", F64)
 ICE_FIELDS["dVi_in"] = ("Aac", F64)
 for _n in _AC:
     ICE_FIELDS[_n] = ("Ac", F64)
 ICE_FIELDS["A_flow_Ac"] = ("Acz", F64)
 ICE_FIELDS["D_SIA_3D_Ac"] = ("Acz", F64)
+ICE_FIELDS["Ti_Ac"] = ("Acz", F64)
 for _n in _AAAC:
     ICE_FIELDS[_n] = ("AaAc", F64)
 for _n in _MASKS:

@@ -43,6 +43,9 @@ MODULE ice_dynamics_module
     REAL(dp) :: SSA_linear_rel_tol = 1E-6_dp
     INTEGER  :: SSA_gmres_restart = 40
     INTEGER  :: SSA_precond_sweeps = 5
+    ! ice_physical_properties at the end of update_general_ice_model_data:
+    ! 0 the A_flow fields are inputs, 1 Arrhenius law from Ti, 2 fixed benchmark values
+    INTEGER  :: physical_properties = 0
   END TYPE type_dynamics_config
 
   TYPE(type_dynamics_config), SAVE :: C
@@ -67,12 +70,14 @@ MODULE ice_dynamics_module
     ! Aa
     REAL(dp), ALLOCATABLE :: Hi(:), Hb(:), SL(:), Hs(:), dHb_dt(:), dHi_dt(:), dHs_dt(:), Hi_prev(:)
     REAL(dp), ALLOCATABLE :: A_flow(:,:), A_flow_mean(:)
+    REAL(dp), ALLOCATABLE :: Ti(:,:), Ti_pmp(:,:), Cpi(:,:), Ki(:,:)
     REAL(dp), ALLOCATABLE :: dHi_dx(:), dHi_dy(:), dHs_dx(:), dHs_dy(:), dHs_dx_shelf(:), dHs_dy_shelf(:)
     REAL(dp), ALLOCATABLE :: U_SIA(:), V_SIA(:), D_SIA(:), U_SSA(:), V_SSA(:), dVi_in(:,:)
     INTEGER,  ALLOCATABLE :: mask_land(:), mask_ocean(:), mask_lake(:), mask_ice(:), mask_sheet(:), &
                              mask_shelf(:), mask_coast(:), mask_margin(:), mask_gl(:), mask_cf(:), mask(:)
     ! Ac
     REAL(dp), ALLOCATABLE :: Hi_Ac(:), Hb_Ac(:), SL_Ac(:), Hs_Ac(:), A_flow_Ac(:,:), A_flow_mean_Ac(:)
+    REAL(dp), ALLOCATABLE :: Ti_Ac(:,:)
     REAL(dp), ALLOCATABLE :: dHi_dx_Ac(:), dHi_dy_Ac(:), dHi_dp_Ac(:), dHi_do_Ac(:)
     REAL(dp), ALLOCATABLE :: dHb_dx_Ac(:), dHb_dy_Ac(:), dHb_dp_Ac(:), dHb_do_Ac(:)
     REAL(dp), ALLOCATABLE :: dHs_dx_Ac(:), dHs_dy_Ac(:), dHs_dp_Ac(:), dHs_do_Ac(:)
@@ -182,6 +187,7 @@ CONTAINS
     p%SSA_linear_rel_tol = ncfg%SSA_linear_rel_tol
     p%SSA_gmres_restart = ncfg%SSA_gmres_restart
     p%SSA_precond_sweeps = ncfg%SSA_precond_sweeps
+    p%physical_properties = ncfg%physical_properties
   END FUNCTION params_of
 
   ! ---- mesh set-up -----------------------------------------------------------
@@ -284,6 +290,8 @@ CONTAINS
              ice%Hi_prev(nV), ice%A_flow(nV, nZ), ice%A_flow_mean(nV), ice%dHi_dx(nV), ice%dHi_dy(nV), &
              ice%dHs_dx(nV), ice%dHs_dy(nV), ice%dHs_dx_shelf(nV), ice%dHs_dy_shelf(nV), ice%U_SIA(nV), &
              ice%V_SIA(nV), ice%D_SIA(nV), ice%U_SSA(nV), ice%V_SSA(nV), ice%dVi_in(nV, mesh%nC_mem), source=0._dp)
+    ALLOCATE(ice%Ti(nV, nZ), ice%Ti_pmp(nV, nZ), ice%Cpi(nV, nZ), ice%Ki(nV, nZ), ice%Ti_Ac(nAc, nZ), &
+             source=0._dp)
     ALLOCATE(ice%mask_land(nV), ice%mask_ocean(nV), ice%mask_lake(nV), ice%mask_ice(nV), ice%mask_sheet(nV), &
              ice%mask_shelf(nV), ice%mask_coast(nV), ice%mask_margin(nV), ice%mask_gl(nV), ice%mask_cf(nV), &
              ice%mask(nV), source=0)
@@ -314,9 +322,21 @@ CONTAINS
       CALL put(mesh, 'SL', ice%SL)
       CALL put(mesh, 'dHb_dt', ice%dHb_dt)
       CALL put(mesh, 'dHi_dt', ice%dHi_dt)
+      IF (C%physical_properties == 1) CALL put(mesh, 'Ti', ice%Ti)
     END IF
     CALL check(mesh, ufe_update_general_ice_model_data(mesh%ufe), 'update_general_ice_model_data')
     IF (.NOT. ufe_host_sync) RETURN
+    IF (C%physical_properties /= 0) THEN
+      ! ice_physical_properties (src/general_ice_model_data_module.f90:298-462)
+      CALL get(mesh, 'A_flow', ice%A_flow)
+      CALL get(mesh, 'A_flow_mean', ice%A_flow_mean)
+      CALL get(mesh, 'Ti_pmp', ice%Ti_pmp)
+      CALL get(mesh, 'A_flow_Ac', ice%A_flow_Ac)
+      CALL get(mesh, 'A_flow_mean_Ac', ice%A_flow_mean_Ac)
+      IF (C%physical_properties == 1) CALL get(mesh, 'Ti_Ac', ice%Ti_Ac)
+      CALL get(mesh, 'Cpi', ice%Cpi)
+      CALL get(mesh, 'Ki', ice%Ki)
+    END IF
     CALL get(mesh, 'Hs', ice%Hs)
     CALL get(mesh, 'dHs_dt', ice%dHs_dt)
     CALL get(mesh, 'dHi_dx', ice%dHi_dx)

@@ -38,6 +38,7 @@ MODULE ufe_c_binding
     INTEGER(C_INT32_T) :: use_tau_c_override, precision, sweep_batch
     REAL(C_DOUBLE)     :: SSA_linear_rel_tol
     INTEGER(C_INT32_T) :: SSA_gmres_restart, SSA_precond_sweeps
+    INTEGER(C_INT32_T) :: physical_properties
   END TYPE ufe_params
 
   TYPE, BIND(C) :: ufe_ssa_stats
@@ -103,6 +104,12 @@ MODULE ufe_c_binding
     FUNCTION ufe_update_general_ice_model_data(ctx) BIND(C, name="ufe_update_general_ice_model_data") RESULT(r)
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT) :: r
+    END FUNCTION
+    FUNCTION ufe_ice_physical_properties(ctx, mode) BIND(C, name="ufe_ice_physical_properties") RESULT(r)
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: mode
       INTEGER(C_INT) :: r
     END FUNCTION
     FUNCTION ufe_solve_SIA(ctx) BIND(C, name="ufe_solve_SIA") RESULT(r)

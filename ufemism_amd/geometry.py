@@ -92,6 +92,17 @@ class Workload:
             tau = tau * np.exp(self.friction_sigma * g)
         return tau
 
+    def temperature(self) -> np.ndarray:
+        """Englacial temperature Ti (nV, nZ) [K] for the Arrhenius branch of
+        ice_physical_properties: 253.15 K minus 6 K per km of ice at the surface,
+        warming with zeta^2 towards a base 25 K per km warmer, capped 0.5 K below
+        the pressure-melting point; it crosses the 263.15 K switch of the law."""
+        z = np.asarray(self.params.zeta, np.float64)
+        H = np.asarray(self.Hi, np.float64)
+        Ts = 253.15 - 6e-3 * H
+        Tb = np.minimum(Ts + 25e-3 * H, 273.16 - 8.7e-4 * H - 0.5)
+        return np.asfortranarray(Ts[:, None] + (Tb - Ts)[:, None] * z[None, :] ** 2)
+
 
 def dome_thickness(x, y, H0=3600.0, R=750e3):
     r = np.hypot(x, y)
