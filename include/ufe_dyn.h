@@ -112,6 +112,41 @@ typedef struct ufe_params {
 #define UFE_PHYS_ARRHENIUS 1
 #define UFE_PHYS_BENCHMARK_FIXED 2
 
+/* Primary-mesh geometry (src/mesh_help_functions_module.f90), the step the
+ * reference takes after creating or reading a mesh before make_Ac_mesh
+ * (src/mesh_creation_module.f90:1725-1731, src/restart_module.f90:89-96).
+ * Host arrays in the reference layout; outputs may be NULL to skip them. */
+typedef struct ufe_mesh_geometry_desc {
+  int32_t nV, nTri, nC_mem;
+  double xmin, xmax, ymin, ymax;
+  double tol_dist;                  /* points closer than this coincide; the reference uses
+                                       ((xmax-xmin)+(ymax-ymin)) * 1e-9 / 2 */
+  const double *V;                  /* (nV,2) */
+  const int32_t *nC, *C;            /* (nV), (nV,nC_mem) */
+  const int32_t *niTri, *iTri;      /* (nV), (nV,nC_mem) */
+  const int32_t *edge_index;        /* (nV) */
+  int32_t *Tri;                     /* (nTri,3); update_triangle_circumcenter rotates the
+                                       vertices of a triangle whose circumcentre comes out invalid */
+  const int32_t *Tri_edge_index;    /* (nTri) */
+  int32_t compute_Tricc;            /* 1: Tricc := update_triangle_circumcenter (:854-897);
+                                       0: Tricc is an input (read from a restart file) */
+  double *Tricc;                    /* (nTri,2) in/out */
+  double *A;                        /* (nV)        find_Voronoi_cell_areas (:15-50, :219-487) */
+  double *Cw;                       /* (nV,nC_mem) find_connection_widths (:101-172) */
+  double *TriA;                     /* (nTri)      find_triangle_areas (:173-192) */
+  double *R;                        /* (nV)        determine_mesh_resolution (:193-216) */
+} ufe_mesh_geometry_desc;
+
+typedef struct ufe_mesh_geometry_stats {
+  double resolution_min, resolution_max;
+  double area_error;                /* |1 - sum(A) / domain area|; the reference warns above 1e-4 (:41-45) */
+  int32_t rotated_triangles;        /* circumcentre retries with rotated vertices */
+  int32_t invalid_circumcentres;    /* still invalid after the retries (the reference prints an error) */
+  int32_t voronoi_outside;          /* Voronoi vertices beyond tol_dist outside the domain (the reference stops) */
+  int32_t connection_failures;      /* connections without a shared triangle, or with a single
+                                       non-boundary one (the reference stops) */
+} ufe_mesh_geometry_stats;
+
 /* Statistics of one solve_SSA call. */
 typedef struct ufe_ssa_stats {
   int32_t picard_iterations;
@@ -195,6 +230,10 @@ int ufe_version(void);
 int ufe_mesh_build(ufe_ctx *ctx, ufe_mesh_desc *m);
 /* Upload a complete mesh (primary + secondary) and derive device layouts. */
 int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m);
+/* Tricc, A, Cw, TriA and R of a primary mesh on the GPU
+ * (src/mesh_help_functions_module.f90:15-216, 219-487, 854-897).  Returns
+ * UFE_BAD_INPUT where the reference stops or reports an invalid mesh. */
+int ufe_mesh_geometry(ufe_ctx *ctx, ufe_mesh_geometry_desc *g, ufe_mesh_geometry_stats *st);
 /* Sizes of the uploaded mesh: nV, nAc, nVAaAc, nC_mem. */
 int ufe_mesh_sizes(ufe_ctx *ctx, int32_t *out4);
 

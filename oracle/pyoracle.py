@@ -109,7 +109,7 @@ def params_struct(p: DynParams) -> _Params:
 class OracleMesh:
     """A primary mesh plus the secondary arrays the oracle builds from it."""
 
-    def __init__(self, mesh, nZ=15):
+    def __init__(self, mesh, nZ=15, geometry=True):
         self.lib = load()
         self.mesh = mesh
         self.nZ = nZ
@@ -121,17 +121,23 @@ class OracleMesh:
             "niTri": mesh.niTri, "iTri": mesh.iTri, "edge_index": mesh.edge_index,
             "Tri": mesh.Tri,
         }
+        if geometry:
+            # Tricc, A and Cw from the primary arrays, as the reference computes
+            # them after creating or reading a mesh
+            self.geometry = mesh_geometry(mesh)
+            for k in ("Tri", "Tricc", "A", "Cw", "TriA", "R"):
+                self.arr[k] = self.geometry[k]
         self.arr.update(F.alloc(F.MESH_SECONDARY, nV, self.nAc, nC_mem, nZ))
         s = _Mesh()
         s.d.nV, s.d.nTri, s.d.nC_mem = nV, mesh.nTri, nC_mem
         s.d.nAc, s.d.nVAaAc = self.nAc, self.nVAaAc
         s.d.xmin, s.d.xmax, s.d.ymin, s.d.ymax = mesh.xmin, mesh.xmax, mesh.ymin, mesh.ymax
         for k, name in enumerate(_names("om_mesh_field_name_d", N_MESH_D)):
-            a = self.arr[name]
+            a = self.arr[name] = np.asfortranarray(self.arr[name], dtype=np.float64)
             assert a.dtype == np.float64 and a.flags.f_contiguous, name
             s.f[k] = a.ctypes.data
         for k, name in enumerate(_names("om_mesh_field_name_i", N_MESH_I)):
-            a = self.arr[name]
+            a = self.arr[name] = np.asfortranarray(self.arr[name], dtype=np.int32)
             assert a.dtype == np.int32 and a.flags.f_contiguous, name
             s.i[k] = a.ctypes.data
         self.s = s
@@ -249,7 +255,7 @@ def run_step(wl, threads=None):
 def mesh_from_arrays(mesh, arrays, colour_nV):
     """OracleMesh filled with secondary arrays built elsewhere (used only to
     time the oracle's sweeps on a mesh too large for its literal colouring)."""
-    om = OracleMesh(mesh)
+    om = OracleMesh(mesh, geometry=False)
     for k, a in om.arr.items():
         if k in arrays and k not in ("V", "A", "Cw", "nC", "C", "niTri", "iTri", "edge_index", "Tri"):
             a[...] = arrays[k]
@@ -299,3 +305,72 @@ def colour_graph(adj, nC_mem=16):
             s.i[k] = keep[name].ctypes.data
     st = load().om_calculate_five_colouring_AaAc(ct.byref(s))
     return st, colour
+
+
+class _Geom(ct.Structure):
+    _fields_ = [("nV", ct.c_int), ("nTri", ct.c_int), ("nC_mem", ct.c_int),
+                ("xmin", ct.c_double), ("xmax", ct.c_double), ("ymin", ct.c_double),
+                ("ymax", ct.c_double), ("tol_dist", ct.c_double),
+                ("V", ct.c_void_p), ("nC", ct.c_void_p), ("C", ct.c_void_p), ("niTri", ct.c_void_p),
+                ("iTri", ct.c_void_p), ("edge_index", ct.c_void_p), ("Tri", ct.c_void_p),
+                ("Tri_edge_index", ct.c_void_p), ("Tricc", ct.c_void_p)]
+
+
+def _geom_struct(mesh, Tri, Tricc):
+    g = _Geom()
+    g.nV, g.nTri, g.nC_mem = mesh.nV, mesh.nTri, mesh.nC_mem
+    g.xmin, g.xmax, g.ymin, g.ymax = mesh.xmin, mesh.xmax, mesh.ymin, mesh.ymax
+    g.tol_dist = mesh.tol_dist
+    keep = []
+    for name, a in (("V", mesh.V), ("nC", mesh.nC), ("C", mesh.C), ("niTri", mesh.niTri),
+                    ("iTri", mesh.iTri), ("edge_index", mesh.edge_index),
+                    ("Tri_edge_index", mesh.Tri_edge_index)):
+        a = np.asfortranarray(a, dtype=np.float64 if name == "V" else np.int32)
+        keep.append(a)
+        setattr(g, name, a.ctypes.data)
+    g.Tri, g.Tricc = Tri.ctypes.data, Tricc.ctypes.data
+    return g, keep
+
+
+def mesh_geometry(mesh, compute_Tricc=True):
+    """The reference's primary-mesh geometry on a mesh (src/mesh_help_functions_module.f90):
+    Tricc (update_triangle_circumcenter, or the mesh's own when compute_Tricc is False),
+    Voronoi areas A, connection widths Cw, triangle areas TriA and resolution R."""
+    lib = load()
+    Tri = np.array(mesh.Tri, dtype=np.int32, order="F")
+    Tricc = (np.zeros((mesh.nTri, 2), order="F") if compute_Tricc
+             else np.array(mesh.Tricc, dtype=np.float64, order="F"))
+    g, keep = _geom_struct(mesh, Tri, Tricc)
+    out = {"Tri": Tri, "Tricc": Tricc}
+    rot = ct.c_int(0)
+    if compute_Tricc:
+        out["invalid_cc"] = lib.om_update_triangle_circumcenters(ct.byref(g), ct.byref(rot))
+    out["rotated"] = rot.value
+    A = np.zeros(mesh.nV)
+    Cw = np.zeros((mesh.nV, mesh.nC_mem), order="F")
+    TriA = np.zeros(mesh.nTri)
+    R = np.zeros(mesh.nV)
+    rmin, rmax = ct.c_double(), ct.c_double()
+    out["outside"] = lib.om_find_Voronoi_cell_areas(ct.byref(g), A.ctypes.data_as(ct.c_void_p))
+    out["cw_fails"] = lib.om_find_connection_widths(ct.byref(g), Cw.ctypes.data_as(ct.c_void_p))
+    lib.om_find_triangle_areas(ct.byref(g), TriA.ctypes.data_as(ct.c_void_p))
+    lib.om_determine_mesh_resolution(ct.byref(g), R.ctypes.data_as(ct.c_void_p), ct.byref(rmin),
+                                     ct.byref(rmax))
+    out.update(A=A, Cw=Cw, TriA=TriA, R=R, resolution_min=rmin.value, resolution_max=rmax.value)
+    del keep
+    return out
+
+
+def voronoi_cell_vertices(mesh, Tricc, vi):
+    """Clamped Voronoi cell vertices of vertex vi (1-based), as the reference lists them."""
+    lib = load()
+    Tri = np.array(mesh.Tri, dtype=np.int32, order="F")
+    Tricc = np.asfortranarray(Tricc, dtype=np.float64)
+    g, keep = _geom_struct(mesh, Tri, Tricc)
+    mx = 2 * mesh.nC_mem + 4
+    Vor = np.zeros(2 * mx)
+    out = ct.c_int(0)
+    n = lib.om_find_Voronoi_cell_vertices(ct.byref(g), int(vi), Vor.ctypes.data_as(ct.c_void_p), mx,
+                                          ct.byref(out))
+    del keep
+    return Vor[: 2 * min(n, mx)].reshape(-1, 2), out.value

@@ -1574,3 +1574,309 @@ void om_calculate_ice_thickness_change(const om_mesh *m, om_ice *ice, const doub
     for (int v = 0; v < nV; v++) if (mask_noice[v] == 1) Hi[v] = 0.0;
   free(Vi_SMB);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Primary-mesh geometry, src/mesh_help_functions_module.f90                 */
+/* ------------------------------------------------------------------------ */
+#define GV(vi, k) E2(g->V, g->nV, vi, k)
+#define GTri(ti, k) E2(g->Tri, g->nTri, ti, k)
+#define GTricc(ti, k) E2(g->Tricc, g->nTri, ti, k)
+#define GiTri(vi, k) E2(g->iTri, g->nV, vi, k)
+
+/* line la*x + lb*y = lc through p and q (:1918-1930) */
+static void g_line_from_points(const double p[2], const double q[2], double *la, double *lb, double *lc) {
+  *la = q[1] - p[1];
+  *lb = p[0] - q[0];
+  *lc = (*la) * p[0] + (*lb) * p[1];
+}
+
+/* (:1931-1948) */
+static void g_perpendicular_bisector(const double p[2], const double q[2], double la1, double lb1,
+                                     double *la2, double *lb2, double *lc2) {
+  const double m0 = (p[0] + q[0]) / 2.0, m1 = (p[1] + q[1]) / 2.0;
+  *lc2 = -lb1 * m0 + la1 * m1;
+  *la2 = -lb1;
+  *lb2 = la1;
+}
+
+/* (:1949-1965); parallel lines give the single-precision literal 1E30 */
+static void g_line_line_intersection(double la1, double lb1, double lc1, double la2, double lb2, double lc2,
+                                     double out[2]) {
+  const double d = la1 * lb2 - la2 * lb1;
+  if (d == 0.0) {
+    out[0] = out[1] = (double)1E30f;
+  } else {
+    out[0] = (lb2 * lc1 - lb1 * lc2) / d;
+    out[1] = (la1 * lc2 - la2 * lc1) / d;
+  }
+}
+
+/* (:1966-1996) */
+static void g_find_circumcenter(const double p[2], const double q[2], const double r[2], double cc[2]) {
+  double la1, lb1, lc1, le1, lf1, lg1, la2, lb2, lc2, le2, lf2, lg2;
+  g_line_from_points(p, q, &la1, &lb1, &lc1);
+  g_line_from_points(q, r, &le1, &lf1, &lg1);
+  g_perpendicular_bisector(p, q, la1, lb1, &la2, &lb2, &lc2);
+  g_perpendicular_bisector(q, r, le1, lf1, &le2, &lf2, &lg2);
+  g_line_line_intersection(la2, lb2, lc2, le2, lf2, lg2, cc);
+}
+
+static void g_tri_cc(const om_geom *g, int ti, double cc[2]) {
+  double v1[2] = {GV(GTri(ti, 1), 1), GV(GTri(ti, 1), 2)};
+  double v2[2] = {GV(GTri(ti, 2), 1), GV(GTri(ti, 2), 2)};
+  double v3[2] = {GV(GTri(ti, 3), 1), GV(GTri(ti, 3), 2)};
+  g_find_circumcenter(v1, v2, v3, cc);
+}
+
+static void g_rotate_tri(om_geom *g, int ti) {
+  const int a = GTri(ti, 1), b = GTri(ti, 2), c = GTri(ti, 3);
+  GTri(ti, 1) = b;
+  GTri(ti, 2) = c;
+  GTri(ti, 3) = a;
+}
+
+int om_update_triangle_circumcenters(om_geom *g, int *rotated) {
+  const double Lx = g->xmax - g->xmin, Ly = g->ymax - g->ymin;
+  int bad = 0, rot = 0;
+  for (int ti = 1; ti <= g->nTri; ti++) {
+    double cc[2];
+    g_tri_cc(g, ti, cc);
+    /* the first retry tests y against 1e4 domain widths, the second against 1e5 (:867-885) */
+    if (cc[0] > Lx * 100000.0 || cc[1] > Ly * 10000.0) {
+      g_rotate_tri(g, ti);
+      g_tri_cc(g, ti, cc);
+      rot++;
+    }
+    if (cc[0] > Lx * 100000.0 || cc[1] > Ly * 100000.0) {
+      g_rotate_tri(g, ti);
+      g_tri_cc(g, ti, cc);
+      rot++;
+    }
+    if (cc[0] > Lx * 100000.0 || cc[1] > Ly * 100000.0) bad++;
+    GTricc(ti, 1) = cc[0];
+    GTricc(ti, 2) = cc[1];
+  }
+  if (rotated) *rotated = rot;
+  return bad;
+}
+
+/* crop the circumcentre of t1 in the direction of t2 (:452-487) */
+static void g_crop_circumcenter(const om_geom *g, int t1, int t2, double ccc[2]) {
+  double la, lb, lc, le, lf, lg;
+  ccc[0] = GTricc(t1, 1);
+  ccc[1] = GTricc(t1, 2);
+  const int te = E1(g->Tri_edge_index, t1);
+  double a[2], b[2];
+  if (te == 1 && GTricc(t1, 2) > g->ymax) {
+    a[0] = g->xmin; a[1] = g->ymax; b[0] = g->xmax; b[1] = g->ymax;
+  } else if (te == 3 && GTricc(t1, 1) > g->xmax) {
+    a[0] = g->xmax; a[1] = g->ymax; b[0] = g->xmax; b[1] = g->ymin;
+  } else if (te == 5 && GTricc(t1, 2) < g->ymin) {
+    a[0] = g->xmin; a[1] = g->ymin; b[0] = g->xmax; b[1] = g->ymin;
+  } else if (te == 7 && GTricc(t1, 1) < g->xmin) {
+    a[0] = g->xmin; a[1] = g->ymax; b[0] = g->xmin; b[1] = g->ymin;
+  } else {
+    return;
+  }
+  g_line_from_points(a, b, &la, &lb, &lc);
+  const double p[2] = {GTricc(t1, 1), GTricc(t1, 2)}, q[2] = {GTricc(t2, 1), GTricc(t2, 2)};
+  g_line_from_points(p, q, &le, &lf, &lg);
+  g_line_line_intersection(la, lb, lc, le, lf, lg, ccc);
+}
+
+static void g_push(double *Vor, int max_vor, int *n, double x, double y) {
+  if (*n < max_vor) {
+    Vor[2 * (*n)] = x;
+    Vor[2 * (*n) + 1] = y;
+  }
+  (*n)++;
+}
+
+/* :253-305 */
+static int g_vor_free(const om_geom *g, int vi, double *Vor, int mx) {
+  int n = 0;
+  const int nt = E1(g->niTri, vi);
+  for (int iti = 1; iti <= nt; iti++) {
+    int ic = iti - 1, ia = iti + 1;
+    if (ic == 0) ic = nt;
+    if (ia > nt) ia = 1;
+    const int ti = GiTri(vi, iti), tc = GiTri(vi, ic), ta = GiTri(vi, ia);
+    double cc_c[2], cc_a[2];
+    g_crop_circumcenter(g, ti, tc, cc_c);
+    g_crop_circumcenter(g, ti, ta, cc_a);
+    if (cc_c[0] != cc_a[0] || cc_c[1] != cc_a[1]) {
+      g_push(Vor, mx, &n, cc_c[0], cc_c[1]);
+      g_push(Vor, mx, &n, cc_a[0], cc_a[1]);
+    } else {
+      g_push(Vor, mx, &n, GTricc(ti, 1), GTricc(ti, 2));
+    }
+  }
+  g_push(Vor, mx, &n, Vor[0], Vor[1]);
+  return n;
+}
+
+/* :306-393 */
+static int g_vor_edge(const om_geom *g, int vi, double *Vor, int mx) {
+  int n = 0;
+  const int nt = E1(g->niTri, vi), ei = E1(g->edge_index, vi);
+  for (int iti = 1; iti <= nt; iti++) {
+    const int ti = GiTri(vi, iti);
+    const double cc0 = GTricc(ti, 1), cc1 = GTricc(ti, 2);
+    double c[2], c2[2];
+    if (iti == 1) {
+      if ((ei == 1 || ei == 2) && cc1 < g->ymax) g_push(Vor, mx, &n, cc0, g->ymax);
+      else if ((ei == 3 || ei == 4) && cc0 < g->xmax) g_push(Vor, mx, &n, g->xmax, cc1);
+      else if ((ei == 5 || ei == 6) && cc1 > g->ymin) g_push(Vor, mx, &n, cc0, g->ymin);
+      else if ((ei == 7 || ei == 8) && cc0 > g->xmin) g_push(Vor, mx, &n, g->xmin, cc1);
+      g_crop_circumcenter(g, GiTri(vi, 1), GiTri(vi, 2), c);
+      g_push(Vor, mx, &n, c[0], c[1]);
+    }
+    if (iti > 1 && iti < nt) {
+      g_crop_circumcenter(g, ti, GiTri(vi, iti + 1), c2);   /* anticlockwise */
+      g_crop_circumcenter(g, ti, GiTri(vi, iti - 1), c);    /* clockwise */
+      if (c2[0] != c[0] || c2[1] != c[1]) {
+        g_push(Vor, mx, &n, c[0], c[1]);
+        g_push(Vor, mx, &n, c2[0], c2[1]);
+      } else {
+        g_push(Vor, mx, &n, cc0, cc1);
+      }
+    }
+    if (iti == nt) {
+      g_crop_circumcenter(g, ti, GiTri(vi, iti - 1), c);
+      g_push(Vor, mx, &n, c[0], c[1]);
+      if ((ei == 1 || ei == 8) && cc1 < g->ymax) g_push(Vor, mx, &n, cc0, g->ymax);
+      else if ((ei == 3 || ei == 2) && cc0 < g->xmax) g_push(Vor, mx, &n, g->xmax, cc1);
+      else if ((ei == 5 || ei == 4) && cc1 > g->ymin) g_push(Vor, mx, &n, cc0, g->ymin);
+      else if ((ei == 7 || ei == 6) && cc0 > g->xmin) g_push(Vor, mx, &n, g->xmin, cc1);
+    }
+  }
+  return n;
+}
+
+/* :394-451 */
+static int g_vor_corner(const om_geom *g, int vi, double *Vor, int mx) {
+  if (E1(g->niTri, vi) > 1) return g_vor_edge(g, vi, Vor, mx);
+  int n = 0;
+  const int ti = GiTri(vi, 1);
+  const double c0 = GTricc(ti, 1), c1 = GTricc(ti, 2);
+  switch (E1(g->edge_index, vi)) {
+    case 2:
+      g_push(Vor, mx, &n, c0, g->ymax); g_push(Vor, mx, &n, c0, c1); g_push(Vor, mx, &n, g->xmax, c1);
+      break;
+    case 4:
+      g_push(Vor, mx, &n, g->xmax, c1); g_push(Vor, mx, &n, c0, c1); g_push(Vor, mx, &n, c0, g->ymin);
+      break;
+    case 6:
+      g_push(Vor, mx, &n, c0, g->ymin); g_push(Vor, mx, &n, c0, c1); g_push(Vor, mx, &n, g->xmin, c1);
+      break;
+    default:
+      g_push(Vor, mx, &n, g->xmin, c1); g_push(Vor, mx, &n, c0, c1); g_push(Vor, mx, &n, c0, g->ymax);
+      break;
+  }
+  return n;
+}
+
+/* :219-252 */
+int om_find_Voronoi_cell_vertices(const om_geom *g, int vi, double *Vor, int max_vor, int *outside) {
+  const int ei = E1(g->edge_index, vi);
+  int n;
+  if (ei == 0) n = g_vor_free(g, vi, Vor, max_vor);
+  else if (ei == 2 || ei == 4 || ei == 6 || ei == 8) n = g_vor_corner(g, vi, Vor, max_vor);
+  else n = g_vor_edge(g, vi, Vor, max_vor);
+  int out = 0;
+  for (int k = 0; k < n && k < max_vor; k++) {
+    double x = Vor[2 * k], y = Vor[2 * k + 1];
+    if (x < g->xmin - g->tol_dist || x > g->xmax + g->tol_dist || y < g->ymin - g->tol_dist ||
+        y > g->ymax + g->tol_dist)
+      out++;
+    Vor[2 * k] = dmax(dmin(x, g->xmax), g->xmin);
+    Vor[2 * k + 1] = dmax(dmin(y, g->ymax), g->ymin);
+  }
+  if (outside) *outside = out;
+  return n;
+}
+
+int om_find_Voronoi_cell_areas(const om_geom *g, double *A) {
+  const int mx = 2 * g->nC_mem + 4;
+  int outside_total = 0;
+  if (g->nC_mem > 64) return -1;
+#pragma omp parallel for schedule(static) reduction(+ : outside_total)
+  for (int vi = 1; vi <= g->nV; vi++) {
+    double Vor[2 * (2 * 64 + 4)];
+    int out = 0;
+    const int n = om_find_Voronoi_cell_vertices(g, vi, Vor, mx, &out);
+    outside_total += out + (n > g->nC_mem + 2);   /* the reference's Vor holds nC_mem+2 points */
+    const double x0 = GV(vi, 1), y0 = GV(vi, 2);
+    double a = 0.0;
+    for (int k = 1; k < n && k < mx; k++) {
+      const double ax = Vor[2 * k] - x0, ay = Vor[2 * k + 1] - y0;
+      const double bx = Vor[2 * k - 2] - x0, by = Vor[2 * k - 1] - y0;
+      a = a + fabs((ax * by) - (ay * bx)) / 2.0;
+    }
+    E1(A, vi) = a;
+  }
+  return outside_total;
+}
+
+int om_find_connection_widths(const om_geom *g, double *Cw) {
+  int fails = 0;
+  for (size_t k = 0; k < (size_t)g->nV * g->nC_mem; k++) Cw[k] = 0.0;
+  for (int v1 = 1; v1 <= g->nV; v1++) {
+    for (int nv2 = 1; nv2 <= E1(g->nC, v1); nv2++) {
+      const int v2 = E2(g->C, g->nV, v1, nv2);
+      int t1 = 0, t2 = 0;
+      for (int iti = 1; iti <= E1(g->niTri, v1); iti++) {
+        const int ti = GiTri(v1, iti);
+        const int has = GTri(ti, 1) == v2 || GTri(ti, 2) == v2 || GTri(ti, 3) == v2;
+        if (has) {
+          if (t1 == 0) t1 = ti;
+          else t2 = ti;
+        }
+      }
+      double w = 0.0;
+      if (t1 == 0) {
+        fails++;
+      } else if (t2 > 0) {
+        const double dx = GTricc(t1, 1) - GTricc(t2, 1), dy = GTricc(t1, 2) - GTricc(t2, 2);
+        w = sqrt(dx * dx + dy * dy);   /* NORM2 */
+      } else {
+        const int te = E1(g->Tri_edge_index, t1);
+        if (te == 1) w = dmax(0.0, g->ymax - GTricc(t1, 2));
+        else if (te == 3) w = dmax(0.0, g->xmax - GTricc(t1, 1));
+        else if (te == 5) w = dmax(0.0, GTricc(t1, 2) - g->ymin);
+        else if (te == 7) w = dmax(0.0, GTricc(t1, 1) - g->xmin);
+        else fails++;
+      }
+      E2(Cw, g->nV, v1, nv2) = w;
+    }
+  }
+  return fails;
+}
+
+void om_find_triangle_areas(const om_geom *g, double *TriA) {
+  for (int ti = 1; ti <= g->nTri; ti++) {
+    const double qx = GV(GTri(ti, 1), 1), qy = GV(GTri(ti, 1), 2);
+    const double ax = GV(GTri(ti, 2), 1) - qx, ay = GV(GTri(ti, 2), 2) - qy;
+    const double bx = GV(GTri(ti, 3), 1) - qx, by = GV(GTri(ti, 3), 2) - qy;
+    E1(TriA, ti) = fabs((ax * by) - (ay * bx)) / 2.0;
+  }
+}
+
+void om_determine_mesh_resolution(const om_geom *g, double *R, double *rmin, double *rmax) {
+  for (int vi = 1; vi <= g->nV; vi++) {
+    double r = g->xmax - g->xmin;
+    for (int ci = 1; ci <= E1(g->nC, vi); ci++) {
+      const int vj = E2(g->C, g->nV, vi, ci);
+      const double dx = GV(vj, 1) - GV(vi, 1), dy = GV(vj, 2) - GV(vi, 2);
+      r = dmin(r, sqrt(dx * dx + dy * dy));
+    }
+    E1(R, vi) = r;
+  }
+  double lo = R[0], hi = R[0];
+  for (int vi = 1; vi < g->nV; vi++) {
+    lo = dmin(lo, R[vi]);
+    hi = dmax(hi, R[vi]);
+  }
+  *rmin = lo;
+  *rmax = hi;
+}

@@ -196,6 +196,34 @@ void om_calculate_ice_thickness_change(const om_mesh *m, om_ice *ice, const doub
                                        const om_params *p);
 int  om_set_threads(int n);
 
+/* Primary-mesh geometry, src/mesh_help_functions_module.f90 (reference layout,
+ * column-major, 1-based index values). */
+typedef struct {
+  int nV, nTri, nC_mem;
+  double xmin, xmax, ymin, ymax, tol_dist;
+  const double *V;                     /* (nV,2) */
+  const int *nC, *C, *niTri, *iTri, *edge_index;
+  int *Tri;                            /* (nTri,3), rotated by the circumcentre retry */
+  const int *Tri_edge_index;           /* (nTri) */
+  double *Tricc;                       /* (nTri,2) */
+} om_geom;
+
+/* update_triangle_circumcenter over every triangle (:854-897); returns the
+ * number of triangles still without a valid circumcentre, *rotated counts the retries */
+int om_update_triangle_circumcenters(om_geom *g, int *rotated);
+/* find_Voronoi_cell_areas (:15-50) with find_Voronoi_cell_vertices(_free,
+ * _edge, _corner) and crop_circumcenter (:219-487); returns the number of
+ * Voronoi vertices outside the domain (the reference stops there) */
+int om_find_Voronoi_cell_areas(const om_geom *g, double *A);
+/* the Voronoi cell vertices of one vertex vi (1-based) after clamping; returns nVor */
+int om_find_Voronoi_cell_vertices(const om_geom *g, int vi, double *Vor, int max_vor, int *outside);
+/* find_connection_widths (:101-172); returns the number of failed connections */
+int om_find_connection_widths(const om_geom *g, double *Cw);
+/* find_triangle_areas (:173-192) */
+void om_find_triangle_areas(const om_geom *g, double *TriA);
+/* determine_mesh_resolution (:193-216) */
+void om_determine_mesh_resolution(const om_geom *g, double *R, double *rmin, double *rmax);
+
 #ifdef __cplusplus
 }
 #endif

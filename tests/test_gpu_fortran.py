@@ -17,7 +17,7 @@ def write_input(path, wl):
     with open(path, "wb") as f:
         np.array([m.nV, m.nTri, m.nC_mem, p.nZ], np.int32).tofile(f)
         np.array([m.xmin, m.xmax, m.ymin, m.ymax], np.float64).tofile(f)
-        for a in (m.V, m.nC, m.C, m.niTri, m.iTri, m.edge_index, m.Tri, m.A, m.Cw):
+        for a in (m.V, m.nC, m.C, m.niTri, m.iTri, m.edge_index, m.Tri, m.A, m.Cw, m.Tri_edge_index):
             np.asfortranarray(a).ravel(order="F").tofile(f)
         np.asarray(p.zeta, np.float64).tofile(f)
         np.array([p.m_enh_sia, p.m_enh_ssa, p.SSA_RN_tol, p.SSA_max_residual_UV, p.SSA_SOR_omega, wl.dt],

@@ -221,8 +221,8 @@ def test_mass_conservation_and_positivity():
     SMB = 0.3 * np.ones(m.nV)
     p = DynParams(zero_Hi_at_edges=False)
     ice.calculate_ice_thickness_change(SMB, np.zeros(m.nV), 2.0, np.zeros(m.nV, np.int32), p)
-    before = np.sum(m.A * H) + np.sum(SMB * m.A * 2.0)
-    assert np.isclose(np.sum(m.A * ice["Hi"]), before, rtol=1e-12)
+    before = np.sum(om["A"] * H) + np.sum(SMB * om["A"] * 2.0)
+    assert np.isclose(np.sum(om["A"] * ice["Hi"]), before, rtol=1e-12)
     # strong outflow and melt: the limiter keeps H >= 0 and volume is still conserved
     m, om, ice = _mc_state(seed=10)
     H = 50 * rng.random(m.nV)
@@ -233,8 +233,8 @@ def test_mass_conservation_and_positivity():
     ice.calculate_ice_thickness_change(melt, np.zeros(m.nV), 1.0, np.zeros(m.nV, np.int32), p)
     Hn = ice["Hi"]
     assert Hn.min() > -1e-9 * H.max()
-    removed = np.maximum(melt * m.A, -m.A * H)       # melt limited by available ice
-    assert np.isclose(np.sum(m.A * Hn), np.sum(m.A * H) + np.sum(removed), rtol=1e-9)
+    removed = np.maximum(melt * om["A"], -om["A"] * H)       # melt limited by available ice
+    assert np.isclose(np.sum(om["A"] * Hn), np.sum(om["A"] * H) + np.sum(removed), rtol=1e-9)
 
 
 def test_uniform_translation_leaves_uniform_ice_unchanged():
@@ -249,8 +249,8 @@ def test_uniform_translation_leaves_uniform_ice_unchanged():
     p = DynParams(zero_Hi_at_edges=False)
     ice.calculate_ice_thickness_change(np.zeros(m.nV), np.zeros(m.nV), 1.0, np.zeros(m.nV, np.int32), p)
     inner = m.edge_index == 0
-    scale = 1000.0 * np.hypot(*U) * np.sqrt(m.A)     # flux magnitude per unit area scale
-    assert np.all(np.abs(ice["dHi_dt"][inner]) * m.A[inner] <= 1e-8 * scale[inner] * np.sqrt(m.A[inner]))
+    scale = 1000.0 * np.hypot(*U) * np.sqrt(om["A"])     # flux magnitude per unit area scale
+    assert np.all(np.abs(ice["dHi_dt"][inner]) * om["A"][inner] <= 1e-8 * scale[inner] * np.sqrt(om["A"][inner]))
 
 
 def test_zero_timestep_and_edges():

@@ -29,7 +29,7 @@ STATUS_NAMES = {0: "OK", 1: "SOR_NOT_CONVERGED", 2: "SSA_RESET", 3: "SSA_UNSTABL
 # every symbol include/ufe_dyn.h declares
 EXPORTS = [
     "ufe_ctx_create", "ufe_ctx_destroy", "ufe_set_params", "ufe_last_error", "ufe_version",
-    "ufe_mesh_build", "ufe_mesh_upload", "ufe_mesh_sizes",
+    "ufe_mesh_build", "ufe_mesh_upload", "ufe_mesh_sizes", "ufe_mesh_geometry",
     "ufe_field_count", "ufe_field_name", "ufe_field_id", "ufe_field_info", "ufe_field_set",
     "ufe_field_get", "ufe_field_set_device", "ufe_field_get_device",
     "ufe_update_general_ice_model_data", "ufe_ice_physical_properties", "ufe_solve_SIA", "ufe_solve_SSA",
@@ -60,6 +60,22 @@ class MeshDesc(ct.Structure):
                 ("Np_Ac", _P), ("edge_index_Ac", _P), ("VAaAc", _P), ("nCAaAc", _P), ("CAaAc", _P),
                 ("Nx_AaAc", _P), ("Ny_AaAc", _P), ("Nxx_AaAc", _P), ("Nxy_AaAc", _P), ("Nyy_AaAc", _P),
                 ("colour", _P), ("colour_vi", _P), ("colour_nV", ct.c_int32 * 5)]
+
+
+class MeshGeometryDesc(ct.Structure):
+    _fields_ = [("nV", ct.c_int32), ("nTri", ct.c_int32), ("nC_mem", ct.c_int32),
+                ("xmin", ct.c_double), ("xmax", ct.c_double), ("ymin", ct.c_double), ("ymax", ct.c_double),
+                ("tol_dist", ct.c_double),
+                ("V", _P), ("nC", _P), ("C", _P), ("niTri", _P), ("iTri", _P), ("edge_index", _P),
+                ("Tri", _P), ("Tri_edge_index", _P), ("compute_Tricc", ct.c_int32),
+                ("Tricc", _P), ("A", _P), ("Cw", _P), ("TriA", _P), ("R", _P)]
+
+
+class MeshGeometryStats(ct.Structure):
+    _fields_ = [("resolution_min", ct.c_double), ("resolution_max", ct.c_double),
+                ("area_error", ct.c_double), ("rotated_triangles", ct.c_int32),
+                ("invalid_circumcentres", ct.c_int32), ("voronoi_outside", ct.c_int32),
+                ("connection_failures", ct.c_int32)]
 
 
 class Params(ct.Structure):
@@ -233,7 +249,7 @@ class MeshArrays:
         d.nV, d.nTri, d.nC_mem, d.nAc = m.nV, m.nTri, m.nC_mem, self.nAc
         d.xmin, d.xmax, d.ymin, d.ymax = m.xmin, m.xmax, m.ymin, m.ymax
         for name in ("V", "nC", "C", "niTri", "iTri", "edge_index", "Tri", "A", "Cw"):
-            a = getattr(m, name)
+            a = self[name]
             assert a.flags.f_contiguous, name
             setattr(d, name, a.ctypes.data)
         for name in F.MESH_SECONDARY:
@@ -278,8 +294,45 @@ class Context:
         self._params = params_struct(p, precision, sweep_batch, **krylov)
         self._chk(self.lib.ufe_set_params(self.h, ct.byref(self._params)), "ufe_set_params")
 
-    def mesh_build(self, mesh) -> MeshArrays:
+    def mesh_geometry(self, mesh, compute_Tricc=True):
+        """Tricc, A, Cw, TriA and R of a primary mesh on the GPU, as the reference
+        computes them after creating or reading a mesh.  Returns a dict with
+        those arrays, Tri (rotated where a circumcentre needed the retry) and the
+        statistics."""
+        nV, nT, nCm = mesh.nV, mesh.nTri, mesh.nC_mem
+        out = {"Tri": np.array(mesh.Tri, dtype=np.int32, order="F"),
+               "Tricc": (np.zeros((nT, 2), order="F") if compute_Tricc
+                         else np.array(mesh.Tricc, dtype=np.float64, order="F")),
+               "A": np.zeros(nV), "Cw": np.zeros((nV, nCm), order="F"), "TriA": np.zeros(nT),
+               "R": np.zeros(nV)}
+        keep = {}
+        g = MeshGeometryDesc()
+        g.nV, g.nTri, g.nC_mem = nV, nT, nCm
+        g.xmin, g.xmax, g.ymin, g.ymax = mesh.xmin, mesh.xmax, mesh.ymin, mesh.ymax
+        g.tol_dist = mesh.tol_dist
+        for name, dt in (("V", np.float64), ("nC", np.int32), ("C", np.int32), ("niTri", np.int32),
+                         ("iTri", np.int32), ("edge_index", np.int32), ("Tri_edge_index", np.int32)):
+            keep[name] = np.asfortranarray(getattr(mesh, name), dtype=dt)
+            setattr(g, name, keep[name].ctypes.data)
+        for name in ("Tri", "Tricc", "A", "Cw", "TriA", "R"):
+            setattr(g, name, out[name].ctypes.data)
+        g.compute_Tricc = int(bool(compute_Tricc))
+        st = MeshGeometryStats()
+        self._chk(self.lib.ufe_mesh_geometry(self.h, ct.byref(g), ct.byref(st)), "ufe_mesh_geometry")
+        out["stats"] = st
+        return out
+
+    def mesh_build(self, mesh, geometry=True) -> MeshArrays:
+        """Secondary mesh data of a primary mesh.  With geometry (the default)
+        the GPU first computes Tricc, A and Cw from the primary arrays, as the
+        reference does after creating or reading a mesh; without it the mesh's
+        own A and Cw are used."""
         ma = MeshArrays(mesh)
+        if geometry:
+            geo = self.mesh_geometry(mesh)
+            for k in ("Tri", "Tricc", "A", "Cw", "TriA", "R"):
+                ma.arr[k] = geo[k]
+            ma.geometry_stats = geo["stats"]
         d = ma.desc()
         r = self.lib.ufe_mesh_build(self.h, ct.byref(d))
         ma.build_status = r

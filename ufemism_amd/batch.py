@@ -105,9 +105,9 @@ def join_meshes(built, members, nZ=15):
     tri_base = {id(m): int(tb[i]) for i, m in enumerate(ms)}
     um.iTri = cat(lambda ma: ma.mesh.iTri, lambda m: tri_base[id(m)])
     um.edge_index = cat(lambda ma: ma.mesh.edge_index)
-    um.Tri = cat(lambda ma: ma.mesh.Tri, lambda m: m.vbase)
-    um.A = cat(lambda ma: ma.mesh.A)
-    um.Cw = cat(lambda ma: ma.mesh.Cw)
+    um.Tri = cat(lambda ma: ma["Tri"], lambda m: m.vbase)
+    um.A = cat(lambda ma: ma["A"])
+    um.Cw = cat(lambda ma: ma["Cw"])
     ua = abi.MeshArrays.__new__(abi.MeshArrays)
     ua.mesh = um
     ua.nV, ua.nC_mem, ua.nAc, ua.nVAaAc = nV, nCm, nAc, nA

@@ -624,6 +624,154 @@ int ufe_mesh_build(ufe_ctx *ctx, ufe_mesh_desc *m) {
   return five_colouring(m, ctx->err);
 }
 
+// ---- primary-mesh geometry ---------------------------------------------------
+namespace {
+// Device scratch of one call, released on every return path.
+struct ScratchDev {
+  std::vector<void *> ptrs;
+  ~ScratchDev() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+  hipError_t alloc(void **p, size_t bytes) {
+    *p = nullptr;
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 8));
+    if (e == hipSuccess) ptrs.push_back(*p);
+    return e;
+  }
+};
+}  // namespace
+
+static int check_geometry(ufe_ctx *ctx, const ufe_mesh_geometry_desc *g) {
+  auto bad = [&](const char *msg) {
+    ctx->err = std::string("ufe_mesh_geometry: ") + msg;
+    return UFE_BAD_INPUT;
+  };
+  if (!g || !g->V || !g->nC || !g->C || !g->niTri || !g->iTri || !g->edge_index || !g->Tri ||
+      !g->Tri_edge_index || !g->Tricc)
+    return bad("descriptor incomplete");
+  if (g->nV < 3 || g->nTri < 1 || g->nC_mem < 3 || g->nC_mem > 64) return bad("sizes out of range");
+  if (!(g->xmax > g->xmin) || !(g->ymax > g->ymin) || !(g->tol_dist >= 0.0)) return bad("domain bounds");
+  const int64_t lv = g->nV, lt = g->nTri;
+  for (int v = 0; v < g->nV; v++) {
+    if (g->nC[v] < 0 || g->nC[v] > g->nC_mem) return bad("nC out of range");
+    if (g->niTri[v] < 0 || g->niTri[v] > g->nC_mem) return bad("niTri out of range");
+    for (int c = 0; c < g->nC[v]; c++) {
+      const int j = g->C[c * lv + v];
+      if (j < 1 || j > g->nV) return bad("C holds an index outside 1..nV");
+    }
+    for (int c = 0; c < g->niTri[v]; c++) {
+      const int t = g->iTri[c * lv + v];
+      if (t < 1 || t > g->nTri) return bad("iTri holds an index outside 1..nTri");
+    }
+  }
+  for (int64_t k = 0; k < 3 * lt; k++)
+    if (g->Tri[k] < 1 || g->Tri[k] > g->nV) return bad("Tri holds an index outside 1..nV");
+  return UFE_OK;
+}
+
+int ufe_mesh_geometry(ufe_ctx *ctx, ufe_mesh_geometry_desc *g, ufe_mesh_geometry_stats *st) {
+  if (!ctx) return UFE_BAD_INPUT;
+  int r = check_geometry(ctx, g);
+  if (r) return r;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t nV = g->nV, nT = g->nTri, nCm = g->nC_mem;
+  ScratchDev s;
+  GeomMesh m;
+  m.nV = g->nV;
+  m.nTri = g->nTri;
+  m.nCm = g->nC_mem;
+  m.xmin = g->xmin;
+  m.xmax = g->xmax;
+  m.ymin = g->ymin;
+  m.ymax = g->ymax;
+  m.tol = g->tol_dist;
+  double *dV, *dcc, *dA = nullptr, *dCw = nullptr, *dTriA = nullptr, *dR = nullptr;
+  int32_t *dnC, *dC, *dniTri, *diTri, *dedge, *dTri, *dtei, *dcount;
+  HIPCHK(ctx, s.alloc((void **)&dV, sizeof(*dV) * (2 * nV)));
+  HIPCHK(ctx, s.alloc((void **)&dnC, sizeof(*dnC) * (nV)));
+  HIPCHK(ctx, s.alloc((void **)&dC, sizeof(*dC) * (nV * nCm)));
+  HIPCHK(ctx, s.alloc((void **)&dniTri, sizeof(*dniTri) * (nV)));
+  HIPCHK(ctx, s.alloc((void **)&diTri, sizeof(*diTri) * (nV * nCm)));
+  HIPCHK(ctx, s.alloc((void **)&dedge, sizeof(*dedge) * (nV)));
+  HIPCHK(ctx, s.alloc((void **)&dTri, sizeof(*dTri) * (3 * nT)));
+  HIPCHK(ctx, s.alloc((void **)&dtei, sizeof(*dtei) * (nT)));
+  HIPCHK(ctx, s.alloc((void **)&dcc, sizeof(*dcc) * (2 * nT)));
+  HIPCHK(ctx, s.alloc((void **)&dcount, sizeof(*dcount) * (GEO_NCOUNT)));
+  if (g->A) HIPCHK(ctx, s.alloc((void **)&dA, sizeof(*dA) * (nV)));
+  if (g->Cw) HIPCHK(ctx, s.alloc((void **)&dCw, sizeof(*dCw) * (nV * nCm)));
+  if (g->TriA) HIPCHK(ctx, s.alloc((void **)&dTriA, sizeof(*dTriA) * (nT)));
+  if (g->R) HIPCHK(ctx, s.alloc((void **)&dR, sizeof(*dR) * (nV)));
+  hipStream_t q = ctx->stream;
+  HIPCHK(ctx, hipMemcpyAsync(dV, g->V, sizeof(double) * 2 * nV, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(dnC, g->nC, sizeof(int32_t) * nV, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(dC, g->C, sizeof(int32_t) * nV * nCm, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(dniTri, g->niTri, sizeof(int32_t) * nV, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(diTri, g->iTri, sizeof(int32_t) * nV * nCm, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(dedge, g->edge_index, sizeof(int32_t) * nV, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(dTri, g->Tri, sizeof(int32_t) * 3 * nT, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(dtei, g->Tri_edge_index, sizeof(int32_t) * nT, hipMemcpyHostToDevice, q));
+  if (!g->compute_Tricc)
+    HIPCHK(ctx, hipMemcpyAsync(dcc, g->Tricc, sizeof(double) * 2 * nT, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemsetAsync(dcount, 0, sizeof(int32_t) * GEO_NCOUNT, q));
+  m.V = dV;
+  m.nC = dnC;
+  m.C = dC;
+  m.niTri = dniTri;
+  m.iTri = diTri;
+  m.edge = dedge;
+  m.Tri = dTri;
+  m.tei = dtei;
+  m.cc = dcc;
+  if (g->compute_Tricc) k_geo_tricc<<<nblk(g->nTri), BLOCK, 0, q>>>(m, dcount);
+  if (dTriA) k_geo_tri_area<<<nblk(g->nTri), BLOCK, 0, q>>>(m, dTriA);
+  if (dA) k_geo_voronoi_area<<<nblk(g->nV), BLOCK, 0, q>>>(m, dA, dcount);
+  if (dCw) k_geo_conn_width<<<nblk(g->nV), BLOCK, 0, q>>>(m, dCw, dcount);
+  if (dR) k_geo_resolution<<<nblk(g->nV), BLOCK, 0, q>>>(m, dR);
+  LAUNCHCHK(ctx);
+  int32_t count[GEO_NCOUNT];
+  HIPCHK(ctx, hipMemcpyAsync(count, dcount, sizeof(count), hipMemcpyDeviceToHost, q));
+  if (g->compute_Tricc) {
+    HIPCHK(ctx, hipMemcpyAsync(g->Tricc, dcc, sizeof(double) * 2 * nT, hipMemcpyDeviceToHost, q));
+    HIPCHK(ctx, hipMemcpyAsync(g->Tri, dTri, sizeof(int32_t) * 3 * nT, hipMemcpyDeviceToHost, q));
+  }
+  if (dA) HIPCHK(ctx, hipMemcpyAsync(g->A, dA, sizeof(double) * nV, hipMemcpyDeviceToHost, q));
+  if (dCw) HIPCHK(ctx, hipMemcpyAsync(g->Cw, dCw, sizeof(double) * nV * nCm, hipMemcpyDeviceToHost, q));
+  if (dTriA) HIPCHK(ctx, hipMemcpyAsync(g->TriA, dTriA, sizeof(double) * nT, hipMemcpyDeviceToHost, q));
+  if (dR) HIPCHK(ctx, hipMemcpyAsync(g->R, dR, sizeof(double) * nV, hipMemcpyDeviceToHost, q));
+  HIPCHK(ctx, hipStreamSynchronize(q));
+  ufe_mesh_geometry_stats t{};
+  t.rotated_triangles = count[GEO_ROTATED];
+  t.invalid_circumcentres = count[GEO_INVALID_CC];
+  t.voronoi_outside = count[GEO_OUTSIDE] + count[GEO_TOPOLOGY];
+  t.connection_failures = count[GEO_CW_FAIL];
+  if (g->A) {
+    double sum = 0.0;   // SUM(mesh%A), sequential
+    for (size_t v = 0; v < nV; v++) sum = sum + g->A[v];
+    t.area_error = std::fabs(1.0 - sum / ((g->xmax - g->xmin) * (g->ymax - g->ymin)));
+  }
+  if (g->R) {
+    t.resolution_min = t.resolution_max = g->R[0];
+    for (size_t v = 1; v < nV; v++) {
+      t.resolution_min = std::min(t.resolution_min, g->R[v]);
+      t.resolution_max = std::max(t.resolution_max, g->R[v]);
+    }
+  }
+  if (st) *st = t;
+  if (t.invalid_circumcentres) {
+    ctx->err = "ufe_mesh_geometry: a triangle does not yield a valid circumcentre";
+    return UFE_BAD_INPUT;
+  }
+  if (t.voronoi_outside) {
+    ctx->err = "ufe_mesh_geometry: found Voronoi cell vertices outside of the mesh domain";
+    return UFE_BAD_INPUT;
+  }
+  if (t.connection_failures) {
+    ctx->err = "ufe_mesh_geometry: a connection has no shared triangle, or one that is not a boundary triangle";
+    return UFE_BAD_INPUT;
+  }
+  return UFE_OK;
+}
+
 // ---- mesh upload -----------------------------------------------------------
 int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
   if (!ctx) return UFE_BAD_INPUT;

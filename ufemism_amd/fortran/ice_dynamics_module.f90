@@ -56,6 +56,11 @@ MODULE ice_dynamics_module
     REAL(dp) :: xmin, xmax, ymin, ymax
     REAL(dp), ALLOCATABLE :: V(:,:), A(:), Cw(:,:)
     INTEGER,  ALLOCATABLE :: nC(:), C(:,:), niTri(:), iTri(:,:), edge_index(:), Tri(:,:)
+    ! triangle data of the primary mesh; with Tri_edge_index present, initialise_mesh_dynamics
+    ! computes A, Cw, TriA and R (and Tricc when absent) as the reference does
+    INTEGER,  ALLOCATABLE :: Tri_edge_index(:)
+    REAL(dp), ALLOCATABLE :: Tricc(:,:), TriA(:), R(:)
+    REAL(dp) :: tol_dist = 0._dp, resolution_min = 0._dp, resolution_max = 0._dp
     ! secondary data (make_Ac_mesh, make_combined_AaAc_mesh, neighbour functions, colouring)
     REAL(dp), ALLOCATABLE :: Nx(:,:), Ny(:,:), Nxx(:,:), Nxy(:,:), Nyy(:,:)
     INTEGER,  ALLOCATABLE :: iAci(:,:), Aci(:,:), edge_index_Ac(:)
@@ -212,6 +217,7 @@ CONTAINS
     END IF
     p = params_of(C)
     CALL check(mesh, ufe_set_params(mesh%ufe, p), 'ufe_set_params')
+    IF (ALLOCATED(mesh%Tri_edge_index)) CALL calculate_mesh_geometry(mesh)
     nV = mesh%nV
     nAc = SUM(mesh%nC(1:nV)) / 2
     nA = nV + nAc
@@ -269,6 +275,58 @@ CONTAINS
     mesh%colour_nV = d%colour_nV
     CALL check(mesh, ufe_mesh_upload(mesh%ufe, d), 'ufe_mesh_upload')
   END SUBROUTINE initialise_mesh_dynamics
+
+  ! Voronoi cell areas, triangle areas, connection widths and resolution, plus
+  ! the circumcentres when the mesh has none: the reference's steps after
+  ! creating or reading a mesh (src/mesh_creation_module.f90:1725-1731,
+  ! src/restart_module.f90:89-96), on the GPU.
+  SUBROUTINE calculate_mesh_geometry(mesh)
+    TYPE(type_mesh), TARGET, INTENT(INOUT) :: mesh
+    TYPE(ufe_mesh_geometry_desc) :: g
+    TYPE(ufe_mesh_geometry_stats) :: st
+    INTEGER :: nV, nT, w
+    nV = mesh%nV
+    nT = mesh%nTri
+    w = mesh%nC_mem
+    ! src/restart_module.f90:85 (tol = 1e-9)
+    IF (mesh%tol_dist <= 0._dp) mesh%tol_dist = ((mesh%xmax - mesh%xmin) + (mesh%ymax - mesh%ymin)) * 1E-9_dp / 2._dp
+    g%compute_Tricc = 0
+    IF (.NOT. ALLOCATED(mesh%Tricc)) THEN
+      ALLOCATE(mesh%Tricc(nT, 2))
+      g%compute_Tricc = 1
+    END IF
+    IF (.NOT. ALLOCATED(mesh%A)) ALLOCATE(mesh%A(nV))
+    IF (.NOT. ALLOCATED(mesh%Cw)) ALLOCATE(mesh%Cw(nV, w))
+    IF (.NOT. ALLOCATED(mesh%TriA)) ALLOCATE(mesh%TriA(nT))
+    IF (.NOT. ALLOCATED(mesh%R)) ALLOCATE(mesh%R(nV))
+    g%nV = nV
+    g%nTri = nT
+    g%nC_mem = w
+    g%xmin = mesh%xmin
+    g%xmax = mesh%xmax
+    g%ymin = mesh%ymin
+    g%ymax = mesh%ymax
+    g%tol_dist = mesh%tol_dist
+    g%V = C_LOC(mesh%V)
+    g%nC = C_LOC(mesh%nC)
+    g%C = C_LOC(mesh%C)
+    g%niTri = C_LOC(mesh%niTri)
+    g%iTri = C_LOC(mesh%iTri)
+    g%edge_index = C_LOC(mesh%edge_index)
+    g%Tri = C_LOC(mesh%Tri)
+    g%Tri_edge_index = C_LOC(mesh%Tri_edge_index)
+    g%Tricc = C_LOC(mesh%Tricc)
+    g%A = C_LOC(mesh%A)
+    g%Cw = C_LOC(mesh%Cw)
+    g%TriA = C_LOC(mesh%TriA)
+    g%R = C_LOC(mesh%R)
+    CALL check(mesh, ufe_mesh_geometry(mesh%ufe, g, st), 'ufe_mesh_geometry')
+    mesh%resolution_min = st%resolution_min
+    mesh%resolution_max = st%resolution_max
+    ! src/mesh_help_functions_module.f90:41-45
+    IF (st%area_error > 0.0001_dp) WRITE(0,*) 'calculate_mesh_geometry - WARNING: sum of Voronoi cell areas ', &
+      'doesnt match square area of mesh! (error of ', st%area_error * 100._dp, ' %)'
+  END SUBROUTINE calculate_mesh_geometry
 
   SUBROUTINE finalise_mesh_dynamics(mesh)
     TYPE(type_mesh), INTENT(INOUT) :: mesh

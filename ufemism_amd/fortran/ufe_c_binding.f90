@@ -25,6 +25,19 @@ MODULE ufe_c_binding
     INTEGER(C_INT32_T) :: colour_nV(5)
   END TYPE ufe_mesh_desc
 
+  TYPE, BIND(C) :: ufe_mesh_geometry_desc
+    INTEGER(C_INT32_T) :: nV, nTri, nC_mem
+    REAL(C_DOUBLE)     :: xmin, xmax, ymin, ymax, tol_dist
+    TYPE(C_PTR) :: V, nC, C, niTri, iTri, edge_index, Tri, Tri_edge_index
+    INTEGER(C_INT32_T) :: compute_Tricc
+    TYPE(C_PTR) :: Tricc, A, Cw, TriA, R
+  END TYPE ufe_mesh_geometry_desc
+
+  TYPE, BIND(C) :: ufe_mesh_geometry_stats
+    REAL(C_DOUBLE)     :: resolution_min, resolution_max, area_error
+    INTEGER(C_INT32_T) :: rotated_triangles, invalid_circumcentres, voronoi_outside, connection_failures
+  END TYPE ufe_mesh_geometry_stats
+
   TYPE, BIND(C) :: ufe_params
     INTEGER(C_INT32_T) :: nZ
     REAL(C_DOUBLE)     :: zeta(32)
@@ -76,6 +89,13 @@ MODULE ufe_c_binding
       IMPORT :: C_INT, C_PTR, ufe_mesh_desc
       TYPE(C_PTR), VALUE :: ctx
       TYPE(ufe_mesh_desc) :: m
+      INTEGER(C_INT) :: r
+    END FUNCTION
+    FUNCTION ufe_mesh_geometry(ctx, g, st) BIND(C, name="ufe_mesh_geometry") RESULT(r)
+      IMPORT :: C_INT, C_PTR, ufe_mesh_geometry_desc, ufe_mesh_geometry_stats
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufe_mesh_geometry_desc) :: g
+      TYPE(ufe_mesh_geometry_stats) :: st
       INTEGER(C_INT) :: r
     END FUNCTION
     FUNCTION ufe_mesh_upload(ctx, m) BIND(C, name="ufe_mesh_upload") RESULT(r)

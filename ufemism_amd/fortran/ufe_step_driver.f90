@@ -30,8 +30,10 @@ PROGRAM ufe_step_driver
   mesh%ymin = hdr(3)
   mesh%ymax = hdr(4)
   ALLOCATE(mesh%V(nV, 2), mesh%nC(nV), mesh%C(nV, nC_mem), mesh%niTri(nV), mesh%iTri(nV, nC_mem), &
-           mesh%edge_index(nV), mesh%Tri(nTri, 3), mesh%A(nV), mesh%Cw(nV, nC_mem))
+           mesh%edge_index(nV), mesh%Tri(nTri, 3), mesh%A(nV), mesh%Cw(nV, nC_mem), mesh%Tri_edge_index(nTri))
   READ(u) mesh%V, mesh%nC, mesh%C, mesh%niTri, mesh%iTri, mesh%edge_index, mesh%Tri, mesh%A, mesh%Cw
+  ! with Tri_edge_index the host computes Tricc, A and Cw itself, as the reference does
+  READ(u) mesh%Tri_edge_index
   C%nZ = nZ
   READ(u) C%zeta(1:nZ)
   READ(u) pr, dt
