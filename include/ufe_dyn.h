@@ -271,6 +271,15 @@ int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st);
 /* calculate_ice_thickness_change, src/ice_dynamics_module.f90:31-237.
  * SMB_year, BMB and mask_noice are the fields of the same names. */
 int ufe_calculate_ice_thickness_change(ufe_ctx *ctx, double dt);
+/* solve_SIA_3D, src/ice_dynamics_module.f90:317-405: fields U_3D, V_3D and
+ * W_3D (Aa x nZ) from U_SSA, V_SSA, A_flow and the geometry of
+ * update_general_ice_model_data (Hi, dHs_dx/dy, dHi_dx/dy, dHb_dt, masks). */
+int ufe_solve_SIA_3D(ufe_ctx *ctx);
+/* The critical time steps of determine_timesteps_and_actions
+ * (src/UFEMISM_main_model.f90:734-780) from D_SIA_Ac, U_SSA, V_SSA, U_3D,
+ * V_3D and the Voronoi areas, after its 0.9 correction factor:
+ * dt_crit[3] = {dt_D_2D_min, dt_V_2D_SSA_min, dt_V_3D_SIA_min}. */
+int ufe_critical_timesteps(ufe_ctx *ctx, double *dt_crit);
 
 /* ---- linearised SSA at fixed viscosity (bench / mixed precision) -------- */
 /* Recompute eta, N, S and the SOR coefficients from the current velocities
@@ -394,6 +403,13 @@ int ufe_op_derivatives_Aa_3D(ufe_ctx *ctx, const double *d, double *ddx, double 
 int ufe_op_neumann_Aa(ufe_ctx *ctx, double *d);
 int ufe_op_neumann_Aa_3D(ufe_ctx *ctx, double *d, int nz);
 /* src/mesh_ArakawaC_module.f90:542-844 */
+/* get_upwind_derivative_vertex_3D, src/mesh_derivatives_module.f90:434-483, on
+ * every vertex and layer of d (nV x nz): the derivative on the triangle
+ * upwind of (U,V), or the vertex derivative where |U|,|V| < 1e-10.  *missing
+ * counts the pairs without an upwind triangle (the reference prints an error
+ * for each); their derivatives are zero. */
+int ufe_op_upwind_derivatives_3D(ufe_ctx *ctx, const double *U, const double *V, const double *d, double *ddx,
+                                 double *ddy, int nz, int *missing);
 int ufe_op_derivatives_Ac(ufe_ctx *ctx, const double *d, double *ddx, double *ddy, double *ddp,
                           double *ddo);
 int ufe_op_derivatives_AaAc(ufe_ctx *ctx, const double *d, double *ddx, double *ddy);

@@ -90,7 +90,7 @@ static const char *ice_d_names[] = {
   "Hi", "Hb", "SL", "Hs", "dHb_dt", "dHi_dt", "dHs_dt", "Hi_prev",
   "A_flow_mean", "dHi_dx", "dHi_dy", "dHs_dx", "dHs_dy",
   "dHs_dx_shelf", "dHs_dy_shelf", "U_SIA", "V_SIA", "D_SIA", "U_SSA", "V_SSA",
-  "A_flow", "Ti", "Ti_pmp", "Cpi", "Ki", "dVi_in",
+  "A_flow", "Ti", "Ti_pmp", "Cpi", "Ki", "U_3D", "V_3D", "W_3D", "dVi_in",
   "Hi_Ac", "Hb_Ac", "SL_Ac", "Hs_Ac",
   "dHi_dx_Ac", "dHi_dy_Ac", "dHi_dp_Ac", "dHi_do_Ac",
   "dHb_dx_Ac", "dHb_dy_Ac", "dHb_dp_Ac", "dHb_do_Ac",
@@ -1573,6 +1573,182 @@ void om_calculate_ice_thickness_change(const om_mesh *m, om_ice *ice, const doub
   if (mask_noice)                                                       /* :232-235 */
     for (int v = 0; v < nV; v++) if (mask_noice[v] == 1) Hi[v] = 0.0;
   free(Vi_SMB);
+}
+
+/* ------------------------------------------------------------------------ */
+/* solve_SIA_3D, src/ice_dynamics_module.f90:317-405                         */
+/* ------------------------------------------------------------------------ */
+static void vertex_deriv_3D(const om_mesh *m, const double *d, int vi, int k, double *ddx, double *ddy) {
+  /* get_mesh_derivatives_vertex_3D, src/mesh_derivatives_module.f90:175-190 (1-based vi, k) */
+  const int n = MnC(vi);
+  double x = E2(m->f[OM_Nx], nV_, vi, n + 1) * E2(d, nV_, vi, k);
+  double y = E2(m->f[OM_Ny], nV_, vi, n + 1) * E2(d, nV_, vi, k);
+  for (int ci = 1; ci <= n; ci++) {
+    const int vc = E2(m->i[OM_C], nV_, vi, ci);
+    x = x + E2(m->f[OM_Nx], nV_, vi, ci) * E2(d, nV_, vc, k);
+    y = y + E2(m->f[OM_Ny], nV_, vi, ci) * E2(d, nV_, vc, k);
+  }
+  *ddx = x;
+  *ddy = y;
+}
+
+void om_solve_SIA_3D(const om_mesh *m, om_ice *ice, const om_params *p) {
+  const int nV = nV_, nZ = p->nZ;
+  double *U3 = IF(OI_U_3D), *V3 = IF(OI_V_3D), *W3 = IF(OI_W_3D);
+  const double *Hi = IF(OI_Hi), *Af = IF(OI_A_flow), *Us = IF(OI_U_SSA), *Vs = IF(OI_V_SSA);
+  const double *sx = IF(OI_dHs_dx), *sy = IF(OI_dHs_dy), *ix = IF(OI_dHi_dx), *iy = IF(OI_dHi_dy);
+  const double D_uv_3D_cutoff = -1E5;
+  for (size_t i = 0; i < (size_t)nV * nZ; i++) U3[i] = V3[i] = W3[i] = 0.0;   /* :339-342 */
+  for (int v = 0; v < nV; v++) {                                                /* :345-364 */
+    for (int k = 0; k < nZ; k++) {
+      U3[(size_t)k * nV + v] = Us[v];
+      V3[(size_t)k * nV + v] = Vs[v];
+    }
+    if (II(OI_mask_shelf)[v] == 1) continue;
+    if (Hi[v] == 0.0) continue;
+    const double D_0 = pow(ice_density * grav * Hi[v], n_flow) *
+                       pow(sx[v] * sx[v] + sy[v] * sy[v], (n_flow - 1.0) / 2.0);
+    double prof[32], Ddef[32];
+    for (int k = 0; k < nZ; k++) prof[k] = p->m_enh_sia * Af[(size_t)k * nV + v] * pow(p->zeta[k], n_flow);
+    om_vertical_integrate(p, prof, Ddef);
+    for (int k = 0; k < nZ; k++) {
+      Ddef[k] = 2.0 * Hi[v] * Ddef[k];
+      const double D3 = dmax(D_0 * Ddef[k], D_uv_3D_cutoff);
+      U3[(size_t)k * nV + v] = D3 * sx[v] + Us[v];
+      V3[(size_t)k * nV + v] = D3 * sy[v] + Vs[v];
+    }
+  }
+  om_apply_Neumann_boundary_3D(m, U3, nZ);                                     /* :366-367 */
+  om_apply_Neumann_boundary_3D(m, V3, nZ);
+  for (int vi = 1; vi <= nV; vi++) {                                            /* :369-399 */
+    const int v = vi - 1;
+    if (Medge(vi) > 0) continue;
+    if (II(OI_mask_sheet)[v] == 0) continue;
+    const double dHb_dx = sx[v] - ix[v], dHb_dy = sy[v] - iy[v];
+    W3[(size_t)(nZ - 1) * nV + v] = IF(OI_dHb_dt)[v] + U3[(size_t)(nZ - 1) * nV + v] * dHb_dx +
+                                    V3[(size_t)(nZ - 1) * nV + v] * dHb_dy;
+    for (int k = nZ - 1; k >= 1; k--) {   /* 1-based layer k, k+1 */
+      double dUdx_k, dUdy_k, dUdx_kp1, dUdy_kp1, dVdx_k, dVdy_k, dVdx_kp1, dVdy_kp1;
+      vertex_deriv_3D(m, U3, vi, k, &dUdx_k, &dUdy_k);
+      vertex_deriv_3D(m, U3, vi, k + 1, &dUdx_kp1, &dUdy_kp1);
+      vertex_deriv_3D(m, V3, vi, k, &dVdx_k, &dVdy_k);
+      vertex_deriv_3D(m, V3, vi, k + 1, &dVdx_kp1, &dVdy_kp1);
+      const double w1 = (dUdx_k + dUdx_kp1) / 2.0;
+      const double w2 = (dVdy_k + dVdy_kp1) / 2.0;
+      const double zk = p->zeta[k - 1], zk1 = p->zeta[k];
+      const double w3 = ((sx[v] - 0.5 * (zk1 + zk) * ix[v]) / dmax(0.1, Hi[v])) *
+                        ((E2(U3, nV, vi, k + 1) - E2(U3, nV, vi, k)) / (zk1 - zk));
+      const double w4 = ((sy[v] - 0.5 * (zk1 + zk) * iy[v]) / dmax(0.1, Hi[v])) *
+                        ((E2(V3, nV, vi, k + 1) - E2(V3, nV, vi, k)) / (zk1 - zk));
+      E2(W3, nV, vi, k) = E2(W3, nV, vi, k + 1) - Hi[v] * (w1 + w2 + w3 + w4) * (zk1 - zk);
+    }
+  }
+  om_apply_Neumann_boundary_3D(m, W3, nZ);                                     /* :403 */
+}
+
+/* ------------------------------------------------------------------------ */
+/* Critical time steps, src/UFEMISM_main_model.f90:734-775                   */
+/* ------------------------------------------------------------------------ */
+void om_critical_timesteps(const om_mesh *m, const om_ice *ice, const om_params *p, double out[3]) {
+  const double pi = 3.141592653589793;                    /* src/parameters_module.f90:9 */
+  const int nV = nV_, nZ = p->nZ;
+  double dD = 1000.0, dS = 1000.0, d3 = 1000.0;
+  const double *Us = IF(OI_U_SSA), *Vs = IF(OI_V_SSA), *D = IF(OI_D_SIA_Ac), *A = m->f[OM_A];
+  for (int aci = 1; aci <= nAc_; aci++) {
+    const int vi = MAci(aci, 1), vj = MAci(aci, 2);
+    const double dx = E2(m->f[OM_V], nV, vj, 1) - E2(m->f[OM_V], nV, vi, 1);
+    const double dy = E2(m->f[OM_V], nV, vj, 2) - E2(m->f[OM_V], nV, vi, 2);
+    const double dist = sqrt(dx * dx + dy * dy);
+    /* 1E-09 is a default-real literal in the reference */
+    dD = dmin(dist * dist / (-6.0 * pi * (D[aci - 1] - (double)1E-09f)), dD);
+    dS = dmin(dist / (fabs(Us[vi - 1]) + fabs(Vs[vi - 1])), dS);
+    dS = dmin(dist / (fabs(Us[vj - 1]) + fabs(Vs[vj - 1])), dS);
+  }
+  for (int v = 0; v < nV; v++) {
+    const double r = sqrt(A[v] / pi);
+    dS = dmin(r / (fabs(Us[v]) + fabs(Vs[v])), dS);
+    for (int k = 0; k < nZ; k++)
+      d3 = dmin(r / (fabs(IF(OI_U_3D)[(size_t)k * nV + v]) + fabs(IF(OI_V_3D)[(size_t)k * nV + v])), d3);
+  }
+  const double dt_correction_factor = 0.9;
+  out[0] = dD * dt_correction_factor;
+  out[1] = dS * dt_correction_factor;
+  out[2] = d3 * dt_correction_factor;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Upwind derivatives, src/mesh_derivatives_module.f90:30-47, 434-483        */
+/* ------------------------------------------------------------------------ */
+#define MTri(ti, k) E2(m->i[OM_Tri], m->d.nTri, ti, k)
+void om_triangle_neighbour_functions(const om_mesh *m, double *NxTri, double *NyTri) {
+  const int nT = m->d.nTri;
+  for (int ti = 1; ti <= nT; ti++) {
+    const double ax = E2(m->f[OM_V], nV_, MTri(ti, 1), 1), ay = E2(m->f[OM_V], nV_, MTri(ti, 1), 2);
+    const double bx = E2(m->f[OM_V], nV_, MTri(ti, 2), 1), by = E2(m->f[OM_V], nV_, MTri(ti, 2), 2);
+    const double cx = E2(m->f[OM_V], nV_, MTri(ti, 3), 1), cy = E2(m->f[OM_V], nV_, MTri(ti, 3), 2);
+    const double D = ax * (by - cy) + bx * (cy - ay) + cx * (ay - by);
+    E2(NxTri, nT, ti, 1) = (by - cy) / D;
+    E2(NxTri, nT, ti, 2) = (cy - ay) / D;
+    E2(NxTri, nT, ti, 3) = (ay - by) / D;
+    E2(NyTri, nT, ti, 1) = (cx - bx) / D;
+    E2(NyTri, nT, ti, 2) = (ax - cx) / D;
+    E2(NyTri, nT, ti, 3) = (bx - ax) / D;
+  }
+}
+
+/* is_in_triangle, src/mesh_help_functions_module.f90:647-672 */
+static int in_triangle(const double pa[2], const double pb[2], const double pc[2], const double p[2]) {
+  const double tol = 1E-8;
+  const double as_x = p[0] - pa[0], as_y = p[1] - pa[1];
+  const double s1 = ((pb[0] - pa[0]) * as_y - (pb[1] - pa[1]) * as_x);
+  const double s2 = ((pc[0] - pa[0]) * as_y - (pc[1] - pa[1]) * as_x);
+  const double s3 = ((pc[0] - pb[0]) * (p[1] - pb[1]) - (pc[1] - pb[1]) * (p[0] - pb[0]));
+  return s1 > -tol && s2 < tol && s3 > -tol;
+}
+
+int om_upwind_derivatives_3D(const om_mesh *m, const double *R, const double *NxTri, const double *NyTri,
+                             const double *U, const double *V, const double *d, int nz, double *ddx,
+                             double *ddy) {
+  const int nV = nV_, nT = m->d.nTri;
+  int missing = 0;
+  for (int vi = 1; vi <= nV; vi++)
+    for (int k = 1; k <= nz; k++) {
+      const double u = E2(U, nV, vi, k), w = E2(V, nV, vi, k);
+      double x = 0.0, y = 0.0;
+      if (fabs(u) < 1E-10 && fabs(w) < 1E-10) {
+        vertex_deriv_3D(m, d, vi, k, &x, &y);
+      } else {
+        /* W = [U,V] * R / (4 |[U,V]|), evaluated left to right */
+        const double den = 4.0 * sqrt(u * u + w * w);
+        const double Wx = u * E1(R, vi) / den, Wy = w * E1(R, vi) / den;
+        const double p[2] = {E2(m->f[OM_V], nV, vi, 1) - Wx, E2(m->f[OM_V], nV, vi, 2) - Wy};
+        int tup = 0;
+        for (int iti = 1; iti <= MniTri(vi); iti++) {
+          const int ti = E2(m->i[OM_iTri], nV, vi, iti);
+          double pa[2], pb[2], pc[2];
+          for (int c = 0; c < 2; c++) {
+            pa[c] = E2(m->f[OM_V], nV, MTri(ti, 1), c + 1);
+            pb[c] = E2(m->f[OM_V], nV, MTri(ti, 2), c + 1);
+            pc[c] = E2(m->f[OM_V], nV, MTri(ti, 3), c + 1);
+          }
+          if (in_triangle(pa, pb, pc, p)) {
+            tup = ti;
+            break;
+          }
+        }
+        if (tup == 0) {
+          missing++;
+        } else {
+          x = E2(NxTri, nT, tup, 1) * E2(d, nV, MTri(tup, 1), k) + E2(NxTri, nT, tup, 2) * E2(d, nV, MTri(tup, 2), k) +
+              E2(NxTri, nT, tup, 3) * E2(d, nV, MTri(tup, 3), k);
+          y = E2(NyTri, nT, tup, 1) * E2(d, nV, MTri(tup, 1), k) + E2(NyTri, nT, tup, 2) * E2(d, nV, MTri(tup, 2), k) +
+              E2(NyTri, nT, tup, 3) * E2(d, nV, MTri(tup, 3), k);
+        }
+      }
+      E2(ddx, nV, vi, k) = x;
+      E2(ddy, nV, vi, k) = y;
+    }
+  return missing;
 }
 
 /* ------------------------------------------------------------------------ */

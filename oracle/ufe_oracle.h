@@ -67,7 +67,7 @@ enum om_ice_d {
   OI_A_flow_mean, OI_dHi_dx, OI_dHi_dy, OI_dHs_dx, OI_dHs_dy,
   OI_dHs_dx_shelf, OI_dHs_dy_shelf, OI_U_SIA, OI_V_SIA, OI_D_SIA, OI_U_SSA, OI_V_SSA,
   /* Aa (nV,nZ) */
-  OI_A_flow, OI_Ti, OI_Ti_pmp, OI_Cpi, OI_Ki,
+  OI_A_flow, OI_Ti, OI_Ti_pmp, OI_Cpi, OI_Ki, OI_U_3D, OI_V_3D, OI_W_3D,
   /* Aa (nV,nC_mem) */
   OI_dVi_in,
   /* Ac (nAc) */
@@ -182,6 +182,20 @@ double om_vertical_average(const om_params *p, const double *f);
 void om_vertical_integrate(const om_params *p, const double *f, double *int_f);
 void om_update_general_ice_model_data(const om_mesh *m, om_ice *ice);
 void om_ice_physical_properties(const om_mesh *m, om_ice *ice, const om_params *p, int mode);
+/* solve_SIA_3D, src/ice_dynamics_module.f90:317-405: U_3D, V_3D, W_3D */
+void om_solve_SIA_3D(const om_mesh *m, om_ice *ice, const om_params *p);
+/* the critical time steps of determine_timesteps_and_actions
+ * (src/UFEMISM_main_model.f90:734-775), after the 0.9 correction:
+ * out = {dt_D_2D_min, dt_V_2D_SSA_min, dt_V_3D_SIA_min} */
+void om_critical_timesteps(const om_mesh *m, const om_ice *ice, const om_params *p, double out[3]);
+/* first-order neighbour functions on the triangles (src/mesh_derivatives_module.f90:30-47) */
+void om_triangle_neighbour_functions(const om_mesh *m, double *NxTri, double *NyTri);
+/* get_upwind_derivative_vertex_3D (src/mesh_derivatives_module.f90:434-483) for
+ * every vertex and layer of d (nV,nz); returns the number of (vertex, layer)
+ * pairs whose upwind triangle was not found (their derivatives are zero) */
+int om_upwind_derivatives_3D(const om_mesh *m, const double *R, const double *NxTri, const double *NyTri,
+                             const double *U, const double *V, const double *d, int nz, double *ddx,
+                             double *ddy);
 void om_determine_masks(const om_mesh *m, om_ice *ice);
 void om_solve_SIA(const om_mesh *m, om_ice *ice, const om_params *p);
 void om_basal_yield_stress(const om_mesh *m, om_ice *ice);

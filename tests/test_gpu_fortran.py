@@ -31,16 +31,19 @@ def write_input(path, wl):
         np.zeros(m.nV, np.int32).tofile(f)
 
 
-def read_output(path, nV):
+def read_output(path, nV, nZ=15):
     with open(path, "rb") as f:
         head = np.fromfile(f, np.int32, 66)
         nAc, picard, sweeps = int(head[0]), int(head[1]), head[2:].tolist()
         vals = np.fromfile(f, np.float64)
     o, out = 0, {}
     for name, n in (("U_SSA", nV), ("V_SSA", nV), ("Up_SSA_Ac", nAc), ("Up_SIA_Ac", nAc),
-                    ("Hi", nV), ("dHi_dt", nV)):
+                    ("Hi", nV), ("dHi_dt", nV), ("U_3D", nV * nZ), ("W_3D", nV * nZ), ("A", nV),
+                    ("dt_crit", 3)):
         out[name] = vals[o:o + n]
         o += n
+    for k in ("U_3D", "W_3D"):
+        out[k] = out[k].reshape((nV, nZ), order="F")
     out["sweeps"] = sweeps[:picard]
     return out
 
@@ -64,3 +67,13 @@ def test_fortran_host_step_matches_oracle(tmp_path):
         e = np.linalg.norm(out[k] - ref[k]) / max(np.linalg.norm(ref[k]), 1e-300)
         print(f"  {k}: rel L2 {e:.2e}", flush=True)
         assert e <= 1e-9, k
+    # the host computed the reference's Voronoi areas from Tri_edge_index
+    assert np.array_equal(out["A"], ref["mesh"]["A"])
+    # solve_SIA_3D and the critical time steps after the thickness change
+    ice = ref["ice"]
+    ice.solve_SIA_3D(wl.params)
+    for k in ("U_3D", "W_3D"):
+        e = np.linalg.norm(out[k] - ice[k]) / max(np.linalg.norm(ice[k]), 1e-300)
+        print(f"  {k}: rel L2 {e:.2e}", flush=True)
+        assert e <= 1e-9, k
+    assert np.allclose(out["dt_crit"], ice.critical_timesteps(wl.params), rtol=1e-9)

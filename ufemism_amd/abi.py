@@ -33,7 +33,8 @@ EXPORTS = [
     "ufe_field_count", "ufe_field_name", "ufe_field_id", "ufe_field_info", "ufe_field_set",
     "ufe_field_get", "ufe_field_set_device", "ufe_field_get_device",
     "ufe_update_general_ice_model_data", "ufe_ice_physical_properties", "ufe_solve_SIA", "ufe_solve_SSA",
-    "ufe_calculate_ice_thickness_change", "ufe_ssa_prepare", "ufe_ssa_linear_solve",
+    "ufe_calculate_ice_thickness_change", "ufe_solve_SIA_3D", "ufe_critical_timesteps",
+    "ufe_op_upwind_derivatives_3D", "ufe_ssa_prepare", "ufe_ssa_linear_solve",
     "ufe_ssa_residual",
     "ufe_op_derivatives_Aa", "ufe_op_curvatures_Aa", "ufe_op_derivatives_Aa_3D",
     "ufe_op_neumann_Aa", "ufe_op_neumann_Aa_3D", "ufe_op_derivatives_Ac",
@@ -386,6 +387,24 @@ class Context:
     def calculate_ice_thickness_change(self, dt):
         self._chk(self.lib.ufe_calculate_ice_thickness_change(self.h, float(dt)),
                   "calculate_ice_thickness_change")
+
+    def solve_SIA_3D(self):
+        self._chk(self.lib.ufe_solve_SIA_3D(self.h), "solve_SIA_3D")
+
+    def critical_timesteps(self):
+        """(dt_D_2D_min, dt_V_2D_SSA_min, dt_V_3D_SIA_min) after the 0.9 correction."""
+        out = (ct.c_double * 3)()
+        self._chk(self.lib.ufe_critical_timesteps(self.h, out), "critical_timesteps")
+        return tuple(out)
+
+    def upwind_derivatives_3D(self, U, V, d, ddx, ddy, nz):
+        """get_upwind_derivative_vertex_3D on device pointers (nV x nz);
+        returns the number of vertex-layer pairs without an upwind triangle."""
+        missing = ct.c_int(0)
+        self._chk(self.lib.ufe_op_upwind_derivatives_3D(
+            self.h, ct.c_void_p(int(U)), ct.c_void_p(int(V)), ct.c_void_p(int(d)), ct.c_void_p(int(ddx)),
+            ct.c_void_p(int(ddy)), ct.c_int(int(nz)), ct.byref(missing)), "upwind_derivatives_3D")
+        return missing.value
 
     def ssa_prepare(self):
         self._chk(self.lib.ufe_ssa_prepare(self.h), "ssa_prepare")

@@ -47,7 +47,8 @@ static const FieldDef FIELDS[] = {
     {"U_SSA", G_AA, D_F64, -1, 0}, {"V_SSA", G_AA, D_F64, -1, 0},
     {"SMB_year", G_AA, D_F64, -1, 0}, {"BMB", G_AA, D_F64, -1, 0},
     {"A_flow", G_AAZ, D_F64, -1, 0}, {"Ti", G_AAZ, D_F64, -1, 0}, {"Ti_pmp", G_AAZ, D_F64, -1, 0},
-    {"Cpi", G_AAZ, D_F64, -1, 0}, {"Ki", G_AAZ, D_F64, -1, 0}, {"dVi_in", G_AAC, D_F64, -1, 0},
+    {"Cpi", G_AAZ, D_F64, -1, 0}, {"Ki", G_AAZ, D_F64, -1, 0}, {"U_3D", G_AAZ, D_F64, -1, 0},
+    {"V_3D", G_AAZ, D_F64, -1, 0}, {"W_3D", G_AAZ, D_F64, -1, 0}, {"dVi_in", G_AAC, D_F64, -1, 0},
     // Ac
     {"Hi_Ac", G_AC, D_F64, -1, 0}, {"Hb_Ac", G_AC, D_F64, -1, 0}, {"SL_Ac", G_AC, D_F64, -1, 0},
     {"Hs_Ac", G_AC, D_F64, -1, 0},
@@ -125,6 +126,11 @@ struct ufe_ctx {
   int32_t *d_nC = nullptr, *d_C = nullptr, *d_edge = nullptr, *d_iAci = nullptr, *d_Crev = nullptr;
   double *d_Nx = nullptr, *d_Ny = nullptr, *d_Nxx = nullptr, *d_Nxy = nullptr, *d_Nyy = nullptr;
   double *d_A = nullptr, *d_Cw = nullptr, *d_V = nullptr;
+  // triangles: upwind derivatives (Tri, iTri, triangle neighbour functions, resolution)
+  int32_t *d_Tri = nullptr, *d_niTri = nullptr, *d_iTri = nullptr;
+  double *d_NxTri = nullptr, *d_NyTri = nullptr, *d_R = nullptr;
+  double *d_crit = nullptr;          // critical time step partial minima
+  int32_t *d_icount = nullptr;       // small device counters (upwind misses)
   int32_t *d_Aci = nullptr, *d_edgeAc = nullptr, *d_slot_i = nullptr, *d_slot_j = nullptr;
   double *d_NxAc = nullptr, *d_NyAc = nullptr, *d_NoAc = nullptr, *d_NpAc = nullptr;
   // AaAc colour order
@@ -282,6 +288,8 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->allocs.clear();
   ctx->field_ptr.clear();
   ctx->have_mesh = false;
+  ctx->d_Tri = ctx->d_niTri = ctx->d_iTri = ctx->d_icount = nullptr;
+  ctx->d_NxTri = ctx->d_NyTri = ctx->d_R = ctx->d_crit = nullptr;
   // work spaces live in allocs too: forget them with the mesh
   ctx->kry_m = 0;
   ctx->kry_V = ctx->kry_Z = ctx->kry_w = ctx->kry_x = nullptr;
@@ -326,6 +334,20 @@ static double *F(ufe_ctx *ctx, const char *name) {
 }
 static int32_t *FI(ufe_ctx *ctx, const char *name) {
   return (int32_t *)ctx->field_ptr[field_id(name)];
+}
+
+static TriMesh tri_mesh(ufe_ctx *c) {
+  TriMesh t;
+  t.nV = c->nV;
+  t.nTri = c->nTri;
+  t.V = c->d_V;
+  t.Tri = c->d_Tri;
+  t.niTri = c->d_niTri;
+  t.iTri = c->d_iTri;
+  t.NxTri = c->d_NxTri;
+  t.NyTri = c->d_NyTri;
+  t.R = c->d_R;
+  return t;
 }
 
 static AaMesh aa_mesh(ufe_ctx *c) {
@@ -559,6 +581,25 @@ static int check_primary(ufe_ctx *ctx, const ufe_mesh_desc *m) {
         ctx->err = "C holds an index outside 1..nV";
         return UFE_BAD_INPUT;
       }
+    }
+  // triangles: the upwind derivatives and triangle neighbour functions index through them
+  for (int v = 0; v < m->nV; v++) {
+    if (m->niTri[v] < 1 || m->niTri[v] > m->nC_mem) {
+      ctx->err = "niTri out of range";
+      return UFE_BAD_INPUT;
+    }
+    for (int c = 0; c < m->niTri[v]; c++) {
+      const int t = m->iTri[(int64_t)c * m->nV + v];
+      if (t < 1 || t > m->nTri) {
+        ctx->err = "iTri holds an index outside 1..nTri";
+        return UFE_BAD_INPUT;
+      }
+    }
+  }
+  for (int64_t k = 0; k < 3 * (int64_t)m->nTri; k++)
+    if (m->Tri[k] < 1 || m->Tri[k] > m->nV) {
+      ctx->err = "Tri holds an index outside 1..nV";
+      return UFE_BAD_INPUT;
     }
   return UFE_OK;
 }
@@ -1188,6 +1229,30 @@ int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
     if ((r = dupload(ctx, &ctx->d_mch_ptr, mptr.data(), mptr.size()))) return r;
     ctx->nb = nbm;
   }
+  // triangles for the upwind derivatives, their neighbour functions
+  // (src/mesh_derivatives_module.f90:30-47) and the resolution R
+  // (src/mesh_help_functions_module.f90:193-216)
+  if ((r = dupload(ctx, &ctx->d_Tri, m->Tri, (size_t)m->nTri * 3))) return r;
+  if ((r = dupload(ctx, &ctx->d_niTri, m->niTri, nV))) return r;
+  if ((r = dupload(ctx, &ctx->d_iTri, m->iTri, (size_t)nV * nCm))) return r;
+  if ((r = dalloc(ctx, &ctx->d_NxTri, (size_t)m->nTri * 3))) return r;
+  if ((r = dalloc(ctx, &ctx->d_NyTri, (size_t)m->nTri * 3))) return r;
+  if ((r = dalloc(ctx, &ctx->d_R, nV))) return r;
+  if ((r = dalloc(ctx, &ctx->d_crit, 3 * (size_t)CRIT_MAX_BLOCKS + 3))) return r;
+  if ((r = dalloc(ctx, &ctx->d_icount, 4))) return r;
+  {
+    TriMesh t = tri_mesh(ctx);
+    k_tri_neighbour_functions<<<nblk(m->nTri), BLOCK, 0, ctx->stream>>>(t);
+    GeomMesh g{};
+    g.nV = nV;
+    g.xmin = m->xmin;
+    g.xmax = m->xmax;
+    g.V = ctx->d_V;
+    g.nC = ctx->d_nC;
+    g.C = ctx->d_C;
+    k_geo_resolution<<<nblk(nV), BLOCK, 0, ctx->stream>>>(g, ctx->d_R);
+    LAUNCHCHK(ctx);
+  }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->have_mesh = true;
   return UFE_OK;
@@ -1336,6 +1401,62 @@ int ufe_solve_SIA(ufe_ctx *ctx) {
   k_map_Ac_to_Aa<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(am, ctx->nAc, 1, F(ctx, "D_SIA_Ac"), F(ctx, "D_SIA"));
   LAUNCHCHK(ctx);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return UFE_OK;
+}
+
+// ---- solve_SIA_3D and the critical time steps ---------------------------------
+int ufe_solve_SIA_3D(ufe_ctx *ctx) {
+  if (!ctx || !ctx->have_mesh) return UFE_BAD_INPUT;
+  SIA3DParams p;
+  p.nZ = ctx->nZ;
+  p.m_enh = ctx->params.m_enh_sia;
+  for (int k = 0; k < ctx->nZ; k++) {
+    p.zeta[k] = ctx->params.zeta[k];
+    p.zeta_n[k] = std::pow(ctx->params.zeta[k], N_FLOW);
+  }
+  SIA3DFields f;
+  f.Hi = F(ctx, "Hi");
+  f.Af = F(ctx, "A_flow");
+  f.Us = F(ctx, "U_SSA");
+  f.Vs = F(ctx, "V_SSA");
+  f.sx = F(ctx, "dHs_dx");
+  f.sy = F(ctx, "dHs_dy");
+  f.ix = F(ctx, "dHi_dx");
+  f.iy = F(ctx, "dHi_dy");
+  f.dHb_dt = F(ctx, "dHb_dt");
+  f.shelf = FI(ctx, "mask_shelf");
+  f.sheet = FI(ctx, "mask_sheet");
+  f.U3 = F(ctx, "U_3D");
+  f.V3 = F(ctx, "V_3D");
+  f.W3 = F(ctx, "W_3D");
+  const AaMesh am = aa_mesh(ctx);
+  const int nz = ctx->nZ;
+  hipStream_t q = ctx->stream;
+  k_sia3d_uv<<<nblk(ctx->nV), BLOCK, 0, q>>>(ctx->nV, p, f);
+  for (double *d : {f.U3, f.V3}) {   // apply_Neumann_boundary_3D (:366-367)
+    k_neumann_Aa_edges<<<nblk((int64_t)ctx->nV * nz), BLOCK, 0, q>>>(am, nz, d);
+    k_neumann_Aa_corners<<<nblk(nz, 64), 64, 0, q>>>(am, nz, d);
+  }
+  k_sia3d_w<<<nblk(ctx->nV), BLOCK, 0, q>>>(am, p, f);
+  k_neumann_Aa_edges<<<nblk((int64_t)ctx->nV * nz), BLOCK, 0, q>>>(am, nz, f.W3);
+  k_neumann_Aa_corners<<<nblk(nz, 64), 64, 0, q>>>(am, nz, f.W3);
+  LAUNCHCHK(ctx);
+  HIPCHK(ctx, hipStreamSynchronize(q));
+  return UFE_OK;
+}
+
+int ufe_critical_timesteps(ufe_ctx *ctx, double *dt_crit) {
+  if (!ctx || !ctx->have_mesh || !dt_crit) return UFE_BAD_INPUT;
+  const int n = std::max(ctx->nAc, ctx->nV);
+  const int nb = std::max(1, std::min(CRIT_MAX_BLOCKS, (n + CRIT_BLOCK - 1) / CRIT_BLOCK));
+  hipStream_t q = ctx->stream;
+  double *out = ctx->d_crit + 3 * (size_t)CRIT_MAX_BLOCKS;
+  k_crit_partial<<<nb, CRIT_BLOCK, 0, q>>>(ac_mesh(ctx), ctx->d_A, ctx->nZ, F(ctx, "D_SIA_Ac"), F(ctx, "U_SSA"),
+                                           F(ctx, "V_SSA"), F(ctx, "U_3D"), F(ctx, "V_3D"), ctx->d_crit);
+  k_crit_finish<<<1, CRIT_BLOCK, 0, q>>>(nb, ctx->d_crit, out);
+  LAUNCHCHK(ctx);
+  HIPCHK(ctx, hipMemcpyAsync(dt_crit, out, 3 * sizeof(double), hipMemcpyDeviceToHost, q));
+  HIPCHK(ctx, hipStreamSynchronize(q));
   return UFE_OK;
 }
 
@@ -2778,6 +2899,21 @@ int ufe_op_neumann_Aa_3D(ufe_ctx *ctx, double *d, int nz) {
   k_neumann_Aa_corners<<<nblk(nz, 64), 64, 0, ctx->stream>>>(aa_mesh(ctx), nz, d);
   LAUNCHCHK(ctx);
   return ufe_synchronize(ctx);
+}
+int ufe_op_upwind_derivatives_3D(ufe_ctx *ctx, const double *U, const double *V, const double *d, double *ddx,
+                                 double *ddy, int nz, int *missing) {
+  NEED_MESH(ctx);
+  if (!U || !V || !d || !ddx || !ddy || nz < 1) return UFE_BAD_INPUT;
+  hipStream_t q = ctx->stream;
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_icount, 0, sizeof(int32_t), q));
+  k_upwind_3D<<<nblk((int64_t)ctx->nV * nz), BLOCK, 0, q>>>(aa_mesh(ctx), tri_mesh(ctx), nz, U, V, d, ddx, ddy,
+                                                            ctx->d_icount);
+  LAUNCHCHK(ctx);
+  int32_t h = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&h, ctx->d_icount, sizeof(int32_t), hipMemcpyDeviceToHost, q));
+  HIPCHK(ctx, hipStreamSynchronize(q));
+  if (missing) *missing = h;
+  return UFE_OK;
 }
 int ufe_op_derivatives_Ac(ufe_ctx *ctx, const double *d, double *ddx, double *ddy, double *ddp, double *ddo) {
   NEED_MESH(ctx);

@@ -8,6 +8,8 @@ state, which is the single-step workload of the benchmark configurations.
 """
 from __future__ import annotations
 
+import dataclasses
+
 import numpy as np
 
 from . import abi
@@ -61,3 +63,103 @@ class Step:
         out["picard"] = st.picard_iterations
         out["stats"] = st
         return out
+
+
+@dataclasses.dataclass
+class TimeConfig:
+    """Time-step configuration, src/configuration_module.f90:34-44 (defaults)."""
+    start_time_of_run: float = 0.0
+    dt_max: float = 10.0
+    dt_thermo: float = 10.0
+    dt_climate: float = 10.0
+    dt_SMB: float = 10.0
+    dt_BMB: float = 10.0
+    dt_output: float = 5000.0
+    dt_bedrock_ELRA: float = 100.0
+
+
+_ACTIONS = ("SIA", "SSA", "thermo", "climate", "SMB", "BMB", "ELRA", "output")
+
+
+class RegionClock:
+    """The time bookkeeping of a model region: the start values of
+    initialise_model_region (src/UFEMISM_main_model.f90:351-396) and
+    determine_timesteps_and_actions (:708-843), given the critical time steps
+    that ufe_critical_timesteps computes on the device (:734-780)."""
+
+    def __init__(self, cfg: TimeConfig = None):
+        self.cfg = cfg = cfg or TimeConfig()
+        t = cfg.start_time_of_run
+        self.time = t
+        self.t0 = {a: t for a in _ACTIONS}
+        self.t1 = {a: t for a in _ACTIONS}
+        self.t1["thermo"] = t + cfg.dt_thermo
+        self.dt = 0.0
+        self.dt_prev = 1000.0
+        self.dt_SIA = self.dt_SSA = 0.0
+        self.do = {a: True for a in _ACTIONS}
+        self.do["thermo"] = False
+        self.critical = None
+
+    def determine_timesteps_and_actions(self, dt_crit, t_end):
+        """dt_crit: (dt_D_2D_min, dt_V_2D_SSA_min, dt_V_3D_SIA_min), already
+        multiplied by the 0.9 correction factor.  Sets dt, the do-flags and
+        advances time, as the reference does (:782-839)."""
+        cfg = self.cfg
+        dt_D, dt_S, dt_3 = (float(x) for x in dt_crit)
+        self.critical = min(min(min(dt_D, dt_S), dt_3), cfg.dt_max)
+        if abs(1.0 - self.critical / self.dt_prev) > 0.1:
+            self.dt_prev = self.critical
+        self.dt_SIA = min(cfg.dt_max, min(dt_D, dt_3))
+        self.dt_SSA = min(cfg.dt_max, dt_S)
+        step = {"SIA": self.dt_SIA, "SSA": self.dt_SSA, "thermo": cfg.dt_thermo, "climate": cfg.dt_climate,
+                "SMB": cfg.dt_SMB, "BMB": cfg.dt_BMB, "ELRA": cfg.dt_bedrock_ELRA, "output": cfg.dt_output}
+        for a in _ACTIONS:
+            self.t1[a] = self.t0[a] + step[a]
+        t_next = min(self.t1[a] for a in _ACTIONS)
+        self.dt = t_next - self.time
+        self.do = {a: t_next == self.t1[a] for a in _ACTIONS}
+        if t_next >= t_end:
+            # crop the step to the coupling interval; everything but output runs
+            self.dt = t_end - self.time
+            for a in ("SIA", "SSA", "thermo", "climate", "SMB", "BMB"):
+                self.do[a] = True
+        self.time = self.time + self.dt
+
+
+def run_dynamics(ctx: abi.Context, t_end, clock: RegionClock, dt_first=0.0, on_step=None):
+    """The ice-dynamics part of run_model's time loop (src/UFEMISM_main_model.f90:74-215)
+    on the device-resident state: calculate_ice_thickness_change with the
+    previous step, update_general_ice_model_data, solve_SIA and solve_SSA when
+    their time has come, solve_SIA_3D with the thermodynamics interval (the
+    velocity part of update_ice_temperature), then the critical time steps and
+    the next actions.  Climate, SMB, BMB, bedrock and output are not modelled
+    here: SMB_year, BMB and the bed stay as set, and their clocks only advance.
+    Returns the list of (time, dt, do-flags, SSA stats or None) per step."""
+    log = []
+    clock.dt = dt_first
+    while clock.time < t_end:
+        ctx.calculate_ice_thickness_change(clock.dt)
+        ctx.update_general_ice_model_data()
+        st = None
+        if clock.do["SIA"]:
+            ctx.solve_SIA()
+            clock.t0["SIA"] = clock.time
+        if clock.do["SSA"]:
+            st = ctx.solve_SSA()
+            clock.t0["SSA"] = clock.time
+        for a in ("climate", "SMB", "BMB"):
+            if clock.do[a]:
+                clock.t0[a] = clock.time
+        if clock.do["thermo"]:
+            ctx.solve_SIA_3D()
+            clock.t0["thermo"] = clock.time
+        if clock.do["output"]:
+            clock.t0["output"] = clock.time
+        if clock.do["ELRA"]:
+            clock.t0["ELRA"] = clock.time
+        clock.determine_timesteps_and_actions(ctx.critical_timesteps(), t_end)
+        log.append((clock.time, clock.dt, dict(clock.do), st))
+        if on_step:
+            on_step(clock)
+    return log

@@ -59,7 +59,7 @@ ICE_FIELDS = {}
 for _n in _AA:
     ICE_FIELDS[_n] = ("Aa", F64)
 ICE_FIELDS["A_flow"] = ("Aaz", F64)
-for _n in ("Ti", "Ti_pmp", "Cpi", "Ki"):
+for _n in ("Ti", "Ti_pmp", "Cpi", "Ki", "U_3D", "V_3D", "W_3D"):
     ICE_FIELDS[_n] = ("Aaz", F64)
 ICE_FIELDS["dVi_in"] = ("Aac", F64)
 for _n in _AC:

@@ -78,6 +78,7 @@ MODULE ice_dynamics_module
     REAL(dp), ALLOCATABLE :: Ti(:,:), Ti_pmp(:,:), Cpi(:,:), Ki(:,:)
     REAL(dp), ALLOCATABLE :: dHi_dx(:), dHi_dy(:), dHs_dx(:), dHs_dy(:), dHs_dx_shelf(:), dHs_dy_shelf(:)
     REAL(dp), ALLOCATABLE :: U_SIA(:), V_SIA(:), D_SIA(:), U_SSA(:), V_SSA(:), dVi_in(:,:)
+    REAL(dp), ALLOCATABLE :: U_3D(:,:), V_3D(:,:), W_3D(:,:)
     INTEGER,  ALLOCATABLE :: mask_land(:), mask_ocean(:), mask_lake(:), mask_ice(:), mask_sheet(:), &
                              mask_shelf(:), mask_coast(:), mask_margin(:), mask_gl(:), mask_cf(:), mask(:)
     ! Ac
@@ -349,7 +350,7 @@ CONTAINS
              ice%dHs_dx(nV), ice%dHs_dy(nV), ice%dHs_dx_shelf(nV), ice%dHs_dy_shelf(nV), ice%U_SIA(nV), &
              ice%V_SIA(nV), ice%D_SIA(nV), ice%U_SSA(nV), ice%V_SSA(nV), ice%dVi_in(nV, mesh%nC_mem), source=0._dp)
     ALLOCATE(ice%Ti(nV, nZ), ice%Ti_pmp(nV, nZ), ice%Cpi(nV, nZ), ice%Ki(nV, nZ), ice%Ti_Ac(nAc, nZ), &
-             source=0._dp)
+             ice%U_3D(nV, nZ), ice%V_3D(nV, nZ), ice%W_3D(nV, nZ), source=0._dp)
     ALLOCATE(ice%mask_land(nV), ice%mask_ocean(nV), ice%mask_lake(nV), ice%mask_ice(nV), ice%mask_sheet(nV), &
              ice%mask_shelf(nV), ice%mask_coast(nV), ice%mask_margin(nV), ice%mask_gl(nV), ice%mask_cf(nV), &
              ice%mask(nV), source=0)
@@ -473,6 +474,45 @@ CONTAINS
     CALL get(mesh, 'V_SIA', ice%V_SIA)
     CALL get(mesh, 'D_SIA', ice%D_SIA)
   END SUBROUTINE solve_SIA
+
+  ! solve_SIA_3D (src/ice_dynamics_module.f90:317-405): U_3D, V_3D, W_3D from the
+  ! SSA velocities, A_flow and the geometry of update_general_ice_model_data
+  SUBROUTINE solve_SIA_3D(mesh, ice)
+    TYPE(type_mesh), INTENT(IN) :: mesh
+    TYPE(type_ice_model), INTENT(INOUT) :: ice
+    IF (ufe_host_sync) THEN
+      CALL put(mesh, 'U_SSA', ice%U_SSA)
+      CALL put(mesh, 'V_SSA', ice%V_SSA)
+      CALL put(mesh, 'A_flow', ice%A_flow)
+      CALL put(mesh, 'dHb_dt', ice%dHb_dt)
+    END IF
+    CALL check(mesh, ufe_solve_SIA_3D(mesh%ufe), 'solve_SIA_3D')
+    IF (.NOT. ufe_host_sync) RETURN
+    CALL get(mesh, 'U_3D', ice%U_3D)
+    CALL get(mesh, 'V_3D', ice%V_3D)
+    CALL get(mesh, 'W_3D', ice%W_3D)
+  END SUBROUTINE solve_SIA_3D
+
+  ! The critical time steps of determine_timesteps_and_actions
+  ! (src/UFEMISM_main_model.f90:734-780, after the 0.9 correction): the
+  ! loops and MPI_ALLREDUCEs there become this one call
+  SUBROUTINE calculate_critical_timesteps(mesh, ice, dt_D_2D_min, dt_V_2D_SSA_min, dt_V_3D_SIA_min)
+    TYPE(type_mesh), INTENT(IN) :: mesh
+    TYPE(type_ice_model), INTENT(IN) :: ice
+    REAL(dp), INTENT(OUT) :: dt_D_2D_min, dt_V_2D_SSA_min, dt_V_3D_SIA_min
+    REAL(C_DOUBLE) :: dt(3)
+    IF (ufe_host_sync) THEN
+      CALL put(mesh, 'D_SIA_Ac', ice%D_SIA_Ac)
+      CALL put(mesh, 'U_SSA', ice%U_SSA)
+      CALL put(mesh, 'V_SSA', ice%V_SSA)
+      CALL put(mesh, 'U_3D', ice%U_3D)
+      CALL put(mesh, 'V_3D', ice%V_3D)
+    END IF
+    CALL check(mesh, ufe_critical_timesteps(mesh%ufe, dt), 'critical_timesteps')
+    dt_D_2D_min = dt(1)
+    dt_V_2D_SSA_min = dt(2)
+    dt_V_3D_SIA_min = dt(3)
+  END SUBROUTINE calculate_critical_timesteps
 
   SUBROUTINE solve_SSA(mesh, ice)
     TYPE(type_mesh), INTENT(IN) :: mesh

@@ -137,6 +137,17 @@ MODULE ufe_c_binding
       TYPE(C_PTR), VALUE :: ctx
       INTEGER(C_INT) :: r
     END FUNCTION
+    FUNCTION ufe_solve_SIA_3D(ctx) BIND(C, name="ufe_solve_SIA_3D") RESULT(r)
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT) :: r
+    END FUNCTION
+    FUNCTION ufe_critical_timesteps(ctx, dt_crit) BIND(C, name="ufe_critical_timesteps") RESULT(r)
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE) :: dt_crit(3)
+      INTEGER(C_INT) :: r
+    END FUNCTION
     FUNCTION ufe_solve_SSA(ctx, st) BIND(C, name="ufe_solve_SSA") RESULT(r)
       IMPORT :: C_INT, C_PTR, ufe_ssa_stats
       TYPE(C_PTR), VALUE :: ctx

@@ -15,7 +15,7 @@ PROGRAM ufe_step_driver
   CHARACTER(LEN=1024) :: fin, fout
   INTEGER :: u, nV, nTri, nC_mem, nZ, k, nmax_out, nmax_in
   INTEGER, ALLOCATABLE :: mask_noice(:)
-  REAL(dp) :: A_flow, tau_c, dt, hdr(4), pr(5)
+  REAL(dp) :: A_flow, tau_c, dt, hdr(4), pr(5), dt_crit(3)
 
   CALL GET_COMMAND_ARGUMENT(1, fin)
   CALL GET_COMMAND_ARGUMENT(2, fout)
@@ -62,10 +62,14 @@ PROGRAM ufe_step_driver
   CALL solve_SIA(mesh, ice)
   CALL solve_SSA(mesh, ice)
   CALL calculate_ice_thickness_change(mesh, ice, SMB, BMB, dt, mask_noice)
+  ! the velocity part of the thermodynamics and the next time steps
+  CALL solve_SIA_3D(mesh, ice)
+  CALL calculate_critical_timesteps(mesh, ice, dt_crit(1), dt_crit(2), dt_crit(3))
 
   OPEN(NEWUNIT=u, FILE=TRIM(fout), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
   WRITE(u) mesh%nAc, ice%SSA_picard_iterations, (ice%SSA_sweeps(k), k = 1, 64)
   WRITE(u) ice%U_SSA, ice%V_SSA, ice%Up_SSA_Ac, ice%Up_SIA_Ac, ice%Hi, ice%dHi_dt
+  WRITE(u) ice%U_3D, ice%W_3D, mesh%A, dt_crit
   CLOSE(u)
   CALL finalise_mesh_dynamics(mesh)
 END PROGRAM ufe_step_driver
