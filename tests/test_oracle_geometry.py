@@ -163,3 +163,21 @@ def test_degenerate_triangle_is_reported():
     m2 = meshgen.dummy_mesh()
     g2 = po.mesh_geometry(m2, compute_Tricc=False)
     assert np.array_equal(g2["Tricc"], np.asarray(m2.Tricc))
+
+
+def test_generator_splits_encroached_boundary_segments():
+    """A jittered marine mesh whose Delaunay triangulation has a triangle with
+    its circumcentre outside the domain beyond a side it does not border: the
+    reference cannot crop that Voronoi vertex and stops.  The generator splits
+    the encroached boundary segment (one extra boundary vertex) and the mesh
+    passes; meshes without such triangles are not touched."""
+    res = geometry.marine_resolution()
+    raw = meshgen.generate_mesh(res, 20213, seed=11062, split_encroached=False)
+    assert po.mesh_geometry(raw)["outside"] > 0
+    fixed = meshgen.generate_mesh(res, 20213, seed=11062)
+    g = po.mesh_geometry(fixed)
+    assert g["outside"] == 0 and g["cw_fails"] == 0
+    assert fixed.nV == raw.nV + 1 and fixed.edge_index[-1] in (1, 3, 5, 7)
+    m = geometry.dome(2000, seed=3).mesh
+    same = meshgen.generate_mesh(geometry.dome_resolution(), 2000, seed=3, split_encroached=False)
+    assert np.array_equal(m.V, same.V)

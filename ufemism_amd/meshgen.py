@@ -393,8 +393,68 @@ def _finish_geometry(mesh: Mesh):
     mesh.A = A
 
 
+# boundary sides: (edge_index of the side, the corners on it, axis along it, fixed coordinate)
+_SIDES = {1: (2, 8), 3: (2, 4), 5: (4, 6), 7: (6, 8)}
+
+
+def _uncroppable_triangles(P, edge_index, simplices, xmin, xmax, ymin, ymax, tol):
+    """Triangles whose circumcentre lies outside the domain beyond a side on
+    which the triangle has no edge.  The reference crops a circumcentre only
+    back across the side its triangle borders (crop_circumcenter,
+    src/mesh_help_functions_module.f90:452-487) and stops on any other Voronoi
+    vertex outside the domain (:240-247); its refinement never leaves such a
+    triangle."""
+    a, b, c = (P[simplices[:, k]] for k in range(3))
+    d = 2.0 * (a[:, 0] * (b[:, 1] - c[:, 1]) + b[:, 0] * (c[:, 1] - a[:, 1]) + c[:, 0] * (a[:, 1] - b[:, 1]))
+    sa, sb, sc = (np.einsum("ij,ij->i", q, q) for q in (a, b, c))
+    ux = (sa * (b[:, 1] - c[:, 1]) + sb * (c[:, 1] - a[:, 1]) + sc * (a[:, 1] - b[:, 1])) / d
+    uy = (sa * (c[:, 0] - b[:, 0]) + sb * (a[:, 0] - c[:, 0]) + sc * (b[:, 0] - a[:, 0])) / d
+    ev = edge_index[simplices]
+    bad = np.zeros(len(simplices), bool)
+    for side, outside in ((1, uy > ymax + tol), (3, ux > xmax + tol), (5, uy < ymin - tol), (7, ux < xmin - tol)):
+        on = np.isin(ev, (side,) + _SIDES[side]).sum(1) >= 2
+        bad |= outside & ~on
+    return np.nonzero(bad)[0]
+
+
+def _split_encroached_boundary(P, edge_index, tri, xmin, xmax, ymin, ymax, max_rounds=20):
+    """Ruppert's boundary-segment splitting, applied where the mesh needs it:
+    a boundary segment whose diametral circle holds a vertex of an
+    uncroppable triangle is split at its midpoint, and the points are
+    triangulated again.  Once no boundary segment is encroached every
+    circumcentre lies inside the domain, so the loop ends; meshes without such
+    triangles are returned unchanged."""
+    from scipy.spatial import Delaunay, cKDTree
+    tol = ((xmax - xmin) + (ymax - ymin)) * 1e-9 / 2.0
+    for _ in range(max_rounds):
+        bad = _uncroppable_triangles(P, edge_index, tri.simplices, xmin, xmax, ymin, ymax, tol)
+        if bad.size == 0:
+            return P, edge_index, tri
+        suspects = set(np.unique(tri.simplices[bad]).tolist())
+        tree = cKDTree(P)
+        new_p, new_e = [], []
+        for side, corners in _SIDES.items():
+            idx = np.nonzero(np.isin(edge_index, (side,) + corners))[0]
+            axis = 0 if side in (1, 5) else 1
+            idx = idx[np.argsort(P[idx, axis], kind="stable")]
+            for i, j in zip(idx[:-1], idx[1:]):
+                m = 0.5 * (P[i] + P[j])
+                r = 0.5 * np.hypot(*(P[j] - P[i]))
+                inside = [k for k in tree.query_ball_point(m, r * (1.0 - 1e-12)) if k not in (i, j)]
+                if any(k in suspects for k in inside):
+                    new_p.append(m)
+                    new_e.append(side)
+        if not new_p:
+            raise ValueError("mesh has a circumcentre outside the domain that no boundary split removes")
+        P = np.concatenate([P, np.array(new_p)])
+        edge_index = np.concatenate([edge_index, np.array(new_e, edge_index.dtype)])
+        tri = Delaunay(P)
+    raise ValueError("boundary splitting did not converge")
+
+
 def generate_mesh(res, n_target, seed=0, xmin=-800e3, xmax=800e3, ymin=-800e3,
-                  ymax=800e3, jitter=0.15, nC_mem=NC_MEM, max_iter=3, smooth=2) -> Mesh:
+                  ymax=800e3, jitter=0.15, nC_mem=NC_MEM, max_iter=3, smooth=2,
+                  split_encroached=True) -> Mesh:
     """Seeded mesh with about ``n_target`` vertices following ``res(x, y)``.
 
     ``res`` gives the relative target edge length [m]; it is rescaled
@@ -436,6 +496,8 @@ def generate_mesh(res, n_target, seed=0, xmin=-800e3, xmax=800e3, ymin=-800e3,
         mean /= cnt[:, None]
         P = np.where(interior[:, None], P + 0.6 * (mean - P), P)
         tri = Delaunay(P)
+    if split_encroached:
+        P, edge_index, tri = _split_encroached_boundary(P, edge_index, tri, xmin, xmax, ymin, ymax)
     if len(tri.coplanar):
         raise ValueError("triangulation dropped points")
     tris, nC, C, niTri, iTri, TriC = _build_topology(
