@@ -147,6 +147,19 @@ typedef struct ufe_mesh_geometry_stats {
                                        non-boundary one (the reference stops) */
 } ufe_mesh_geometry_stats;
 
+/* Remapping between two meshes, src/mesh_mapping_module.f90. */
+#define UFE_REMAP_TRILIN 0              /* remap_trilin_* (:3845-3918) */
+#define UFE_REMAP_NEAREST_NEIGHBOUR 1   /* remap_nearest_neighbour_* (:3919-3963) */
+#define UFE_REMAP_CONS_1ST_ORDER 2      /* remap_cons_1st_order_* (:3964-4012) */
+#define UFE_REMAP_CONS_2ND_ORDER 3      /* remap_cons_2nd_order_* (:4013-4124) */
+
+typedef struct ufe_remap_stats {
+  int32_t n_entries;                /* conservative entries (source cells per destination cell, summed) */
+  int32_t n_fallback;               /* destination vertices given triangle weights by the conservativity check */
+  int32_t trilin_missing;           /* destination vertices without a containing source triangle */
+  double seconds;
+} ufe_remap_stats;
+
 /* Statistics of one solve_SSA call. */
 typedef struct ufe_ssa_stats {
   int32_t picard_iterations;
@@ -230,6 +243,31 @@ int ufe_version(void);
 int ufe_mesh_build(ufe_ctx *ctx, ufe_mesh_desc *m);
 /* Upload a complete mesh (primary + secondary) and derive device layouts. */
 int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m);
+/* create_remapping_arrays (src/mesh_mapping_module.f90:673-697): trilinear,
+ * nearest-neighbour and conservative (1st and 2nd order) remapping arrays from
+ * the mesh of src to the mesh of dst, both uploaded on the same device.
+ * gs and gd describe the two meshes with their geometry (Tricc and A as
+ * ufe_mesh_geometry computes them).  The conservative weights are the
+ * reference's line integrals over the overlap of each destination cell with
+ * each source cell, without the domain boundary (:2348-2415); destination
+ * edge vertices get zero weights, and destination vertices that fail the
+ * conservativity check take triangle weights (:2416-2566).  The arrays live
+ * in dst until its next mesh upload or remapping build.  UFE_BAD_INPUT where
+ * the reference stops (more than 1 % of the vertices fail the check). */
+int ufe_remap_build(ufe_ctx *dst, ufe_ctx *src, const ufe_mesh_geometry_desc *gs,
+                    const ufe_mesh_geometry_desc *gd, ufe_remap_stats *st);
+/* remap_field_dp / remap_field_dp_3D (:4125-4228) on device buffers:
+ * d_src (nV_src x nz) -> d_dst (nV_dst x nz), method UFE_REMAP_*. */
+int ufe_remap_field(ufe_ctx *dst, ufe_ctx *src, int method, const double *d_src, double *d_dst, int nz);
+/* remap_ice_model (src/ice_dynamics_module.f90:1198-1394) for the fields of
+ * this library: Hi, Hi_prev and Ti conservatively (1st order) from the fields
+ * of src into those of dst; the SSA velocities of dst start from zero. */
+int ufe_remap_ice_model(ufe_ctx *dst, ufe_ctx *src);
+/* Copies the remapping arrays of dst to host memory (NULL skips an array):
+ * nn (nV_dst), tvi/tw (nV_dst x 3), start/end (nV_dst, 0-based, end
+ * exclusive) and vi (1-based)/w0/w1x/w1y (*n_entries). */
+int ufe_remap_export(ufe_ctx *dst, int *n_entries, int32_t *nn, int32_t *tvi, double *tw, int32_t *start,
+                     int32_t *end, int32_t *vi, double *w0, double *w1x, double *w1y);
 /* Tricc, A, Cw, TriA and R of a primary mesh on the GPU
  * (src/mesh_help_functions_module.f90:15-216, 219-487, 854-897).  Returns
  * UFE_BAD_INPUT where the reference stops or reports an invalid mesh. */

@@ -444,3 +444,151 @@ def run_dynamics(wl, t_end, clock, dt_first=0.0):
         clock.determine_timesteps_and_actions(ice.critical_timesteps(p), t_end)
         log.append((clock.time, clock.dt, dict(clock.do), st))
     return log, ice, om
+
+
+# ---- remapping (ufe_oracle_remap.c), src/mesh_mapping_module.f90 ----------
+
+def _geom_of(mesh, geo):
+    """om_geom of a mesh with its reference geometry (Tricc, Tri from mesh_geometry)."""
+    Tri = np.array(geo["Tri"], dtype=np.int32, order="F")
+    Tricc = np.array(geo["Tricc"], dtype=np.float64, order="F")
+    g, keep = _geom_struct(mesh, Tri, Tricc)
+    keep += [g, Tri, Tricc]
+    return g, keep
+
+
+def geometric_centres(mesh, geo):
+    """find_Voronoi_cell_geometric_centres: VorGC (nV, 2)."""
+    lib = load()
+    g, keep = _geom_of(mesh, geo)
+    A = np.ascontiguousarray(geo["A"])
+    GC = np.zeros((mesh.nV, 2), order="F")
+    lib.om_voronoi_geometric_centres(ct.byref(g), A.ctypes.data_as(ct.c_void_p), GC.ctypes.data_as(ct.c_void_p))
+    del keep
+    return GC
+
+
+def cell_polygon(mesh, geo, vi):
+    lib = load()
+    g, keep = _geom_of(mesh, geo)
+    P = np.zeros(2 * 140)
+    n = lib.om_cell_polygon(ct.byref(g), int(vi), P.ctypes.data_as(ct.c_void_p), 136)
+    del keep
+    return P[: 2 * n].reshape(-1, 2)
+
+
+class ConservativeMap:
+    """Entries start[vd]:end[vd] (0-based, end exclusive) of each destination
+    vertex: source vertex vi (1-based) and weights w0, w1x, w1y."""
+
+    def __init__(self, ptr, vi, w0, w1x, w1y):
+        self.start = ptr[:-1].copy()
+        self.end = ptr[1:].copy()
+        self.vi, self.w0, self.w1x, self.w1y = vi, w0, w1x, w1y
+
+
+def remap_conservative_weights(src, src_geo, dst, dst_geo):
+    lib = load()
+    gs, ks = _geom_of(src, src_geo)
+    gd, kd = _geom_of(dst, dst_geo)
+    As = np.ascontiguousarray(src_geo["A"])
+    Ad = np.ascontiguousarray(dst_geo["A"])
+    GC = np.asfortranarray(geometric_centres(src, src_geo))
+    ptr = np.zeros(dst.nV + 1, np.int32)
+    P = lambda a: a.ctypes.data_as(ct.c_void_p)
+    n = lib.om_remap_conservative(ct.byref(gs), P(As), P(GC), ct.byref(gd), P(Ad), P(ptr), None, None, None, None, 0)
+    if n < 0:
+        raise RuntimeError("om_remap_conservative: polygon overflow")
+    vi = np.zeros(n, np.int32)
+    w0, w1x, w1y = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib.om_remap_conservative(ct.byref(gs), P(As), P(GC), ct.byref(gd), P(Ad), P(ptr), P(vi), P(w0), P(w1x),
+                              P(w1y), ct.c_int(n))
+    del ks, kd
+    return ConservativeMap(ptr, vi, w0, w1x, w1y)
+
+
+def remap_trilin_nearest(src, src_geo, dst):
+    """create_remapping_arrays_trilin / _nearest_neighbour: (tri_vi (nV,3) 1-based,
+    tri_w (nV,3), nn_vi (nV) 1-based, missing)."""
+    lib = load()
+    gs, ks = _geom_of(src, src_geo)
+    gd, kd = _geom_of(dst, {"Tri": dst.Tri, "Tricc": np.zeros((dst.nTri, 2))})
+    TriC = np.asfortranarray(src.TriC, dtype=np.int32)
+    tvi = np.zeros((dst.nV, 3), np.int32, order="F")
+    tw = np.zeros((dst.nV, 3), order="F")
+    nn = np.zeros(dst.nV, np.int32)
+    P = lambda a: a.ctypes.data_as(ct.c_void_p)
+    miss = lib.om_remap_trilin_nearest(ct.byref(gs), P(TriC), ct.byref(gd), P(tvi), P(tw), P(nn))
+    del ks, kd
+    return tvi, tw, nn, miss
+
+
+def apply_conservative(cmap, d_src, ddx=None, ddy=None):
+    """remap_cons_1st_order_2D (src/mesh_mapping_module.f90:3964-3986) or, with
+    the source derivatives, remap_cons_2nd_order_2D (:4013-4047), summed in
+    entry order."""
+    nd = len(cmap.start)
+    out = np.zeros(nd)
+    for vd in range(nd):
+        s = 0.0
+        for e in range(cmap.start[vd], cmap.end[vd]):
+            v = cmap.vi[e] - 1
+            if ddx is None:
+                s = s + (d_src[v] * cmap.w0[e])
+            else:
+                s = s + (d_src[v] * cmap.w0[e]) + (ddx[v] * cmap.w1x[e]) + (ddy[v] * cmap.w1y[e])
+        out[vd] = s
+    return out
+
+
+def _tri_area(a, b, c):
+    return abs((b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0])) / 2.0
+
+
+def check_conservative(cmap, src, src_geo, src_om, dst):
+    """check_if_remapping_is_conservative (src/mesh_mapping_module.f90:2416-2566):
+    remap ones with the 2nd-order weights; where an interior destination vertex
+    is off by more than 0.1, replace its entries by triangle weights of the
+    containing source triangle (keeping the reference's weights for two
+    entries, which repeat Aa/(Aa+Ab) when C is the smallest).  Raises where the
+    reference stops (more than ceil(nV/100) such vertices).  Returns the number
+    of replaced vertices."""
+    lib = load()
+    ones = np.ones(src.nV)
+    ddx, ddy = np.zeros(src.nV), np.zeros(src.nV)
+    op("get_mesh_derivatives", src_om, ones, ddx, ddy)
+    d = apply_conservative(cmap, ones, ddx, ddy)
+    d[dst.edge_index > 0] = 1.0
+    gs, ks = _geom_of(src, src_geo)
+    TriC = np.asfortranarray(src.TriC, dtype=np.int32)
+    n_wrong = 0
+    for vd in np.nonzero(np.abs(1.0 - d) > 1e-1)[0]:
+        n_wrong += 1
+        p = (ct.c_double * 2)(dst.V[vd, 0], dst.V[vd, 1])
+        t = lib.om_find_containing_triangle(ct.byref(gs), TriC.ctypes.data_as(ct.c_void_p), p, 1)
+        tri = src_geo["Tri"][t - 1]
+        pa, pb, pc = (src.V[tri[k] - 1] for k in range(3))
+        pp = dst.V[vd]
+        At, Aa, Ab, Ac = _tri_area(pa, pb, pc), _tri_area(pb, pc, pp), _tri_area(pc, pa, pp), _tri_area(pa, pb, pp)
+        s, e = cmap.start[vd], cmap.end[vd]
+        cmap.w1x[s:e] = cmap.w1y[s:e] = 0.0
+        if e - s == 2:
+            if Aa <= Ab and Aa <= Ac:
+                vals = ([tri[1], tri[2]], [Ab / (Ab + Ac), Ac / (Ab + Ac)])
+            elif Ab <= Aa and Ab <= Ac:
+                vals = ([tri[0], tri[2]], [Aa / (Aa + Ac), Ac / (Aa + Ac)])
+            else:
+                vals = ([tri[0], tri[1]], [Aa / (Aa + Ab), Aa / (Aa + Ab)])
+        elif e - s == 1:
+            k = 0 if (Aa >= Ab and Aa >= Ac) else (1 if (Ab >= Aa and Ab >= Ac) else 2)
+            vals = ([tri[k]], [1.0])
+        else:
+            vals = ([tri[0], tri[1], tri[2]], [Aa / At, Ab / At, Ac / At])
+        n = len(vals[0])
+        cmap.vi[s:s + n] = vals[0]
+        cmap.w0[s:s + n] = vals[1]
+        cmap.end[vd] = s + n
+    del ks
+    if n_wrong > int(np.ceil(dst.nV / 100.0)):
+        raise RuntimeError(f"2nd order conservative remapping goes wrong for {n_wrong} out of {dst.nV} vertices")
+    return n_wrong

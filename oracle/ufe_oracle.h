@@ -238,6 +238,18 @@ void om_find_triangle_areas(const om_geom *g, double *TriA);
 /* determine_mesh_resolution (:193-216) */
 void om_determine_mesh_resolution(const om_geom *g, double *R, double *rmin, double *rmax);
 
+/* ---- remapping, src/mesh_mapping_module.f90 (ufe_oracle_remap.c) ---- */
+void om_voronoi_geometric_centres(const om_geom *g, const double *A, double *GC);
+int om_cell_polygon(const om_geom *g, int vi, double *P, int cap);
+int om_clip_convex(const double *S, int n, const double *C, int k, double *out, int cap);
+int om_find_containing_vertex(const om_geom *g, const double p[2], int vi);
+int om_find_containing_triangle(const om_geom *g, const int *TriC, const double p[2], int t);
+int om_remap_conservative(const om_geom *src, const double *A_src, const double *GC_src, const om_geom *dst,
+                          const double *A_dst, int *ptr, int *vi, double *w0, double *w1x, double *w1y,
+                          int max_total);
+int om_remap_trilin_nearest(const om_geom *src, const int *TriC, const om_geom *dst, int *tri_vi, double *tri_w,
+                            int *nn_vi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,7 @@ STATUS_NAMES = {0: "OK", 1: "SOR_NOT_CONVERGED", 2: "SSA_RESET", 3: "SSA_UNSTABL
 EXPORTS = [
     "ufe_ctx_create", "ufe_ctx_destroy", "ufe_set_params", "ufe_last_error", "ufe_version",
     "ufe_mesh_build", "ufe_mesh_upload", "ufe_mesh_sizes", "ufe_mesh_geometry",
+    "ufe_remap_build", "ufe_remap_field", "ufe_remap_ice_model", "ufe_remap_export",
     "ufe_field_count", "ufe_field_name", "ufe_field_id", "ufe_field_info", "ufe_field_set",
     "ufe_field_get", "ufe_field_set_device", "ufe_field_get_device",
     "ufe_update_general_ice_model_data", "ufe_ice_physical_properties", "ufe_solve_SIA", "ufe_solve_SSA",
@@ -77,6 +78,36 @@ class MeshGeometryStats(ct.Structure):
                 ("area_error", ct.c_double), ("rotated_triangles", ct.c_int32),
                 ("invalid_circumcentres", ct.c_int32), ("voronoi_outside", ct.c_int32),
                 ("connection_failures", ct.c_int32)]
+
+
+class RemapStats(ct.Structure):
+    _fields_ = [("n_entries", ct.c_int32), ("n_fallback", ct.c_int32), ("trilin_missing", ct.c_int32),
+                ("seconds", ct.c_double)]
+
+
+UFE_REMAP_TRILIN, UFE_REMAP_NEAREST_NEIGHBOUR, UFE_REMAP_CONS_1ST_ORDER, UFE_REMAP_CONS_2ND_ORDER = 0, 1, 2, 3
+REMAP_METHODS = {"trilin": 0, "nearest_neighbour": 1, "cons_1st_order": 2, "cons_2nd_order": 3}
+
+
+def geometry_desc(ma):
+    """MeshGeometryDesc of a built mesh (MeshArrays with the geometry of
+    Context.mesh_build); the returned tuple keeps its arrays alive."""
+    m = ma.mesh
+    g = MeshGeometryDesc()
+    g.nV, g.nTri, g.nC_mem = m.nV, m.nTri, m.nC_mem
+    g.xmin, g.xmax, g.ymin, g.ymax = m.xmin, m.xmax, m.ymin, m.ymax
+    g.tol_dist = m.tol_dist
+    keep = []
+    for name, a, dt in (("V", m.V, np.float64), ("nC", m.nC, np.int32), ("C", m.C, np.int32),
+                        ("niTri", m.niTri, np.int32), ("iTri", m.iTri, np.int32),
+                        ("edge_index", m.edge_index, np.int32), ("Tri_edge_index", m.Tri_edge_index, np.int32),
+                        ("Tri", ma["Tri"], np.int32), ("Tricc", ma["Tricc"], np.float64),
+                        ("A", ma["A"], np.float64)):
+        a = np.asfortranarray(a, dtype=dt)
+        keep.append(a)
+        setattr(g, name, a.ctypes.data)
+    g.compute_Tricc = 0
+    return g, keep
 
 
 class Params(ct.Structure):
@@ -387,6 +418,39 @@ class Context:
     def calculate_ice_thickness_change(self, dt):
         self._chk(self.lib.ufe_calculate_ice_thickness_change(self.h, float(dt)),
                   "calculate_ice_thickness_change")
+
+    # -- remapping from another context's mesh onto this one --------------------
+    def remap_build(self, src: "Context", ma_src, ma_dst) -> RemapStats:
+        """create_remapping_arrays from the mesh of src (ma_src) to this
+        context's mesh (ma_dst); both built with their geometry."""
+        gs, ks = geometry_desc(ma_src)
+        gd, kd = geometry_desc(ma_dst)
+        st = RemapStats()
+        self._chk(self.lib.ufe_remap_build(self.h, src.h, ct.byref(gs), ct.byref(gd), ct.byref(st)),
+                  "ufe_remap_build")
+        del ks, kd
+        return st
+
+    def remap_field(self, src: "Context", method, d_src, d_dst, nz=1):
+        """remap_field_dp on device pointers; method is a name or UFE_REMAP_*."""
+        m = REMAP_METHODS.get(method, method)
+        self._chk(self.lib.ufe_remap_field(self.h, src.h, ct.c_int(int(m)), ct.c_void_p(int(d_src)),
+                                           ct.c_void_p(int(d_dst)), ct.c_int(int(nz))), "ufe_remap_field")
+
+    def remap_ice_model(self, src: "Context"):
+        self._chk(self.lib.ufe_remap_ice_model(self.h, src.h), "ufe_remap_ice_model")
+
+    def remap_export(self):
+        n = ct.c_int(0)
+        self._chk(self.lib.ufe_remap_export(self.h, ct.byref(n), *([None] * 9)), "ufe_remap_export")
+        nVd = self.sizes["nV"]
+        out = {"nn": np.zeros(nVd, np.int32), "tvi": np.zeros((nVd, 3), np.int32, order="F"),
+               "tw": np.zeros((nVd, 3), order="F"), "start": np.zeros(nVd, np.int32),
+               "end": np.zeros(nVd, np.int32), "vi": np.zeros(n.value, np.int32), "w0": np.zeros(n.value),
+               "w1x": np.zeros(n.value), "w1y": np.zeros(n.value)}
+        ptrs = [_ptr(out[k]) for k in ("nn", "tvi", "tw", "start", "end", "vi", "w0", "w1x", "w1y")]
+        self._chk(self.lib.ufe_remap_export(self.h, ct.byref(n), *ptrs), "ufe_remap_export")
+        return out
 
     def solve_SIA_3D(self):
         self._chk(self.lib.ufe_solve_SIA_3D(self.h), "solve_SIA_3D")

@@ -130,6 +130,11 @@ struct ufe_ctx {
   int32_t *d_Tri = nullptr, *d_niTri = nullptr, *d_iTri = nullptr;
   double *d_NxTri = nullptr, *d_NyTri = nullptr, *d_R = nullptr;
   double *d_crit = nullptr;          // critical time step partial minima
+  // remapping arrays from another context's mesh onto this one (ufe_remap_build)
+  bool rm_ready = false;
+  int rm_nVs = 0, rm_nVd = 0, rm_n = 0;
+  int32_t *d_rm_nn = nullptr, *d_rm_tvi = nullptr, *d_rm_st = nullptr, *d_rm_en = nullptr, *d_rm_vi = nullptr;
+  double *d_rm_tw = nullptr, *d_rm_w0 = nullptr, *d_rm_w1x = nullptr, *d_rm_w1y = nullptr;
   int32_t *d_icount = nullptr;       // small device counters (upwind misses)
   int32_t *d_Aci = nullptr, *d_edgeAc = nullptr, *d_slot_i = nullptr, *d_slot_j = nullptr;
   double *d_NxAc = nullptr, *d_NyAc = nullptr, *d_NoAc = nullptr, *d_NpAc = nullptr;
@@ -290,6 +295,10 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->have_mesh = false;
   ctx->d_Tri = ctx->d_niTri = ctx->d_iTri = ctx->d_icount = nullptr;
   ctx->d_NxTri = ctx->d_NyTri = ctx->d_R = ctx->d_crit = nullptr;
+  ctx->rm_ready = false;
+  ctx->rm_nVs = ctx->rm_nVd = ctx->rm_n = 0;
+  ctx->d_rm_nn = ctx->d_rm_tvi = ctx->d_rm_st = ctx->d_rm_en = ctx->d_rm_vi = nullptr;
+  ctx->d_rm_tw = ctx->d_rm_w0 = ctx->d_rm_w1x = ctx->d_rm_w1y = nullptr;
   // work spaces live in allocs too: forget them with the mesh
   ctx->kry_m = 0;
   ctx->kry_V = ctx->kry_Z = ctx->kry_w = ctx->kry_x = nullptr;
@@ -810,6 +819,350 @@ int ufe_mesh_geometry(ufe_ctx *ctx, ufe_mesh_geometry_desc *g, ufe_mesh_geometry
     ctx->err = "ufe_mesh_geometry: a connection has no shared triangle, or one that is not a boundary triangle";
     return UFE_BAD_INPUT;
   }
+  return UFE_OK;
+}
+
+
+// ---- remapping between meshes -------------------------------------------------
+namespace {
+// Device copy of one primary mesh with its geometry, for the remapping build.
+struct GeoDev {
+  GeomMesh m{};
+  const double *A = nullptr;
+};
+}  // namespace
+
+static int upload_geo(ufe_ctx *ctx, ScratchDev &s, const ufe_mesh_geometry_desc *g, GeoDev &out) {
+  const size_t nV = g->nV, nT = g->nTri, nCm = g->nC_mem;
+  double *V, *cc, *A;
+  int32_t *nC, *C, *niTri, *iTri, *edge, *Tri, *tei;
+  HIPCHK(ctx, s.alloc((void **)&V, sizeof(double) * 2 * nV));
+  HIPCHK(ctx, s.alloc((void **)&cc, sizeof(double) * 2 * nT));
+  HIPCHK(ctx, s.alloc((void **)&A, sizeof(double) * nV));
+  HIPCHK(ctx, s.alloc((void **)&nC, sizeof(int32_t) * nV));
+  HIPCHK(ctx, s.alloc((void **)&C, sizeof(int32_t) * nV * nCm));
+  HIPCHK(ctx, s.alloc((void **)&niTri, sizeof(int32_t) * nV));
+  HIPCHK(ctx, s.alloc((void **)&iTri, sizeof(int32_t) * nV * nCm));
+  HIPCHK(ctx, s.alloc((void **)&edge, sizeof(int32_t) * nV));
+  HIPCHK(ctx, s.alloc((void **)&Tri, sizeof(int32_t) * 3 * nT));
+  HIPCHK(ctx, s.alloc((void **)&tei, sizeof(int32_t) * nT));
+  hipStream_t q = ctx->stream;
+  HIPCHK(ctx, hipMemcpyAsync(V, g->V, sizeof(double) * 2 * nV, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(cc, g->Tricc, sizeof(double) * 2 * nT, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(A, g->A, sizeof(double) * nV, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(nC, g->nC, sizeof(int32_t) * nV, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(C, g->C, sizeof(int32_t) * nV * nCm, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(niTri, g->niTri, sizeof(int32_t) * nV, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(iTri, g->iTri, sizeof(int32_t) * nV * nCm, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(edge, g->edge_index, sizeof(int32_t) * nV, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(Tri, g->Tri, sizeof(int32_t) * 3 * nT, hipMemcpyHostToDevice, q));
+  HIPCHK(ctx, hipMemcpyAsync(tei, g->Tri_edge_index, sizeof(int32_t) * nT, hipMemcpyHostToDevice, q));
+  GeomMesh &m = out.m;
+  m.nV = g->nV;
+  m.nTri = g->nTri;
+  m.nCm = g->nC_mem;
+  m.xmin = g->xmin;
+  m.xmax = g->xmax;
+  m.ymin = g->ymin;
+  m.ymax = g->ymax;
+  m.tol = g->tol_dist;
+  m.V = V;
+  m.nC = nC;
+  m.C = C;
+  m.niTri = niTri;
+  m.iTri = iTri;
+  m.edge = edge;
+  m.Tri = Tri;
+  m.tei = tei;
+  m.cc = cc;
+  out.A = A;
+  return UFE_OK;
+}
+
+static double rm_area_host(const double *a, const double *b, const double *c) {
+  return std::fabs((b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0])) / 2.0;
+}
+
+int ufe_remap_build(ufe_ctx *dst, ufe_ctx *src, const ufe_mesh_geometry_desc *gs, const ufe_mesh_geometry_desc *gd,
+                    ufe_remap_stats *st) {
+  if (!dst || !src || !dst->have_mesh || !src->have_mesh) return UFE_BAD_INPUT;
+  if (dst->device != src->device) {
+    dst->err = "ufe_remap_build: both contexts must use the same device";
+    return UFE_BAD_INPUT;
+  }
+  int r;
+  if ((r = check_geometry(dst, gs)) || (r = check_geometry(dst, gd))) return r;
+  if (!gs->A || !gd->A || gs->nV != src->nV || gd->nV != dst->nV || gs->nC_mem > 32 || gd->nC_mem > 32) {
+    dst->err = "ufe_remap_build: geometry descriptors must hold A and match the two meshes (nC_mem <= 32)";
+    return UFE_BAD_INPUT;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(dst, hipSetDevice(dst->device));
+  HIPCHK(dst, hipStreamSynchronize(src->stream));
+  hipStream_t q = dst->stream;
+  const int nVs = gs->nV, nVd = gd->nV;
+  ScratchDev s;
+  GeoDev S, D;
+  if ((r = upload_geo(dst, s, gs, S)) || (r = upload_geo(dst, s, gd, D))) return r;
+  double *Ps, *Pd, *GCs;
+  int32_t *nPs, *nPd, *cnt, *count;
+  HIPCHK(dst, s.alloc((void **)&Ps, sizeof(double) * 2 * RM_MAXP * (size_t)nVs));
+  HIPCHK(dst, s.alloc((void **)&Pd, sizeof(double) * 2 * RM_MAXP * (size_t)nVd));
+  HIPCHK(dst, s.alloc((void **)&GCs, sizeof(double) * 2 * (size_t)nVs));
+  HIPCHK(dst, s.alloc((void **)&nPs, sizeof(int32_t) * nVs));
+  HIPCHK(dst, s.alloc((void **)&nPd, sizeof(int32_t) * nVd));
+  HIPCHK(dst, s.alloc((void **)&cnt, sizeof(int32_t) * nVd));
+  HIPCHK(dst, s.alloc((void **)&count, sizeof(int32_t) * RM_NCOUNT));
+  HIPCHK(dst, hipMemsetAsync(count, 0, sizeof(int32_t) * RM_NCOUNT, q));
+  k_rm_cells<<<nblk(nVs, 64), 64, 0, q>>>(S.m, S.A, Ps, nPs, GCs, count);
+  k_rm_cells<<<nblk(nVd, 64), 64, 0, q>>>(D.m, D.A, Pd, nPd, nullptr, count);
+  LAUNCHCHK(dst);
+  // bucket grid of the source vertices: about one vertex per cell
+  const double Lx = gs->xmax - gs->xmin, Ly = gs->ymax - gs->ymin;
+  const double h = std::sqrt(Lx * Ly / std::max(nVs, 1));
+  RmGrid G;
+  G.nx = std::max(1, (int)std::ceil(Lx / h));
+  G.ny = std::max(1, (int)std::ceil(Ly / h));
+  G.x0 = gs->xmin;
+  G.y0 = gs->ymin;
+  G.h = h;
+  std::vector<int32_t> gstart((size_t)G.nx * G.ny + 1, 0), gitem(nVs), cell(nVs);
+  for (int v = 0; v < nVs; v++) {
+    int cx = (int)std::floor((gs->V[v] - G.x0) / h), cy = (int)std::floor((gs->V[nVs + v] - G.y0) / h);
+    cx = std::min(std::max(cx, 0), G.nx - 1);
+    cy = std::min(std::max(cy, 0), G.ny - 1);
+    cell[v] = cy * G.nx + cx;
+    gstart[cell[v] + 1]++;
+  }
+  for (size_t c = 0; c + 1 < gstart.size(); c++) gstart[c + 1] += gstart[c];
+  {
+    std::vector<int32_t> fillp(gstart.begin(), gstart.end() - 1);
+    for (int v = 0; v < nVs; v++) gitem[fillp[cell[v]]++] = v;
+  }
+  int32_t *dgs, *dgi;
+  HIPCHK(dst, s.alloc((void **)&dgs, sizeof(int32_t) * gstart.size()));
+  HIPCHK(dst, s.alloc((void **)&dgi, sizeof(int32_t) * std::max(nVs, 1)));
+  HIPCHK(dst, hipMemcpyAsync(dgs, gstart.data(), sizeof(int32_t) * gstart.size(), hipMemcpyHostToDevice, q));
+  HIPCHK(dst, hipMemcpyAsync(dgi, gitem.data(), sizeof(int32_t) * nVs, hipMemcpyHostToDevice, q));
+  G.start = dgs;
+  G.item = dgi;
+  // persistent arrays of the remapping, freed with the destination mesh
+  dst->rm_ready = false;
+  if ((r = dalloc(dst, &dst->d_rm_nn, nVd)) || (r = dalloc(dst, &dst->d_rm_tvi, 3 * (size_t)nVd)) ||
+      (r = dalloc(dst, &dst->d_rm_tw, 3 * (size_t)nVd)) || (r = dalloc(dst, &dst->d_rm_st, nVd)) ||
+      (r = dalloc(dst, &dst->d_rm_en, nVd)))
+    return r;
+  k_rm_nearest<<<nblk(nVd), BLOCK, 0, q>>>(nVd, D.m.V, G, S.m.V, nVs, dst->d_rm_nn);
+  k_rm_trilin<<<nblk(nVd, 64), 64, 0, q>>>(nVd, D.m.V, S.m, dst->d_rm_nn, dst->d_rm_tvi, dst->d_rm_tw, count);
+  LAUNCHCHK(dst);
+  RmCons R;
+  R.nVd = nVd;
+  R.Pd = Pd;
+  R.Ps = Ps;
+  R.nPd = nPd;
+  R.nPs = nPs;
+  R.Ad = D.A;
+  R.GCs = GCs;
+  R.edge_d = D.m.edge;
+  R.nn = dst->d_rm_nn;
+  R.nCs = S.m.nC;
+  R.Cs = S.m.C;
+  R.nVs = nVs;
+  R.xmin = gd->xmin;
+  R.xmax = gd->xmax;
+  R.ymin = gd->ymin;
+  R.ymax = gd->ymax;
+  R.tol = gd->tol_dist;
+  k_rm_cons<<<nblk(nVd, 64), 64, 0, q>>>(R, 0, nullptr, cnt, nullptr, nullptr, nullptr, nullptr, count);
+  LAUNCHCHK(dst);
+  std::vector<int32_t> hc(nVd), hst(nVd), hen(nVd);
+  HIPCHK(dst, hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * nVd, hipMemcpyDeviceToHost, q));
+  int32_t hcount[RM_NCOUNT];
+  HIPCHK(dst, hipMemcpyAsync(hcount, count, sizeof(hcount), hipMemcpyDeviceToHost, q));
+  HIPCHK(dst, hipStreamSynchronize(q));
+  if (hcount[RM_OVERFLOW]) {
+    dst->err = "ufe_remap_build: a cell polygon or overlap list exceeds its capacity";
+    return UFE_BAD_INPUT;
+  }
+  int64_t tot = 0;
+  for (int v = 0; v < nVd; v++) {
+    hst[v] = (int32_t)tot;
+    tot += hc[v];
+    hen[v] = (int32_t)tot;
+  }
+  if (tot > INT32_MAX) {
+    dst->err = "ufe_remap_build: too many remapping entries";
+    return UFE_BAD_INPUT;
+  }
+  const int n = (int)tot;
+  if ((r = dalloc(dst, &dst->d_rm_vi, n)) || (r = dalloc(dst, &dst->d_rm_w0, n)) ||
+      (r = dalloc(dst, &dst->d_rm_w1x, n)) || (r = dalloc(dst, &dst->d_rm_w1y, n)))
+    return r;
+  HIPCHK(dst, hipMemcpyAsync(dst->d_rm_st, hst.data(), sizeof(int32_t) * nVd, hipMemcpyHostToDevice, q));
+  HIPCHK(dst, hipMemcpyAsync(dst->d_rm_en, hen.data(), sizeof(int32_t) * nVd, hipMemcpyHostToDevice, q));
+  k_rm_cons<<<nblk(nVd, 64), 64, 0, q>>>(R, 1, dst->d_rm_st, cnt, dst->d_rm_vi, dst->d_rm_w0, dst->d_rm_w1x,
+                                         dst->d_rm_w1y, count);
+  LAUNCHCHK(dst);
+  // check_if_remapping_is_conservative (src/mesh_mapping_module.f90:2416-2566):
+  // 2nd-order remapping of ones; interior destination vertices off by more
+  // than 0.1 take triangle weights instead
+  double *ones, *dx, *dy, *dd;
+  HIPCHK(dst, s.alloc((void **)&ones, sizeof(double) * nVs));
+  HIPCHK(dst, s.alloc((void **)&dx, sizeof(double) * nVs));
+  HIPCHK(dst, s.alloc((void **)&dy, sizeof(double) * nVs));
+  HIPCHK(dst, s.alloc((void **)&dd, sizeof(double) * nVd));
+  k_fill<double><<<nblk(nVs), BLOCK, 0, q>>>(ones, nVs, 1.0);
+  k_deriv_Aa<<<nblk(nVs), BLOCK, 0, q>>>(aa_mesh(src), ones, dx, dy);
+  k_rm_apply_cons<<<nblk(nVd), BLOCK, 0, q>>>(nVd, nVs, 1, dst->d_rm_st, dst->d_rm_en, dst->d_rm_vi, dst->d_rm_w0,
+                                              dst->d_rm_w1x, dst->d_rm_w1y, ones, dx, dy, dd);
+  LAUNCHCHK(dst);
+  std::vector<double> hd(nVd);
+  HIPCHK(dst, hipMemcpyAsync(hd.data(), dd, sizeof(double) * nVd, hipMemcpyDeviceToHost, q));
+  HIPCHK(dst, hipMemcpyAsync(hcount, count, sizeof(hcount), hipMemcpyDeviceToHost, q));
+  HIPCHK(dst, hipStreamSynchronize(q));
+  std::vector<int> wrong;
+  for (int v = 0; v < nVd; v++)
+    if (gd->edge_index[v] == 0 && std::fabs(1.0 - hd[v]) > 1E-1) wrong.push_back(v);
+  if (!wrong.empty()) {
+    std::vector<int32_t> hvi(n), htvi(3 * (size_t)nVd);
+    std::vector<double> hw0(n), hw1x(n), hw1y(n);
+    HIPCHK(dst, hipMemcpyAsync(hvi.data(), dst->d_rm_vi, sizeof(int32_t) * n, hipMemcpyDeviceToHost, q));
+    HIPCHK(dst, hipMemcpyAsync(hw0.data(), dst->d_rm_w0, sizeof(double) * n, hipMemcpyDeviceToHost, q));
+    HIPCHK(dst, hipMemcpyAsync(hw1x.data(), dst->d_rm_w1x, sizeof(double) * n, hipMemcpyDeviceToHost, q));
+    HIPCHK(dst, hipMemcpyAsync(hw1y.data(), dst->d_rm_w1y, sizeof(double) * n, hipMemcpyDeviceToHost, q));
+    HIPCHK(dst, hipMemcpyAsync(htvi.data(), dst->d_rm_tvi, sizeof(int32_t) * 3 * nVd, hipMemcpyDeviceToHost, q));
+    HIPCHK(dst, hipStreamSynchronize(q));
+    for (int v : wrong) {
+      const int s0 = hst[v], ne = hen[v] - hst[v];
+      if (ne < 1) continue;
+      const int ia = htvi[v], ib = htvi[nVd + v], ic = htvi[2 * (size_t)nVd + v];
+      const double pa[2] = {gs->V[ia - 1], gs->V[nVs + ia - 1]}, pb[2] = {gs->V[ib - 1], gs->V[nVs + ib - 1]},
+                   pc[2] = {gs->V[ic - 1], gs->V[nVs + ic - 1]}, p[2] = {gd->V[v], gd->V[nVd + v]};
+      const double At = rm_area_host(pa, pb, pc), Aa = rm_area_host(pb, pc, p), Ab = rm_area_host(pc, pa, p),
+                   Ac = rm_area_host(pa, pb, p);
+      for (int e = s0; e < hen[v]; e++) hvi[e] = 0, hw0[e] = hw1x[e] = hw1y[e] = 0.0;
+      if (ne == 2) {
+        if (Aa <= Ab && Aa <= Ac) {
+          hvi[s0] = ib, hvi[s0 + 1] = ic, hw0[s0] = Ab / (Ab + Ac), hw0[s0 + 1] = Ac / (Ab + Ac);
+        } else if (Ab <= Aa && Ab <= Ac) {
+          hvi[s0] = ia, hvi[s0 + 1] = ic, hw0[s0] = Aa / (Aa + Ac), hw0[s0 + 1] = Ac / (Aa + Ac);
+        } else {   // the reference lists Aa/(Aa+Ab) twice here (:2491)
+          hvi[s0] = ia, hvi[s0 + 1] = ib, hw0[s0] = Aa / (Aa + Ab), hw0[s0 + 1] = Aa / (Aa + Ab);
+        }
+        hen[v] = s0 + 2;
+      } else if (ne == 1) {
+        hvi[s0] = (Aa >= Ab && Aa >= Ac) ? ia : ((Ab >= Aa && Ab >= Ac) ? ib : ic);
+        hw0[s0] = 1.0;
+        hen[v] = s0 + 1;
+      } else {
+        hvi[s0] = ia, hvi[s0 + 1] = ib, hvi[s0 + 2] = ic;
+        hw0[s0] = Aa / At, hw0[s0 + 1] = Ab / At, hw0[s0 + 2] = Ac / At;
+        hen[v] = s0 + 3;
+      }
+    }
+    HIPCHK(dst, hipMemcpyAsync(dst->d_rm_vi, hvi.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, q));
+    HIPCHK(dst, hipMemcpyAsync(dst->d_rm_w0, hw0.data(), sizeof(double) * n, hipMemcpyHostToDevice, q));
+    HIPCHK(dst, hipMemcpyAsync(dst->d_rm_w1x, hw1x.data(), sizeof(double) * n, hipMemcpyHostToDevice, q));
+    HIPCHK(dst, hipMemcpyAsync(dst->d_rm_w1y, hw1y.data(), sizeof(double) * n, hipMemcpyHostToDevice, q));
+    HIPCHK(dst, hipMemcpyAsync(dst->d_rm_en, hen.data(), sizeof(int32_t) * nVd, hipMemcpyHostToDevice, q));
+    HIPCHK(dst, hipStreamSynchronize(q));
+  }
+  ufe_remap_stats t{};
+  t.n_entries = n;
+  t.n_fallback = (int32_t)wrong.size();
+  t.trilin_missing = hcount[RM_TRI_MISSING];
+  t.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (st) *st = t;
+  if ((int64_t)wrong.size() > (int64_t)std::ceil((double)nVd / 100.0)) {
+    dst->err = "ufe_remap_build: 2nd order conservative remapping goes wrong for more than 1% of the vertices";
+    return UFE_BAD_INPUT;
+  }
+  if (hcount[RM_TRI_MISSING]) {
+    dst->err = "ufe_remap_build: couldnt find the source triangle containing a destination vertex";
+    return UFE_BAD_INPUT;
+  }
+  dst->rm_nVs = nVs;
+  dst->rm_nVd = nVd;
+  dst->rm_n = n;
+  dst->rm_ready = true;
+  return UFE_OK;
+}
+
+int ufe_remap_field(ufe_ctx *dst, ufe_ctx *src, int method, const double *d_src, double *d_dst, int nz) {
+  if (!dst || !src || !dst->rm_ready || !d_src || !d_dst || nz < 1) return UFE_BAD_INPUT;
+  if (src->nV != dst->rm_nVs || dst->nV != dst->rm_nVd) {
+    dst->err = "ufe_remap_field: the contexts do not hold the meshes of the remapping";
+    return UFE_BAD_INPUT;
+  }
+  hipStream_t q = dst->stream;
+  HIPCHK(dst, hipStreamSynchronize(src->stream));
+  const int nVs = dst->rm_nVs, nVd = dst->rm_nVd;
+  const int64_t nd = (int64_t)nVd * nz;
+  ScratchDev s;
+  switch (method) {
+    case UFE_REMAP_TRILIN:
+      k_rm_apply_trilin<<<nblk(nd), BLOCK, 0, q>>>(nVd, nVs, nz, dst->d_rm_tvi, dst->d_rm_tw, d_src, d_dst);
+      break;
+    case UFE_REMAP_NEAREST_NEIGHBOUR:
+      k_rm_apply_nn<<<nblk(nd), BLOCK, 0, q>>>(nVd, nVs, nz, dst->d_rm_nn, d_src, d_dst);
+      break;
+    case UFE_REMAP_CONS_1ST_ORDER:
+      k_rm_apply_cons<<<nblk(nd), BLOCK, 0, q>>>(nVd, nVs, nz, dst->d_rm_st, dst->d_rm_en, dst->d_rm_vi, dst->d_rm_w0,
+                                                 dst->d_rm_w1x, dst->d_rm_w1y, d_src, nullptr, nullptr, d_dst);
+      break;
+    case UFE_REMAP_CONS_2ND_ORDER: {
+      double *dx, *dy;
+      HIPCHK(dst, s.alloc((void **)&dx, sizeof(double) * (size_t)nVs * nz));
+      HIPCHK(dst, s.alloc((void **)&dy, sizeof(double) * (size_t)nVs * nz));
+      k_deriv_Aa_3D<<<nblk((int64_t)nVs * nz), BLOCK, 0, q>>>(aa_mesh(src), nz, d_src, dx, dy);
+      k_rm_apply_cons<<<nblk(nd), BLOCK, 0, q>>>(nVd, nVs, nz, dst->d_rm_st, dst->d_rm_en, dst->d_rm_vi, dst->d_rm_w0,
+                                                 dst->d_rm_w1x, dst->d_rm_w1y, d_src, dx, dy, d_dst);
+      LAUNCHCHK(dst);
+      HIPCHK(dst, hipStreamSynchronize(q));
+      return UFE_OK;
+    }
+    default:
+      dst->err = "ufe_remap_field: unknown method";
+      return UFE_BAD_INPUT;
+  }
+  LAUNCHCHK(dst);
+  HIPCHK(dst, hipStreamSynchronize(q));
+  return UFE_OK;
+}
+
+int ufe_remap_ice_model(ufe_ctx *dst, ufe_ctx *src) {
+  if (!dst || !src || !dst->rm_ready) return UFE_BAD_INPUT;
+  if (src->nZ != dst->nZ) {
+    dst->err = "ufe_remap_ice_model: the two contexts have different nZ";
+    return UFE_BAD_INPUT;
+  }
+  int r;
+  // remap_ice_model (src/ice_dynamics_module.f90:1198-1394): Hi, Hi_prev and
+  // Ti conservatively (1st order); the velocities start again from zero
+  if ((r = ufe_remap_field(dst, src, UFE_REMAP_CONS_1ST_ORDER, F(src, "Hi"), F(dst, "Hi"), 1))) return r;
+  if ((r = ufe_remap_field(dst, src, UFE_REMAP_CONS_1ST_ORDER, F(src, "Hi_prev"), F(dst, "Hi_prev"), 1))) return r;
+  if ((r = ufe_remap_field(dst, src, UFE_REMAP_CONS_1ST_ORDER, F(src, "Ti"), F(dst, "Ti"), dst->nZ))) return r;
+  for (const char *f : {"U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac"})
+    if ((r = ufe_field_fill(dst, field_id(f), 0.0))) return r;
+  return UFE_OK;
+}
+
+int ufe_remap_export(ufe_ctx *dst, int *n_entries, int32_t *nn, int32_t *tvi, double *tw, int32_t *start,
+                     int32_t *end, int32_t *vi, double *w0, double *w1x, double *w1y) {
+  if (!dst || !dst->rm_ready || !n_entries) return UFE_BAD_INPUT;
+  *n_entries = dst->rm_n;
+  hipStream_t q = dst->stream;
+  const size_t nVd = dst->rm_nVd, n = dst->rm_n;
+  if (nn) HIPCHK(dst, hipMemcpyAsync(nn, dst->d_rm_nn, sizeof(int32_t) * nVd, hipMemcpyDeviceToHost, q));
+  if (tvi) HIPCHK(dst, hipMemcpyAsync(tvi, dst->d_rm_tvi, sizeof(int32_t) * 3 * nVd, hipMemcpyDeviceToHost, q));
+  if (tw) HIPCHK(dst, hipMemcpyAsync(tw, dst->d_rm_tw, sizeof(double) * 3 * nVd, hipMemcpyDeviceToHost, q));
+  if (start) HIPCHK(dst, hipMemcpyAsync(start, dst->d_rm_st, sizeof(int32_t) * nVd, hipMemcpyDeviceToHost, q));
+  if (end) HIPCHK(dst, hipMemcpyAsync(end, dst->d_rm_en, sizeof(int32_t) * nVd, hipMemcpyDeviceToHost, q));
+  if (vi) HIPCHK(dst, hipMemcpyAsync(vi, dst->d_rm_vi, sizeof(int32_t) * n, hipMemcpyDeviceToHost, q));
+  if (w0) HIPCHK(dst, hipMemcpyAsync(w0, dst->d_rm_w0, sizeof(double) * n, hipMemcpyDeviceToHost, q));
+  if (w1x) HIPCHK(dst, hipMemcpyAsync(w1x, dst->d_rm_w1x, sizeof(double) * n, hipMemcpyDeviceToHost, q));
+  if (w1y) HIPCHK(dst, hipMemcpyAsync(w1y, dst->d_rm_w1y, sizeof(double) * n, hipMemcpyDeviceToHost, q));
+  HIPCHK(dst, hipStreamSynchronize(q));
   return UFE_OK;
 }
 
