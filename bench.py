@@ -86,6 +86,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble")
     ap.add_argument("--no-tune", action="store_true", help="keep the default sweep launch shape")
     ap.add_argument("--no-ref-steps", action="store_true", help="skip the FP64 reference-step configs")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the paths around the solve (geometry, physical properties, SIA-3D, time step, "
+                         "upwind derivatives, remapping)")
+    ap.add_argument("--remap-nv", type=int, default=200_000, help="vertices of the remapping meshes")
     ap.add_argument("--batch-n", type=int, default=64, help="members of the batched ensemble")
     ap.add_argument("--batch-min", type=int, default=5000)
     ap.add_argument("--batch-max", type=int, default=100_000)
@@ -116,6 +120,13 @@ def main():
         ens = batch.ensemble(n=args.batch_n, nv_min=args.batch_min, nv_max=args.batch_max, seed=args.seed,
                              picard=args.batch_picard, workers=min(16, os.cpu_count() or 1))
         log(f"[batch] {len(ens)} meshes, {sum(w.mesh.nV for w in ens)} vertices, generated in {time.time() - tb:.1f}s")
+    remap_wls = None
+    if not args.no_rows:        # two meshes to remap between, generated before any GPU context
+        tb = time.time()
+        remap_wls = [geometry.marine(n_target=args.remap_nv, seed=args.seed + 100, picard=1),
+                     geometry.marine(n_target=int(args.remap_nv * 1.25), seed=args.seed + 200, picard=1)]
+        log(f"[rows] remapping meshes nV={remap_wls[0].mesh.nV} -> {remap_wls[1].mesh.nV} "
+            f"generated in {time.time() - tb:.1f}s")
     t0 = time.time()
     wl = geometry.marine(n_target=args.nv, seed=args.seed, picard=args.picard0)
     if args.omega > 0:
@@ -275,6 +286,8 @@ def main():
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
     if not args.no_step:
         out["precision_error"] = bench_precision(args, ctx, step, wl)
+    if not args.no_rows:
+        out["rows"] = bench_rows(args, ctx, step, wl, remap_wls)
     if not args.no_ref_steps:
         out["reference_steps"] = bench_reference_steps(args, ctx)
     if not args.no_adjoint:
@@ -283,6 +296,101 @@ def main():
         out["batch"] = bench_batch(args, ens)
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def bench_rows(args, ctx, step, wl, remap_wls, reps=10):
+    """The paths around the solve on the 1M mesh, each timed over `reps`
+    calls (every call ends with a stream synchronisation): the mesh geometry
+    builder, ice_physical_properties, solve_SIA_3D, the critical-time-step
+    reduction and the upwind derivatives; HBM rates use the algorithmic bytes
+    below (nZ = 15, mean degree n).  Remapping: build and field rates between
+    two marine meshes."""
+    import numpy as np
+    import torch
+    from ufemism_amd import abi
+    from ufemism_amd.dynamics import Step
+
+    nV, nAc, nZ = ctx.sizes["nV"], ctx.sizes["nAc"], wl.params.nZ
+    n = 6.0
+    res = {}
+
+    def timed(fn, k=reps):
+        fn()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(k):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t) / k
+
+    def row(name, sec, nbytes, extra=None):
+        r = {"ms": round(sec * 1e3, 3)}
+        if nbytes:
+            r["algorithmic_bytes"] = int(nbytes)
+            r["GBps"] = round(nbytes / sec / 1e9, 1)
+            r["hbm_frac"] = round(nbytes / sec / 8e12, 3)
+        if extra:
+            r.update(extra)
+        res[name] = r
+        log(f"[rows] {name}: {r}")
+
+    tg = time.perf_counter()
+    geo = ctx.mesh_geometry(wl.mesh)
+    row("mesh_geometry", time.perf_counter() - tg, None,
+        {"note": "Tricc, A, Cw, TriA, R of the 1M mesh, host-to-device copies and checks included",
+         "area_error": geo["stats"].area_error})
+    step.reset()
+    ctx.update_general_ice_model_data()
+    ctx.set("Ti", wl.temperature())
+    # Aa: Ti, Hi, mask in; Ti_pmp, A, Cpi, Ki, A_mean out; Ac: Aci, two Ti gathers, mask in; Ti_Ac, A, mean out
+    row("ice_physical_properties", timed(lambda: ctx.ice_physical_properties(1)),
+        nV * (40 * nZ + 20) + nAc * (32 * nZ + 20))
+    ctx.solve_SIA()
+    # U_3D/V_3D: 5 Aa fields + A_flow in, U_3D/V_3D out; W_3D: 6 Aa fields, Nx/Ny, own and
+    # neighbour U_3D/V_3D per layer in, W_3D out
+    sia3d = nV * ((24 * nZ + 44) + (56 + 20 * (n + 1) + 16 * (n + 1) * nZ + 8 * nZ))
+    row("solve_SIA_3D", timed(ctx.solve_SIA_3D), sia3d)
+    # per Ac point: D_SIA_Ac, Aci, both vertices' V, U_SSA, V_SSA; per vertex: A, U_SSA, V_SSA, U_3D, V_3D
+    row("critical_timesteps", timed(ctx.critical_timesteps), nAc * 80 + nV * (24 + 16 * nZ),
+        {"dt_crit": [round(x, 6) for x in ctx.critical_timesteps()]})
+    U = torch.randn(nV * nZ, dtype=torch.float64, device="cuda") * 100
+    V = torch.randn_like(U) * 100
+    d = torch.randn_like(U)
+    dx, dy = torch.empty_like(U), torch.empty_like(U)
+    torch.cuda.synchronize()
+    up = lambda: ctx.upwind_derivatives_3D(U.data_ptr(), V.data_ptr(), d.data_ptr(), dx.data_ptr(), dy.data_ptr(), nZ)
+    t_up = timed(up)
+    row("upwind_derivatives_3D", t_up, None, {"Mvertex_layers_per_s": round(nV * nZ / t_up / 1e6, 1)})
+    del U, V, d, dx, dy
+    if remap_wls:
+        ca, cb = abi.Context(0), abi.Context(0)
+        try:
+            Step(ca, remap_wls[0])
+            Step(cb, remap_wls[1])
+            ma, mb = ca.mesh_build(remap_wls[0].mesh), cb.mesh_build(remap_wls[1].mesh)
+            tb = time.perf_counter()
+            st = cb.remap_build(ca, ma, mb)
+            tbuild = time.perf_counter() - tb
+            nVa, nVb = remap_wls[0].mesh.nV, remap_wls[1].mesh.nV
+            src = torch.rand(nVa * nZ, dtype=torch.float64, device="cuda")
+            dst = torch.empty(nVb * nZ, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            rates = {}
+            for m in ("trilin", "nearest_neighbour", "cons_1st_order", "cons_2nd_order"):
+                sec = timed(lambda m=m: cb.remap_field(ca, m, src.data_ptr(), dst.data_ptr(), nZ))
+                rates[m] = {"ms": round(sec * 1e3, 3), "Mvertex_layers_per_s": round(nVb * nZ / sec / 1e6, 1)}
+            tr = time.perf_counter()
+            cb.remap_ice_model(ca)
+            res["remap"] = {"nV_src": nVa, "nV_dst": nVb, "build_seconds": round(tbuild, 3),
+                            "build_kernel_seconds": round(st.seconds, 3), "entries": st.n_entries,
+                            "fallback_vertices": st.n_fallback, "apply_nZ_15": rates,
+                            "remap_ice_model_ms": round((time.perf_counter() - tr) * 1e3, 3)}
+            log(f"[rows] remap: {res['remap']}")
+        finally:
+            ca.close()
+            cb.close()
+    step.reset()
+    return res
 
 
 def bench_precision(args, ctx, step, wl):
