@@ -71,7 +71,11 @@ def main():
     ap.add_argument("--check", type=int, default=256, help="residual checks of the plain SOR path")
     ap.add_argument("--max-sweeps", type=int, default=400000)
     ap.add_argument("--gmres", type=int, default=40, help="GMRES restart length")
-    ap.add_argument("--precond", type=int, default=5, help="SOR sweeps per preconditioner")
+    ap.add_argument("--precond", type=int, default=0,
+                    help="SOR sweeps per preconditioner application (sor: default 5) or smoothing sweeps "
+                         "before and after the coarse correction (multigrid: default 2)")
+    ap.add_argument("--pc", default="sor", choices=["sor", "multigrid"],
+                    help="preconditioner of the mixed_gmres path")
     ap.add_argument("--orth-passes", type=int, default=1, help="Gram-Schmidt passes (1 or 2)")
     ap.add_argument("--full-sweeps", action="store_true",
                     help="preconditioner sweeps repeat the colour at each turn of direction")
@@ -102,6 +106,9 @@ def main():
     ap.add_argument("--adj-fd", type=int, default=4, help="finite-difference directions to check")
     ap.add_argument("--adj-every", type=int, default=1, help="checkpoint every c-th Picard iterate")
     args = ap.parse_args()
+    args.pc_type = 1 if args.pc == "multigrid" else 0     # UFE_PRECOND_MULTIGRID / UFE_PRECOND_SOR
+    args.sor_sweeps = args.precond if (args.precond > 0 and args.pc == "sor") else 5
+    args.pc_sweeps = args.precond if args.precond > 0 else (2 if args.pc == "multigrid" else 5)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -162,12 +169,14 @@ def main():
         if path == "fp64_sor":      # the reference iteration: FP64 SOR sweeps, config omega
             return ctx.ssa_linear_solve(args.tol, args.max_sweeps, precision=abi.UFE_FP64,
                                         omega=omega, check_interval=args.check)
-        prec = abi.UFE_MIXED if path == "mixed_gmres" else abi.UFE_FP64
-        return ctx.ssa_linear_solve(args.tol, args.max_sweeps, precision=prec, omega=omega_mixed,
-                                    gmres=args.gmres, precond_sweeps=args.precond,
+        mixed = path == "mixed_gmres"
+        return ctx.ssa_linear_solve(args.tol, args.max_sweeps, precision=abi.UFE_MIXED if mixed else abi.UFE_FP64,
+                                    omega=omega_mixed, gmres=args.gmres,
+                                    precond_sweeps=args.pc_sweeps if mixed else args.sor_sweeps,
                                     orth_passes=args.orth_passes,
                                     method=1 if args.krylov == "bicgstab" else 0,
-                                    full_sweeps=args.full_sweeps)
+                                    full_sweeps=args.full_sweeps,
+                                    precond_type=args.pc_type if mixed else abi.UFE_PRECOND_SOR)
 
     def run(path, steps, warmup):
         for _ in range(warmup):
@@ -234,7 +243,8 @@ def main():
                         f"FP64 relative residual {args.tol:g}",
             "nV": sizes["nV"], "nVAaAc": nA, "mean_degree": round(sb["mean_degree"], 4),
             "path": head, "omega": omega, "omega_inner": omega_mixed,
-            "krylov": args.krylov, "gmres_restart": args.gmres, "precond_sweeps": args.precond,
+            "krylov": args.krylov, "gmres_restart": args.gmres, "preconditioner": args.pc,
+            "precond_sweeps": args.pc_sweeps,
             "orth_passes": args.orth_passes,
             "value_definition": "nVAaAc * (FP64 reference-SOR sweeps to the tolerance) / "
                                 "(this path's seconds to the tolerance) / 1e6",
@@ -250,6 +260,7 @@ def main():
         },
         "paths": paths,
         "sweep_tuning": tuning,
+        "multigrid": ctx.multigrid_info() if args.pc == "multigrid" else None,
         "reference_ssa_seconds": round(ref_ssa_s, 3),
     }
     if sor_ok:
@@ -414,8 +425,10 @@ def bench_precision(args, ctx, step, wl):
             step.reset()
             for f in ("U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac", "N_AaAc"):
                 ctx.fill(f, 0.0)
+            mixed = mode == abi.UFE_MIXED
             ctx.set_params(p, mode, linear_rel_tol=tol, gmres_restart=args.gmres,
-                           precond_sweeps=args.precond)
+                           precond_sweeps=args.pc_sweeps if mixed else args.sor_sweeps,
+                           precond_type=args.pc_type if mixed else abi.UFE_PRECOND_SOR)
             ctx.synchronize()
             ta = time.perf_counter()
             r = step.run()
@@ -539,7 +552,7 @@ def bench_batch(args, wls):
         ctx.synchronize()
         ta = time.perf_counter()
         st, rel = ctx.batch_solve_SSA(args.batch_picard, rel_tol=args.tol, precision=prec,
-                                      gmres_restart=args.gmres, precond_sweeps=args.precond, use_graph=graph)
+                                      gmres_restart=args.gmres, precond_sweeps=args.sor_sweeps, use_graph=graph)
         ctx.synchronize()
         el = time.perf_counter() - ta
         return st, rel, el, B.member_values("U_SSA_AaAc"), B.member_values("V_SSA_AaAc")

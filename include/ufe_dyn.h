@@ -39,6 +39,8 @@ enum {
 };
 
 enum { UFE_FP64 = 0, UFE_MIXED = 1, UFE_FP64_KRYLOV = 2 };
+/* preconditioner of the mixed-precision Krylov solves */
+enum { UFE_PRECOND_SOR = 0, UFE_PRECOND_MULTIGRID = 1 };
 
 /* Mesh in reference layout.  Primary arrays are inputs everywhere; the
  * secondary arrays are written by ufe_mesh_build and read by ufe_mesh_upload.
@@ -106,6 +108,11 @@ typedef struct ufe_params {
    * UFE_PHYS_BENCHMARK_FIXED (2): the fixed values of the EISMINT/Halfar/Bueler
    * benchmarks (:323-348). */
   int32_t physical_properties;
+  /* UFE_MIXED only: the preconditioner of the GMRES-IR linear solves,
+   * UFE_PRECOND_SOR (SSA_precond_sweeps SOR sweeps, default 5) or
+   * UFE_PRECOND_MULTIGRID (aggregation multigrid, SSA_precond_sweeps SOR
+   * smoothing sweeps before and after the coarse correction, default 2). */
+  int32_t SSA_precond_type;
 } ufe_params;
 
 #define UFE_PHYS_INPUT 0
@@ -200,6 +207,11 @@ typedef struct ufe_linsolve_opts {
   int32_t precond_neumann_every_sweep; /* 0: Neumann update of the edge rows after the last sweep of a
                                        preconditioner application only (the vector handed to the
                                        Krylov method satisfies it); 1: after every sweep */
+  int32_t precond_type;             /* UFE_PRECOND_SOR: precond_sweeps SOR sweeps from zero;
+                                       UFE_PRECOND_MULTIGRID (UFE_MIXED, a Krylov method): one
+                                       aggregation-multigrid K-cycle, precond_sweeps (<= 0: 2) FP32
+                                       SOR sweeps before and after the coarse correction; set up on
+                                       the first use per mesh (ufe_multigrid_info) */
 } ufe_linsolve_opts;
 
 /* Statistics of one linearised solve. */
@@ -325,6 +337,19 @@ int ufe_critical_timesteps(ufe_ctx *ctx, double *dt_crit);
 int ufe_ssa_prepare(ufe_ctx *ctx);
 /* Solve the linearised system from the current velocities. */
 int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *opts, ufe_linsolve_stats *st);
+
+/* The aggregation multigrid hierarchy of the uploaded mesh (built by the first
+ * UFE_PRECOND_MULTIGRID solve; levels = 0 before).  No counterpart in the
+ * reference, whose linearised SSA is solved by SOR alone. */
+typedef struct ufe_multigrid_info {
+  int32_t levels;                   /* including the fine level */
+  int32_t points[16];               /* points per level; [0]: the SOR rows */
+  int64_t couplings[16];            /* off-diagonal couplings per coarse level */
+  int32_t direct;                   /* the coarsest level is solved by a dense inverse */
+  int32_t kstep[16];                /* level solved by two cycles (K-cycle) */
+  double setup_seconds;
+} ufe_multigrid_info;
+int ufe_multigrid_info_get(ufe_ctx *ctx, ufe_multigrid_info *out);
 /* FP64 relative residual of the current velocities. */
 int ufe_ssa_residual(ufe_ctx *ctx, double *rel_residual);
 

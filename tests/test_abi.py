@@ -41,7 +41,8 @@ STRUCTS = {"ufe_mesh_desc": abi.MeshDesc, "ufe_params": abi.Params, "ufe_ssa_sta
            "ufe_adjoint_opts": abi.AdjointOpts, "ufe_adjoint_stats": abi.AdjointStats,
            "ufe_batch_opts": abi.BatchOpts, "ufe_batch_stats": abi.BatchStats,
            "ufe_sweep_tuning": abi.SweepTuning, "ufe_mesh_geometry_desc": abi.MeshGeometryDesc,
-           "ufe_mesh_geometry_stats": abi.MeshGeometryStats, "ufe_remap_stats": abi.RemapStats}
+           "ufe_mesh_geometry_stats": abi.MeshGeometryStats, "ufe_remap_stats": abi.RemapStats,
+           "ufe_multigrid_info": abi.MultigridInfo}
 
 
 def test_struct_layouts_match_header(tmp_path):

@@ -21,6 +21,7 @@ UFE_OK, UFE_SOR_NOT_CONVERGED, UFE_SSA_RESET, UFE_SSA_UNSTABLE = 0, 1, 2, 3
 UFE_COLOURING_IDENTIFY, UFE_COLOURING_FAILED, UFE_BAD_INPUT = 4, 5, 6
 UFE_HIP_ERROR, UFE_NOT_SUPPORTED = 7, 8
 UFE_FP64, UFE_MIXED, UFE_FP64_KRYLOV = 0, 1, 2
+UFE_PRECOND_SOR, UFE_PRECOND_MULTIGRID = 0, 1
 
 STATUS_NAMES = {0: "OK", 1: "SOR_NOT_CONVERGED", 2: "SSA_RESET", 3: "SSA_UNSTABLE",
                 4: "COLOURING_IDENTIFY", 5: "COLOURING_FAILED", 6: "BAD_INPUT",
@@ -31,6 +32,7 @@ EXPORTS = [
     "ufe_ctx_create", "ufe_ctx_destroy", "ufe_set_params", "ufe_last_error", "ufe_version",
     "ufe_mesh_build", "ufe_mesh_upload", "ufe_mesh_sizes", "ufe_mesh_geometry",
     "ufe_remap_build", "ufe_remap_field", "ufe_remap_ice_model", "ufe_remap_export",
+    "ufe_multigrid_info_get",
     "ufe_field_count", "ufe_field_name", "ufe_field_id", "ufe_field_info", "ufe_field_set",
     "ufe_field_get", "ufe_field_set_device", "ufe_field_get_device",
     "ufe_update_general_ice_model_data", "ufe_ice_physical_properties", "ufe_solve_SIA", "ufe_solve_SSA",
@@ -119,7 +121,8 @@ class Params(ct.Structure):
                 ("zero_Hi_at_edges", ct.c_int32), ("use_tau_c_override", ct.c_int32),
                 ("precision", ct.c_int32), ("sweep_batch", ct.c_int32),
                 ("SSA_linear_rel_tol", ct.c_double), ("SSA_gmres_restart", ct.c_int32),
-                ("SSA_precond_sweeps", ct.c_int32), ("physical_properties", ct.c_int32)]
+                ("SSA_precond_sweeps", ct.c_int32), ("physical_properties", ct.c_int32),
+                ("SSA_precond_type", ct.c_int32)]
 
 
 class SSAStats(ct.Structure):
@@ -134,7 +137,19 @@ class LinSolveOpts(ct.Structure):
                 ("omega", ct.c_double), ("check_interval", ct.c_int32), ("use_graph", ct.c_int32),
                 ("gmres_restart", ct.c_int32), ("precond_sweeps", ct.c_int32),
                 ("orth_passes", ct.c_int32), ("method", ct.c_int32),
-                ("precond_full_sweeps", ct.c_int32), ("precond_neumann_every_sweep", ct.c_int32)]
+                ("precond_full_sweeps", ct.c_int32), ("precond_neumann_every_sweep", ct.c_int32),
+                ("precond_type", ct.c_int32)]
+
+
+class MultigridInfo(ct.Structure):
+    _fields_ = [("levels", ct.c_int32), ("points", ct.c_int32 * 16), ("couplings", ct.c_int64 * 16),
+                ("direct", ct.c_int32), ("kstep", ct.c_int32 * 16), ("setup_seconds", ct.c_double)]
+
+    def as_dict(self):
+        L = self.levels
+        return {"levels": L, "points": list(self.points)[:L], "couplings": list(self.couplings)[:L],
+                "direct": bool(self.direct), "kstep": [bool(k) for k in list(self.kstep)[:L]],
+                "setup_seconds": self.setup_seconds}
 
 
 class KernelProfile(ct.Structure):
@@ -238,7 +253,7 @@ def _ptr(a):
 
 
 def params_struct(p: DynParams, precision=UFE_FP64, sweep_batch=0, linear_rel_tol=1e-6,
-                  gmres_restart=40, precond_sweeps=5) -> Params:
+                  gmres_restart=40, precond_sweeps=5, precond_type=0) -> Params:
     s = Params()
     s.nZ = p.nZ
     for k, z in enumerate(p.zeta):
@@ -257,6 +272,7 @@ def params_struct(p: DynParams, precision=UFE_FP64, sweep_batch=0, linear_rel_to
     s.SSA_gmres_restart = int(gmres_restart)
     s.SSA_precond_sweeps = int(precond_sweeps)
     s.physical_properties = int(p.physical_properties)
+    s.SSA_precond_type = int(precond_type)
     return s
 
 
@@ -322,7 +338,8 @@ class Context:
         return r
 
     def set_params(self, p: DynParams, precision=UFE_FP64, sweep_batch=0, **krylov):
-        """krylov: linear_rel_tol, gmres_restart, precond_sweeps (UFE_MIXED / UFE_FP64_KRYLOV)."""
+        """krylov: linear_rel_tol, gmres_restart, precond_sweeps, precond_type (UFE_MIXED /
+        UFE_FP64_KRYLOV; precond_type UFE_PRECOND_MULTIGRID with UFE_MIXED only)."""
         self._params = params_struct(p, precision, sweep_batch, **krylov)
         self._chk(self.lib.ufe_set_params(self.h, ct.byref(self._params)), "ufe_set_params")
 
@@ -476,15 +493,24 @@ class Context:
     def ssa_linear_solve(self, rel_tol, max_sweeps, precision=UFE_FP64, omega=0.0,
                          check_interval=16, gmres=0, precond_sweeps=2,
                          orth_passes=2, method=0, full_sweeps=False,
-                         neumann_every_sweep=False) -> LinSolveStats:
-        """method 0: GMRES(gmres) (plain SOR if gmres == 0); 1: BiCGStab."""
+                         neumann_every_sweep=False, precond_type=0) -> LinSolveStats:
+        """method 0: GMRES(gmres) (plain SOR if gmres == 0); 1: BiCGStab.
+        precond_type UFE_PRECOND_SOR: precond_sweeps SOR sweeps; UFE_PRECOND_MULTIGRID:
+        an aggregation-multigrid K-cycle with precond_sweeps smoothing sweeps (mixed only)."""
         o = LinSolveOpts(float(rel_tol), int(max_sweeps), int(precision), float(omega),
                          int(check_interval), 0, int(gmres), int(precond_sweeps), int(orth_passes),
-                         int(method), int(bool(full_sweeps)), int(bool(neumann_every_sweep)))
+                         int(method), int(bool(full_sweeps)), int(bool(neumann_every_sweep)),
+                         int(precond_type))
         st = LinSolveStats()
         self._chk(self.lib.ufe_ssa_linear_solve(self.h, ct.byref(o), ct.byref(st)),
                   "ssa_linear_solve")
         return st
+
+    def multigrid_info(self) -> dict:
+        """The multigrid hierarchy of the uploaded mesh (levels 0 before its first use)."""
+        o = MultigridInfo()
+        self._chk(self.lib.ufe_multigrid_info_get(self.h, ct.byref(o)), "multigrid_info")
+        return o.as_dict()
 
     def set_device(self, name, ptr):
         """Copy a device buffer (reference layout) into a field."""

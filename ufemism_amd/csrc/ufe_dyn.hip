@@ -114,6 +114,28 @@ struct DevBuf {
 
 using namespace ufe;
 
+// One coarse level of the aggregation multigrid preconditioner (device arrays;
+// allocated with the mesh).  Level 1 aggregates the SOR rows of the fine
+// operator, level l + 1 the points of level l.
+struct AmgLevel {
+  int n = 0;                        // points
+  int64_t nnz = 0;
+  int32_t *ptr = nullptr, *col = nullptr;   // off-diagonal couplings, CSR over points
+  float2 *cf = nullptr;
+  float *S = nullptr;               // u-v coupling of each point with itself
+  double2 *fd = nullptr;            // diagonal pair without the members' own diagonals (fixed per mesh)
+  double2 *ev64 = nullptr;          // diagonal pair of the current system
+  float2 *ev32 = nullptr;
+  int32_t *mptr = nullptr, *mem = nullptr;  // members: rows of the fine level (level 1) or points above
+  int32_t *agg = nullptr;           // row / point above -> this level's point (-1: not aggregated)
+  float2 *b = nullptr, *t = nullptr, *e1 = nullptr, *e2 = nullptr, *w1 = nullptr, *w2 = nullptr,
+         *rc2 = nullptr;
+  double *part = nullptr, *sc = nullptr;
+  double2 *cf64 = nullptr;          // coarsest level: FP64 couplings for the dense inverse
+  double *S64 = nullptr;
+  bool kstep = false;               // solved by two cycles with a GCR(2) combination
+};
+
 struct ufe_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -229,6 +251,12 @@ struct ufe_ctx {
   SorFlags *h_flags = nullptr;      // pinned
   double *h_scalar = nullptr;       // pinned
   std::vector<void *> allocs;       // everything to free with the mesh
+  // aggregation multigrid preconditioner (built on first use per mesh)
+  std::vector<AmgLevel> amg;        // [0] unused (the fine level), coarse levels from 1
+  bool amg_ready = false, amg_direct = false, pc_amg = false;
+  int amg_nu = 2;                   // smoothing sweeps before and after the coarse correction
+  double *amg_W = nullptr, *amg_inv = nullptr;
+  double amg_setup_seconds = 0.0;
 };
 
 #define HIPCHK(ctx, x)                                                            \
@@ -324,6 +352,9 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->bk_part = ctx->bk_H = ctx->bk_Hs = ctx->bk_y = ctx->bk_nrm = ctx->bk_scale = ctx->bk_rel = nullptr;
   if (ctx->bk_graph) (void)hipGraphExecDestroy(ctx->bk_graph);
   ctx->bk_graph = nullptr;
+  ctx->amg.clear();
+  ctx->amg_ready = ctx->amg_direct = false;
+  ctx->amg_W = ctx->amg_inv = nullptr;
 }
 
 static size_t grid_len(const ufe_ctx *ctx, int grid) {
@@ -515,6 +546,11 @@ int ufe_set_params(ufe_ctx *ctx, const ufe_params *p) {
   }
   if (p->SSA_gmres_restart > 64) {
     ctx->err = "SSA_gmres_restart must be at most 64";
+    return UFE_BAD_INPUT;
+  }
+  if (p->SSA_precond_type != UFE_PRECOND_SOR &&
+      !(p->SSA_precond_type == UFE_PRECOND_MULTIGRID && p->precision == UFE_MIXED)) {
+    ctx->err = "SSA_precond_type must be UFE_PRECOND_SOR, or UFE_PRECOND_MULTIGRID with precision UFE_MIXED";
     return UFE_BAD_INPUT;
   }
   ctx->params = *p;
@@ -1981,6 +2017,372 @@ static void enqueue_sweep_fp64_vec(ufe_ctx *ctx, const AaAcMesh &m, double omega
   k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), D);
 }
 
+// ---------------------------------------------------------------------------
+// Aggregation multigrid preconditioner for the FP32 correction solve: set-up
+// on the host once per mesh (the off-diagonal coefficients depend on the mesh
+// only), the diagonal pairs of every level rebuilt on the device with each
+// new system, and one K-cycle per application.
+// ---------------------------------------------------------------------------
+constexpr int AMG_KSTEP_MIN = 1000;      // levels with at least this many points take two cycles
+constexpr float AMG_BJ_OMEGA = 0.8f;     // damping of the coarse-level block Jacobi
+constexpr int AMG_COARSE_SWEEPS = 20;    // coarsest level too large for the dense inverse
+constexpr int AMG_MAX_LEVELS = 16;
+
+struct AmgHostOp {              // a point operator: off-diagonal pairs, fixed diagonal pairs, u-v coupling
+  int n = 0;
+  std::vector<int32_t> ptr, col;
+  std::vector<double2> cf, fd;
+  std::vector<double> S;
+};
+
+// Greedy aggregation: a point whose neighbours are all free takes them into a
+// new aggregate; every point left joins the aggregate of its first aggregated
+// neighbour (or forms its own).
+static std::vector<int32_t> amg_aggregate(const AmgHostOp &A, int *na) {
+  std::vector<int32_t> agg(A.n, -1);
+  int k = 0;
+  for (int i = 0; i < A.n; i++) {
+    if (agg[i] >= 0) continue;
+    bool free = true;
+    for (int q = A.ptr[i]; q < A.ptr[i + 1] && free; q++) free = agg[A.col[q]] < 0;
+    if (!free) continue;
+    agg[i] = k;
+    for (int q = A.ptr[i]; q < A.ptr[i + 1]; q++) agg[A.col[q]] = k;
+    k++;
+  }
+  for (int i = 0; i < A.n; i++) {
+    if (agg[i] >= 0) continue;
+    int a = -1;
+    for (int q = A.ptr[i]; q < A.ptr[i + 1] && a < 0; q++) a = agg[A.col[q]];
+    agg[i] = a >= 0 ? a : k++;
+  }
+  *na = k;
+  return agg;
+}
+
+// Galerkin product P^T A P for piecewise-constant P, and the member lists
+static void amg_galerkin(const AmgHostOp &A, const std::vector<int32_t> &agg, int na, AmgHostOp &C,
+                         std::vector<int32_t> &mptr, std::vector<int32_t> &mem) {
+  mptr.assign(na + 1, 0);
+  for (int i = 0; i < A.n; i++) mptr[agg[i] + 1]++;
+  for (int I = 0; I < na; I++) mptr[I + 1] += mptr[I];
+  mem.resize(A.n);
+  {
+    std::vector<int32_t> cur(mptr.begin(), mptr.end() - 1);
+    for (int i = 0; i < A.n; i++) mem[cur[agg[i]]++] = i;
+  }
+  C.n = na;
+  C.ptr.assign(1, 0);
+  C.col.clear();
+  C.cf.clear();
+  C.fd.assign(na, make_double2(0.0, 0.0));
+  C.S.assign(na, 0.0);
+  std::vector<int64_t> mark(na, -1);
+  for (int I = 0; I < na; I++) {
+    const int64_t row0 = (int64_t)C.col.size();
+    for (int k = mptr[I]; k < mptr[I + 1]; k++) {
+      const int i = mem[k];
+      C.fd[I].x += A.fd[i].x;
+      C.fd[I].y += A.fd[i].y;
+      C.S[I] += A.S[i];
+      for (int q = A.ptr[i]; q < A.ptr[i + 1]; q++) {
+        const int J = agg[A.col[q]];
+        const double2 c = A.cf[q];
+        if (J == I) {
+          C.fd[I].x += c.x;
+          C.fd[I].y += c.y;
+        } else if (mark[J] >= row0) {
+          C.cf[mark[J]].x += c.x;
+          C.cf[mark[J]].y += c.y;
+        } else {
+          mark[J] = (int64_t)C.col.size();
+          C.col.push_back(J);
+          C.cf.push_back(c);
+        }
+      }
+    }
+    C.ptr.push_back((int32_t)C.col.size());
+  }
+}
+
+static int amg_setup(ufe_ctx *ctx) {
+  if (ctx->amg_ready) return UFE_OK;
+  const auto t0 = std::chrono::steady_clock::now();
+  const int nA = ctx->nA, nchunk = (nA + SELL - 1) / SELL;
+  int r;
+  // the fine operator, read back from the device
+  std::vector<int64_t> cptr(nchunk + 1);
+  std::vector<uint8_t> deg(nA);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(cptr.data(), ctx->d_chunk_ptr, sizeof(int64_t) * (nchunk + 1), hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(deg.data(), ctx->d_deg, nA, hipMemcpyDeviceToHost));
+  const int64_t ne = cptr[nchunk];
+  std::vector<int32_t> col(ne);
+  std::vector<double2> cU(ne);
+  std::vector<double> Sxy(nA);
+  HIPCHK(ctx, hipMemcpy(col.data(), ctx->d_col, sizeof(int32_t) * ne, hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(cU.data(), ctx->d_cU, sizeof(double2) * ne, hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(Sxy.data(), ctx->d_Sxy, sizeof(double) * nA, hipMemcpyDeviceToHost));
+  auto get_list = [&](int n, const int32_t *drow, const int32_t *dptr, const int32_t *dcol,
+                      std::vector<int32_t> &row, std::vector<int32_t> &ptr, std::vector<int32_t> &cl) -> int {
+    row.resize(n);
+    ptr.assign(n + 1, 0);
+    if (n == 0) return UFE_OK;
+    HIPCHK(ctx, hipMemcpy(row.data(), drow, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(ptr.data(), dptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
+    cl.resize(ptr[n]);
+    HIPCHK(ctx, hipMemcpy(cl.data(), dcol, sizeof(int32_t) * ptr[n], hipMemcpyDeviceToHost));
+    return UFE_OK;
+  };
+  std::vector<int32_t> nrow, nptr, ncol, crow, cptr2, ccol;
+  if ((r = get_list(ctx->n_neu, ctx->d_neu_row, ctx->d_neu_ptr, ctx->d_neu_col, nrow, nptr, ncol))) return r;
+  if ((r = get_list(ctx->n_cor, ctx->d_cor_row, ctx->d_cor_ptr, ctx->d_cor_col, crow, cptr2, ccol))) return r;
+  // fine points: the SOR rows in colour order
+  std::vector<int32_t> pid(nA, -1), rows;
+  for (int c = 0; c < 5; c++)
+    for (int p = ctx->colour_start[c]; p < ctx->colour_start[c] + ctx->colour_active[c]; p++) {
+      pid[p] = (int32_t)rows.size();
+      rows.push_back(p);
+    }
+  const int n0 = (int)rows.size();
+  if (n0 == 0) {
+    ctx->err = "no SOR rows to aggregate";
+    return UFE_BAD_INPUT;
+  }
+  // the Neumann rows as weighted sums of fine points: edges average SOR rows,
+  // corners average their neighbours (edge rows expanded)
+  std::vector<int32_t> exi(nA, -1);
+  std::vector<std::vector<std::pair<int32_t, double>>> exl;
+  for (int i = 0; i < ctx->n_neu; i++) {
+    const double w = 1.0 / (double)(nptr[i + 1] - nptr[i]);
+    std::vector<std::pair<int32_t, double>> e;
+    for (int k = nptr[i]; k < nptr[i + 1]; k++)
+      if (pid[ncol[k]] >= 0) e.push_back({pid[ncol[k]], w});
+    exi[nrow[i]] = (int32_t)exl.size();
+    exl.push_back(std::move(e));
+  }
+  for (int i = 0; i < ctx->n_cor; i++) {
+    const double w = 1.0 / (double)(cptr2[i + 1] - cptr2[i]);
+    std::vector<std::pair<int32_t, double>> e;
+    for (int k = cptr2[i]; k < cptr2[i + 1]; k++) {
+      const int q = ccol[k];
+      if (pid[q] >= 0) e.push_back({pid[q], w});
+      else if (exi[q] >= 0)
+        for (auto &pr : exl[exi[q]]) e.push_back({pr.first, pr.second * w});
+    }
+    exi[crow[i]] = (int32_t)exl.size();
+    exl.push_back(std::move(e));
+  }
+  // the operator on the fine points with the Neumann rows eliminated
+  std::vector<AmgHostOp> ops(1);
+  {
+    AmgHostOp &A = ops[0];
+    A.n = n0;
+    A.ptr.assign(1, 0);
+    A.fd.assign(n0, make_double2(0.0, 0.0));
+    A.S.resize(n0);
+    std::vector<int64_t> mark(n0, -1);
+    for (int i = 0; i < n0; i++) {
+      const int p = rows[i];
+      const int64_t base = cptr[p / SELL] + (p % SELL), row0 = (int64_t)A.col.size();
+      auto add = [&](int j, double cx, double cy) {
+        if (j == i) {
+          A.fd[i].x += cx;
+          A.fd[i].y += cy;
+        } else if (mark[j] >= row0) {
+          A.cf[mark[j]].x += cx;
+          A.cf[mark[j]].y += cy;
+        } else {
+          mark[j] = (int64_t)A.col.size();
+          A.col.push_back(j);
+          A.cf.push_back(make_double2(cx, cy));
+        }
+      };
+      for (int s = 0; s < deg[p]; s++) {
+        const int64_t e = base + (int64_t)s * SELL;
+        const int q = col[e];
+        const double2 c = cU[e];
+        if (pid[q] >= 0) add(pid[q], c.x, c.y);
+        else if (exi[q] >= 0)
+          for (auto &pr : exl[exi[q]]) add(pr.first, c.x * pr.second, c.y * pr.second);
+      }
+      A.ptr.push_back((int32_t)A.col.size());
+      A.S[i] = 3.0 * Sxy[p];
+    }
+  }
+  // coarsen until the dense inverse fits (at least one coarse level)
+  std::vector<std::vector<int32_t>> aggs, mptrs, mems;
+  while ((int)ops.size() < AMG_MAX_LEVELS && (ops.size() == 1 || ops.back().n > AMG_DIRECT)) {
+    int na = 0;
+    std::vector<int32_t> agg = amg_aggregate(ops.back(), &na);
+    if (ops.size() > 1 && na > (int)(0.85 * ops.back().n)) break;   // coarsening stalled
+    AmgHostOp C;
+    std::vector<int32_t> mptr, mem;
+    amg_galerkin(ops.back(), agg, na, C, mptr, mem);
+    aggs.push_back(std::move(agg));
+    mptrs.push_back(std::move(mptr));
+    mems.push_back(std::move(mem));
+    ops.push_back(std::move(C));
+  }
+  const int L = (int)ops.size();
+  ctx->amg.assign(L, AmgLevel{});
+  ctx->amg[0].n = n0;
+  for (int l = 1; l < L; l++) {
+    AmgLevel &V = ctx->amg[l];
+    const AmgHostOp &C = ops[l];
+    V.n = C.n;
+    V.nnz = (int64_t)C.col.size();
+    V.kstep = l < L - 1 && C.n >= AMG_KSTEP_MIN;
+    std::vector<float2> cf32(std::max<size_t>(C.cf.size(), 1));
+    for (size_t k = 0; k < C.cf.size(); k++) cf32[k] = make_float2((float)C.cf[k].x, (float)C.cf[k].y);
+    std::vector<float> S32(C.n);
+    for (int i = 0; i < C.n; i++) S32[i] = (float)C.S[i];
+    std::vector<int32_t> colv = C.col;
+    if (colv.empty()) colv.push_back(0);
+    if ((r = dupload(ctx, &V.ptr, C.ptr.data(), C.ptr.size()))) return r;
+    if ((r = dupload(ctx, &V.col, colv.data(), colv.size()))) return r;
+    if ((r = dupload(ctx, &V.cf, cf32.data(), cf32.size()))) return r;
+    if ((r = dupload(ctx, &V.S, S32.data(), S32.size()))) return r;
+    if ((r = dupload(ctx, &V.fd, C.fd.data(), C.fd.size()))) return r;
+    if ((r = dalloc(ctx, &V.ev64, (size_t)C.n))) return r;
+    if ((r = dalloc(ctx, &V.ev32, (size_t)C.n))) return r;
+    // members and aggregate map: level 1 in fine rows, deeper levels in points
+    std::vector<int32_t> mem = mems[l - 1];
+    std::vector<int32_t> agg;
+    if (l == 1) {
+      for (auto &i : mem) i = rows[i];
+      agg.assign(nA, -1);
+      for (int i = 0; i < n0; i++) agg[rows[i]] = aggs[0][i];
+    } else {
+      agg = aggs[l - 1];
+    }
+    if ((r = dupload(ctx, &V.mptr, mptrs[l - 1].data(), mptrs[l - 1].size()))) return r;
+    if ((r = dupload(ctx, &V.mem, mem.data(), mem.size()))) return r;
+    if ((r = dupload(ctx, &V.agg, agg.data(), agg.size()))) return r;
+    float2 **vecs[7] = {&V.b, &V.t, &V.e1, &V.e2, &V.w1, &V.w2, &V.rc2};
+    for (float2 **v : vecs)
+      if ((r = dalloc(ctx, v, (size_t)C.n))) return r;
+    if ((r = dalloc(ctx, &V.part, (size_t)3 * AMG_DOT_BLOCKS))) return r;
+    if ((r = dalloc(ctx, &V.sc, (size_t)8))) return r;
+    HIPCHK(ctx, hipMemset(V.sc, 0, 8 * sizeof(double)));
+  }
+  AmgLevel &Lc = ctx->amg[L - 1];
+  ctx->amg_direct = Lc.n <= AMG_DIRECT;
+  if (ctx->amg_direct) {
+    const AmgHostOp &C = ops[L - 1];
+    std::vector<double2> cf = C.cf;
+    if (cf.empty()) cf.push_back(make_double2(0.0, 0.0));
+    if ((r = dupload(ctx, &Lc.cf64, cf.data(), cf.size()))) return r;
+    if ((r = dupload(ctx, &Lc.S64, C.S.data(), C.S.size()))) return r;
+    const size_t N = 2 * (size_t)Lc.n;
+    if ((r = dalloc(ctx, &ctx->amg_W, N * 2 * N))) return r;
+    if ((r = dalloc(ctx, &ctx->amg_inv, N * N))) return r;
+  }
+  HIPCHK(ctx, hipDeviceSynchronize());
+  ctx->amg_ready = true;
+  ctx->amg_setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return UFE_OK;
+}
+
+static AmgOp amg_op(const AmgLevel &V) {
+  AmgOp A;
+  A.n = V.n;
+  A.ptr = V.ptr;
+  A.col = V.col;
+  A.cf = V.cf;
+  A.S = V.S;
+  A.ev = V.ev32;
+  return A;
+}
+
+// the diagonal pairs of every level for the current system (FP64 EV in d_pair[1])
+static void amg_refresh(ufe_ctx *ctx) {
+  const int L = (int)ctx->amg.size();
+  for (int l = 1; l < L; l++) {
+    AmgLevel &V = ctx->amg[l];
+    const double2 *evf = l == 1 ? ctx->d_pair[1] : ctx->amg[l - 1].ev64;
+    k_amg_ev<<<nblk(V.n), BLOCK, 0, ctx->stream>>>(V.n, V.mptr, V.mem, evf, V.fd, V.ev64, V.ev32);
+  }
+  if (ctx->amg_direct) {
+    AmgLevel &V = ctx->amg[L - 1];
+    k_amg_dense_inv<<<1, 256, 0, ctx->stream>>>(V.n, V.ptr, V.col, V.cf64, V.S64, V.ev64, ctx->amg_W, ctx->amg_inv);
+  }
+}
+
+static void amg_solve_level(ufe_ctx *ctx, int l);
+
+// x ~ A_l^-1 b: nu block-Jacobi sweeps, the coarse correction, nu sweeps.  The
+// 2 nu sweeps alternate between t and x, starting in t, so they end in x.
+static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
+  AmgLevel &V = ctx->amg[l];
+  const AmgOp A = amg_op(V);
+  const int nb = nblk(V.n), last = (int)ctx->amg.size() - 1;
+  if (l == last) {
+    if (ctx->amg_direct) {
+      k_amg_dense_apply<<<1, 2 * AMG_DIRECT, 0, ctx->stream>>>(V.n, ctx->amg_inv, b, x);
+      return;
+    }
+    k_amg_bj<<<nb, BLOCK, 0, ctx->stream>>>(A, b, nullptr, V.t, AMG_BJ_OMEGA);
+    float2 *src = V.t, *dst = x;
+    for (int s = 1; s < 2 * AMG_COARSE_SWEEPS; s++) {
+      k_amg_bj<<<nb, BLOCK, 0, ctx->stream>>>(A, b, src, dst, AMG_BJ_OMEGA);
+      std::swap(src, dst);
+    }
+    return;
+  }
+  const int nu = std::max(1, ctx->amg_nu);
+  float2 *cur = V.t, *oth = x;
+  k_amg_bj<<<nb, BLOCK, 0, ctx->stream>>>(A, b, nullptr, cur, AMG_BJ_OMEGA);
+  for (int s = 1; s < nu; s++) {
+    k_amg_bj<<<nb, BLOCK, 0, ctx->stream>>>(A, b, cur, oth, AMG_BJ_OMEGA);
+    std::swap(cur, oth);
+  }
+  AmgLevel &C = ctx->amg[l + 1];
+  k_amg_restrict<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(A, C.n, C.mptr, C.mem, cur, b, C.b);
+  amg_solve_level(ctx, l + 1);
+  k_amg_prolong<<<nb, BLOCK, 0, ctx->stream>>>(V.n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, cur);
+  for (int s = 0; s < nu; s++) {
+    k_amg_bj<<<nb, BLOCK, 0, ctx->stream>>>(A, b, cur, oth, AMG_BJ_OMEGA);
+    std::swap(cur, oth);
+  }
+}
+
+// A_l e = b_l: one cycle into e1, or (kstep) two cycles combined by GCR(2),
+// the coefficients left in sc[6], sc[7] for the prolongation
+static void amg_solve_level(ufe_ctx *ctx, int l) {
+  AmgLevel &V = ctx->amg[l];
+  amg_cycle(ctx, l, V.b, V.e1);
+  if (!V.kstep) return;
+  const AmgOp A = amg_op(V);
+  const int nd = std::min(AMG_DOT_BLOCKS, nblk(V.n, 256));
+  k_amg_matvec_dot<<<nd, 256, 0, ctx->stream>>>(A, V.e1, V.w1, V.b, nullptr, V.part);
+  k_amg_kfin<<<1, 256, 0, ctx->stream>>>(nd, V.part, V.sc, 1);
+  k_amg_axpy<<<nblk(V.n), BLOCK, 0, ctx->stream>>>(V.n, V.b, V.w1, V.sc, V.rc2);
+  amg_cycle(ctx, l, V.rc2, V.e2);
+  k_amg_matvec_dot<<<nd, 256, 0, ctx->stream>>>(A, V.e2, V.w2, V.w1, V.rc2, V.part);
+  k_amg_kfin<<<1, 256, 0, ctx->stream>>>(nd, V.part, V.sc, 2);
+}
+
+// z ~ A^-1 v on the fine level: nu forward FP32 SOR sweeps from zero, the
+// coarse correction of the fine residual, nu backward sweeps; z leaves with
+// the Neumann condition on the edge rows, like the SOR preconditioner's.
+static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float2 *v, float2 *z) {
+  const int n = ctx->nA, nu = std::max(1, ctx->amg_nu);
+  k_fill<float2><<<nblk(n), BLOCK, 0, ctx->stream>>>(z, n, make_float2(0.0f, 0.0f));
+  for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, false, false, true);
+  AmgLevel &C = ctx->amg[1];
+  k_amg_restrict_fine<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32, C.n, C.mptr,
+                                                             C.mem, z, v, C.b);
+  amg_solve_level(ctx, 1);
+  k_amg_prolong<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, z);
+  if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), z);
+  k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), z);
+  for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, true, false, true);
+}
+
+// colour passes of one multigrid application (for the sweep count)
+static int64_t amg_passes(const ufe_ctx *ctx) { return 10 * (int64_t)std::max(1, ctx->amg_nu); }
+
 // solve_SSA_linearised sweep loop (:601-691): returns the flags at exit.
 static int run_sor_reference(ufe_ctx *ctx, double tol, int max_sweeps, SorFlags *out, double *seconds) {
   const AaAcMesh m = aaac_mesh(ctx);
@@ -2121,7 +2523,9 @@ int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) {
       lo.max_sweeps = 1 << 30;
       lo.precision = P.precision == UFE_MIXED ? UFE_MIXED : UFE_FP64;
       lo.gmres_restart = P.SSA_gmres_restart > 0 ? P.SSA_gmres_restart : 40;
-      lo.precond_sweeps = P.SSA_precond_sweeps > 0 ? P.SSA_precond_sweeps : 5;
+      lo.precond_type = P.SSA_precond_type;
+      lo.precond_sweeps = P.SSA_precond_sweeps > 0 ? P.SSA_precond_sweeps
+                                                   : (P.SSA_precond_type == UFE_PRECOND_MULTIGRID ? 2 : 5);
       lo.orth_passes = 1;
       ufe_linsolve_stats ls;
       if ((r = ufe_ssa_linear_solve(ctx, &lo, &ls))) return r;
@@ -2306,14 +2710,19 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
   }
   for (int j = 0; j < m; j++) {
     T2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;
-    k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
-    for (int k = 0; k < ksweeps; k++) {
-      const bool skip = k > 0 && !ctx->repeat_turn;
-      const bool neu = !ctx->neumann_last || k == ksweeps - 1;
-      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1, skip, neu);
-      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1, skip, neu);
+    if (f32 && ctx->pc_amg) {
+      amg_apply(ctx, am, (float)omega, (const float2 *)vj, (float2 *)zj);
+      *cpass += amg_passes(ctx);
+    } else {
+      k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
+      for (int k = 0; k < ksweeps; k++) {
+        const bool skip = k > 0 && !ctx->repeat_turn;
+        const bool neu = !ctx->neumann_last || k == ksweeps - 1;
+        if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1, skip, neu);
+        else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1, skip, neu);
+      }
+      *cpass += precond_passes(ctx, ksweeps);
     }
-    *cpass += precond_passes(ctx, ksweeps);
     if (f32)
       k_matvec<float><<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32, ar, (const float2 *)zj, (float2 *)w);
     else
@@ -2383,6 +2792,10 @@ static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max
     k_multi_dot_reduce<<<k, 256, 0, ctx->stream>>>(DOT_BLOCKS, k, ctx->kry_part, out, 0);
   };
   auto precond = [&](const T2 *in, T2 *out) {
+    if (f32 && ctx->pc_amg) {
+      amg_apply(ctx, am, (float)omega, (const float2 *)in, (float2 *)out);
+      return;
+    }
     k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(out, n, T2{T(0), T(0)});
     for (int k = 0; k < ks; k++) {
       const bool skip = k > 0 && !ctx->repeat_turn;
@@ -2423,7 +2836,7 @@ static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max
     k_bicg_update<T><<<nb, BLOCK, 0, ctx->stream>>>(n, ph, sh, s, t, x, r, S);
     dot(rh, 2, r, S + 1);                     // new rho and |r|^2: r^ and r are adjacent
     LAUNCHCHK(ctx);
-    *cpass += 2 * precond_passes(ctx, ks);
+    *cpass += 2 * (ctx->pc_amg ? amg_passes(ctx) : precond_passes(ctx, ks));
     *matvecs += 2;
     if (++it % 2 == 0) {
       HIPCHK(ctx, hipMemcpyAsync(h, S + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -2433,6 +2846,22 @@ static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max
   }
   k_add_to_w<T><<<nb, BLOCK, 0, ctx->stream>>>(n, x, ctx->d_pair[0]);
   LAUNCHCHK(ctx);
+  return UFE_OK;
+}
+
+int ufe_multigrid_info_get(ufe_ctx *ctx, ufe_multigrid_info *out) {
+  if (!ctx || !out) return UFE_BAD_INPUT;
+  std::memset(out, 0, sizeof(*out));
+  if (!ctx->amg_ready) return UFE_OK;
+  const int L = std::min<int>((int)ctx->amg.size(), 16);
+  out->levels = L;
+  for (int l = 0; l < L; l++) {
+    out->points[l] = ctx->amg[l].n;
+    out->couplings[l] = ctx->amg[l].nnz;
+    out->kstep[l] = ctx->amg[l].kstep ? 1 : 0;
+  }
+  out->direct = ctx->amg_direct ? 1 : 0;
+  out->setup_seconds = ctx->amg_setup_seconds;
   return UFE_OK;
 }
 
@@ -2448,14 +2877,30 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
   auto t0 = std::chrono::steady_clock::now();
   double rel = 0.0;
   int r;
-  if (mixed) {  // FP32 centre coefficients of the current system
-    k_to_f32_pair<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_pair[1], ctx->d_EV32);
-    LAUNCHCHK(ctx);
-  }
   const int mr = o->gmres_restart;
   if (mr < 0 || mr > 64) {
     ctx->err = "gmres_restart must be in 0..64";
     return UFE_BAD_INPUT;
+  }
+  if (o->precond_type != UFE_PRECOND_SOR && o->precond_type != UFE_PRECOND_MULTIGRID) {
+    ctx->err = "precond_type must be UFE_PRECOND_SOR or UFE_PRECOND_MULTIGRID";
+    return UFE_BAD_INPUT;
+  }
+  ctx->pc_amg = o->precond_type == UFE_PRECOND_MULTIGRID;
+  if (ctx->pc_amg && (!mixed || (o->method == 0 && mr == 0))) {
+    ctx->pc_amg = false;
+    ctx->err = "the multigrid preconditioner needs precision UFE_MIXED and a Krylov method";
+    return UFE_BAD_INPUT;
+  }
+  ctx->amg_nu = o->precond_sweeps > 0 ? o->precond_sweeps : 2;
+  if (mixed) {  // FP32 centre coefficients of the current system
+    k_to_f32_pair<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_pair[1], ctx->d_EV32);
+    LAUNCHCHK(ctx);
+  }
+  if (ctx->pc_amg) {
+    if ((r = amg_setup(ctx))) return r;
+    amg_refresh(ctx);
+    LAUNCHCHK(ctx);
   }
   if (o->method == 1) {
     // BiCGStab-IR: FP64 residual, inner BiCGStab in FP32 (mixed) or FP64

@@ -43,6 +43,8 @@ MODULE ice_dynamics_module
     REAL(dp) :: SSA_linear_rel_tol = 1E-6_dp
     INTEGER  :: SSA_gmres_restart = 40
     INTEGER  :: SSA_precond_sweeps = 5
+    ! preconditioner of those solves (UFE_MIXED): 0 SOR sweeps, 1 aggregation multigrid
+    INTEGER  :: SSA_precond_type = 0
     ! ice_physical_properties at the end of update_general_ice_model_data:
     ! 0 the A_flow fields are inputs, 1 Arrhenius law from Ti, 2 fixed benchmark values
     INTEGER  :: physical_properties = 0
@@ -194,6 +196,7 @@ CONTAINS
     p%SSA_gmres_restart = ncfg%SSA_gmres_restart
     p%SSA_precond_sweeps = ncfg%SSA_precond_sweeps
     p%physical_properties = ncfg%physical_properties
+    p%SSA_precond_type = ncfg%SSA_precond_type
   END FUNCTION params_of
 
   ! ---- mesh set-up -----------------------------------------------------------

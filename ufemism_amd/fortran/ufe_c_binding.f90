@@ -52,6 +52,7 @@ MODULE ufe_c_binding
     REAL(C_DOUBLE)     :: SSA_linear_rel_tol
     INTEGER(C_INT32_T) :: SSA_gmres_restart, SSA_precond_sweeps
     INTEGER(C_INT32_T) :: physical_properties
+    INTEGER(C_INT32_T) :: SSA_precond_type
   END TYPE ufe_params
 
   TYPE, BIND(C) :: ufe_ssa_stats
