@@ -123,13 +123,13 @@ struct AmgLevel {
   int32_t *ptr = nullptr, *col = nullptr;   // off-diagonal couplings, CSR over points
   float2 *cf = nullptr;
   float *S = nullptr;               // u-v coupling of each point with itself
-  double2 *fd = nullptr;            // diagonal pair without the members' own diagonals (fixed per mesh)
+  double2 *fd = nullptr;            // diagonal pair added to the members' diagonals (fixed per mesh)
   double2 *ev64 = nullptr;          // diagonal pair of the current system
   float2 *ev32 = nullptr;
   int32_t *mptr = nullptr, *mem = nullptr;  // members: rows of the fine level (level 1) or points above
   int32_t *agg = nullptr;           // row / point above -> this level's point (-1: not aggregated)
   float2 *b = nullptr, *t = nullptr, *e1 = nullptr, *e2 = nullptr, *w1 = nullptr, *w2 = nullptr,
-         *rc2 = nullptr;
+         *rc2 = nullptr, *r = nullptr;
   double *part = nullptr, *sc = nullptr;
   double2 *cf64 = nullptr;          // coarsest level: FP64 couplings for the dense inverse
   double *S64 = nullptr;
@@ -256,6 +256,10 @@ struct ufe_ctx {
   bool amg_ready = false, amg_direct = false, pc_amg = false;
   int amg_nu = 2;                   // smoothing sweeps before and after the coarse correction
   double *amg_W = nullptr, *amg_inv = nullptr;
+  float2 *amg_y = nullptr;          // fine-level A z of the restriction
+  float2 *amg_v32 = nullptr, *amg_z32 = nullptr;   // FP32 copies around an FP64 Krylov method
+  hipGraphExec_t amg_graph = nullptr;   // the coarse levels of one application, captured
+  int amg_graph_nu = 0;
   double amg_setup_seconds = 0.0;
 };
 
@@ -355,6 +359,10 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->amg.clear();
   ctx->amg_ready = ctx->amg_direct = false;
   ctx->amg_W = ctx->amg_inv = nullptr;
+  ctx->amg_y = ctx->amg_v32 = ctx->amg_z32 = nullptr;
+  if (ctx->amg_graph) (void)hipGraphExecDestroy(ctx->amg_graph);
+  ctx->amg_graph = nullptr;
+  ctx->amg_graph_nu = 0;
 }
 
 static size_t grid_len(const ufe_ctx *ctx, int grid) {
@@ -2023,6 +2031,8 @@ static void enqueue_sweep_fp64_vec(ufe_ctx *ctx, const AaAcMesh &m, double omega
 // only), the diagonal pairs of every level rebuilt on the device with each
 // new system, and one K-cycle per application.
 // ---------------------------------------------------------------------------
+static ActiveRanges active_ranges(ufe_ctx *ctx);
+
 constexpr int AMG_KSTEP_MIN = 1000;      // levels with at least this many points take two cycles
 constexpr float AMG_BJ_OMEGA = 0.8f;     // damping of the coarse-level block Jacobi
 constexpr int AMG_COARSE_SWEEPS = 20;    // coarsest level too large for the dense inverse
@@ -2060,9 +2070,12 @@ static std::vector<int32_t> amg_aggregate(const AmgHostOp &A, int *na) {
   return agg;
 }
 
-// Galerkin product P^T A P for piecewise-constant P, and the member lists
+// Galerkin product P^T A P for piecewise-constant P, and the member lists.
+// C.fd holds the whole fixed diagonal part (the members' fixed parts plus the
+// couplings inside each aggregate); intra the couplings inside alone, which is
+// what a level adds to the sum of its members' full diagonals.
 static void amg_galerkin(const AmgHostOp &A, const std::vector<int32_t> &agg, int na, AmgHostOp &C,
-                         std::vector<int32_t> &mptr, std::vector<int32_t> &mem) {
+                         std::vector<int32_t> &mptr, std::vector<int32_t> &mem, std::vector<double2> &intra) {
   mptr.assign(na + 1, 0);
   for (int i = 0; i < A.n; i++) mptr[agg[i] + 1]++;
   for (int I = 0; I < na; I++) mptr[I + 1] += mptr[I];
@@ -2077,6 +2090,7 @@ static void amg_galerkin(const AmgHostOp &A, const std::vector<int32_t> &agg, in
   C.cf.clear();
   C.fd.assign(na, make_double2(0.0, 0.0));
   C.S.assign(na, 0.0);
+  intra.assign(na, make_double2(0.0, 0.0));
   std::vector<int64_t> mark(na, -1);
   for (int I = 0; I < na; I++) {
     const int64_t row0 = (int64_t)C.col.size();
@@ -2091,6 +2105,8 @@ static void amg_galerkin(const AmgHostOp &A, const std::vector<int32_t> &agg, in
         if (J == I) {
           C.fd[I].x += c.x;
           C.fd[I].y += c.y;
+          intra[I].x += c.x;
+          intra[I].y += c.y;
         } else if (mark[J] >= row0) {
           C.cf[mark[J]].x += c.x;
           C.cf[mark[J]].y += c.y;
@@ -2212,13 +2228,16 @@ static int amg_setup(ufe_ctx *ctx) {
   }
   // coarsen until the dense inverse fits (at least one coarse level)
   std::vector<std::vector<int32_t>> aggs, mptrs, mems;
+  std::vector<std::vector<double2>> intras;
   while ((int)ops.size() < AMG_MAX_LEVELS && (ops.size() == 1 || ops.back().n > AMG_DIRECT)) {
     int na = 0;
     std::vector<int32_t> agg = amg_aggregate(ops.back(), &na);
     if (ops.size() > 1 && na > (int)(0.85 * ops.back().n)) break;   // coarsening stalled
     AmgHostOp C;
     std::vector<int32_t> mptr, mem;
-    amg_galerkin(ops.back(), agg, na, C, mptr, mem);
+    std::vector<double2> intra;
+    amg_galerkin(ops.back(), agg, na, C, mptr, mem, intra);
+    intras.push_back(std::move(intra));
     aggs.push_back(std::move(agg));
     mptrs.push_back(std::move(mptr));
     mems.push_back(std::move(mem));
@@ -2243,7 +2262,12 @@ static int amg_setup(ufe_ctx *ctx) {
     if ((r = dupload(ctx, &V.col, colv.data(), colv.size()))) return r;
     if ((r = dupload(ctx, &V.cf, cf32.data(), cf32.size()))) return r;
     if ((r = dupload(ctx, &V.S, S32.data(), S32.size()))) return r;
-    if ((r = dupload(ctx, &V.fd, C.fd.data(), C.fd.size()))) return r;
+    // the part of the diagonal that does not come from the level above: at level
+    // 1 the fine fixed parts (Neumann self-couplings) as well, since the fine
+    // diagonal pairs EV do not hold them; below, the couplings inside each
+    // aggregate only (the members' diagonals already hold their fixed parts)
+    const std::vector<double2> &own = l == 1 ? C.fd : intras[l - 1];
+    if ((r = dupload(ctx, &V.fd, own.data(), own.size()))) return r;
     if ((r = dalloc(ctx, &V.ev64, (size_t)C.n))) return r;
     if ((r = dalloc(ctx, &V.ev32, (size_t)C.n))) return r;
     // members and aggregate map: level 1 in fine rows, deeper levels in points
@@ -2259,13 +2283,16 @@ static int amg_setup(ufe_ctx *ctx) {
     if ((r = dupload(ctx, &V.mptr, mptrs[l - 1].data(), mptrs[l - 1].size()))) return r;
     if ((r = dupload(ctx, &V.mem, mem.data(), mem.size()))) return r;
     if ((r = dupload(ctx, &V.agg, agg.data(), agg.size()))) return r;
-    float2 **vecs[7] = {&V.b, &V.t, &V.e1, &V.e2, &V.w1, &V.w2, &V.rc2};
+    float2 **vecs[8] = {&V.b, &V.t, &V.e1, &V.e2, &V.w1, &V.w2, &V.rc2, &V.r};
     for (float2 **v : vecs)
       if ((r = dalloc(ctx, v, (size_t)C.n))) return r;
     if ((r = dalloc(ctx, &V.part, (size_t)3 * AMG_DOT_BLOCKS))) return r;
     if ((r = dalloc(ctx, &V.sc, (size_t)8))) return r;
     HIPCHK(ctx, hipMemset(V.sc, 0, 8 * sizeof(double)));
   }
+  if ((r = dalloc(ctx, &ctx->amg_y, (size_t)nA))) return r;
+  if ((r = dalloc(ctx, &ctx->amg_v32, (size_t)nA))) return r;
+  if ((r = dalloc(ctx, &ctx->amg_z32, (size_t)nA))) return r;
   AmgLevel &Lc = ctx->amg[L - 1];
   ctx->amg_direct = Lc.n <= AMG_DIRECT;
   if (ctx->amg_direct) {
@@ -2338,7 +2365,8 @@ static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
     std::swap(cur, oth);
   }
   AmgLevel &C = ctx->amg[l + 1];
-  k_amg_restrict<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(A, C.n, C.mptr, C.mem, cur, b, C.b);
+  k_amg_residual<<<nb, BLOCK, 0, ctx->stream>>>(A, cur, b, V.r);
+  k_amg_gather<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(C.n, C.mptr, C.mem, V.r, nullptr, C.b);
   amg_solve_level(ctx, l + 1);
   k_amg_prolong<<<nb, BLOCK, 0, ctx->stream>>>(V.n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, cur);
   for (int s = 0; s < nu; s++) {
@@ -2363,6 +2391,35 @@ static void amg_solve_level(ufe_ctx *ctx, int l) {
   k_amg_kfin<<<1, 256, 0, ctx->stream>>>(nd, V.part, V.sc, 2);
 }
 
+// The coarse levels of one application (level 1's b in, e1, e2 and sc out)
+// only touch fixed buffers, so they are captured once per mesh and smoothing
+// count and replayed: about 170 small kernels become one graph launch.
+static int amg_capture(ufe_ctx *ctx) {
+  if (ctx->amg_graph && ctx->amg_graph_nu == ctx->amg_nu) return UFE_OK;
+  if (ctx->amg_graph) (void)hipGraphExecDestroy(ctx->amg_graph);
+  ctx->amg_graph = nullptr;
+  hipGraph_t g = nullptr;
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  amg_solve_level(ctx, 1);
+  const hipError_t e1 = hipGetLastError();
+  const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    ctx->err = std::string("multigrid graph capture: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2);
+    if (g) (void)hipGraphDestroy(g);
+    return UFE_HIP_ERROR;
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e3 != hipSuccess) {
+    ctx->err = std::string("multigrid graph instantiate: ") + hipGetErrorString(e3);
+    return UFE_HIP_ERROR;
+  }
+  ctx->amg_graph = ex;
+  ctx->amg_graph_nu = ctx->amg_nu;
+  return UFE_OK;
+}
+
 // z ~ A^-1 v on the fine level: nu forward FP32 SOR sweeps from zero, the
 // coarse correction of the fine residual, nu backward sweeps; z leaves with
 // the Neumann condition on the edge rows, like the SOR preconditioner's.
@@ -2371,13 +2428,23 @@ static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float
   k_fill<float2><<<nblk(n), BLOCK, 0, ctx->stream>>>(z, n, make_float2(0.0f, 0.0f));
   for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, false, false, true);
   AmgLevel &C = ctx->amg[1];
-  k_amg_restrict_fine<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32, C.n, C.mptr,
-                                                             C.mem, z, v, C.b);
-  amg_solve_level(ctx, 1);
+  k_matvec<float><<<nblk(n), BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32,
+                                                      active_ranges(ctx), z, ctx->amg_y);
+  k_amg_gather<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(C.n, C.mptr, C.mem, v, ctx->amg_y, C.b);
+  if (ctx->amg_graph) (void)hipGraphLaunch(ctx->amg_graph, ctx->stream);
+  else amg_solve_level(ctx, 1);
   k_amg_prolong<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, z);
   if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), z);
   k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), z);
   for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, true, false, true);
+}
+
+// The same for an FP64 Krylov method: the cycle runs in FP32 on rounded copies
+static void amg_apply_f64(ufe_ctx *ctx, const AaAcMesh &am, float omega, const double2 *v, double2 *z) {
+  const int n = ctx->nA;
+  k_to_f32_pair<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, v, ctx->amg_v32);
+  amg_apply(ctx, am, omega, ctx->amg_v32, ctx->amg_z32);
+  k_to_f64_pair<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->amg_z32, z);
 }
 
 // colour passes of one multigrid application (for the sweep count)
@@ -2647,7 +2714,7 @@ static int krylov_alloc(ufe_ctx *ctx, int m, bool f32) {
 // H(i,j) = H1[65 j + i] (+ H2[65 j + i]) for i <= j and H(j+1,j) = Hs[j].
 // Returns the number of columns used (fewer on breakdown).
 static int hessenberg_lsq(int m, const double *H1, const double *H2, const double *Hs, double beta,
-                          std::vector<double> &y) {
+                          std::vector<double> &y, double *resid = nullptr) {
   std::vector<double> H((size_t)(m + 1) * m, 0.0), g(m + 1, 0.0), cs(m), sn(m);
   auto Hat = [&](int i, int j) -> double & { return H[(size_t)j * (m + 1) + i]; };
   for (int j = 0; j < m; j++) {
@@ -2673,6 +2740,7 @@ static int hessenberg_lsq(int m, const double *H1, const double *H2, const doubl
     g[j + 1] = -sn[j] * g[j];
     g[j] = cs[j] * g[j];
   }
+  if (resid) *resid = std::fabs(g[kk]);
   y.assign(std::max(kk, 1), 0.0);
   for (int i = kk - 1; i >= 0; i--) {
     double t = g[i];
@@ -2689,9 +2757,13 @@ static int64_t precond_passes(const ufe_ctx *ctx, int ks) {
 }
 
 // One GMRES(m) cycle on A d = r (r held in R32 or kry_R64 with |r|^2 in d_scalar[1]),
-// then w += d in FP64.  Templated on the Krylov precision.
+// then w += d in FP64.  Templated on the Krylov precision.  target > 0: every
+// GMRES_CHECK iterations the least-squares residual is evaluated on the host
+// and the cycle ends early once it has fallen below target |r|.
+constexpr int GMRES_CHECK = 4;
 extern "C++" template <typename T>
-static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passes, int64_t *cpass, int *matvecs) {
+static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passes, int64_t *cpass, int *matvecs,
+                       double target = 0.0) {
   using T2 = typename V2T<T>::t;
   const bool f32 = sizeof(T) == 4;
   const int n = ctx->nA;
@@ -2710,8 +2782,9 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
   }
   for (int j = 0; j < m; j++) {
     T2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;
-    if (f32 && ctx->pc_amg) {
-      amg_apply(ctx, am, (float)omega, (const float2 *)vj, (float2 *)zj);
+    if (ctx->pc_amg) {
+      if (f32) amg_apply(ctx, am, (float)omega, (const float2 *)vj, (float2 *)zj);
+      else amg_apply_f64(ctx, am, (float)omega, (const double2 *)vj, (double2 *)zj);
       *cpass += amg_passes(ctx);
     } else {
       k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
@@ -2743,6 +2816,22 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
     k_multi_dot<T><<<DOT_BLOCKS, BLOCK, 0, ctx->stream>>>(n, 1, ar, w, w, ctx->kry_part);
     k_multi_dot_reduce<<<1, 256, 0, ctx->stream>>>(DOT_BLOCKS, 1, ctx->kry_part, ctx->kry_nrm, 0);
     k_normalize<T><<<nb, BLOCK, 0, ctx->stream>>>(n, w, ctx->kry_nrm, V + (int64_t)(j + 1) * n, ctx->kry_Hs + j);
+    if (target > 0.0 && j + 1 < m && (j + 1) % GMRES_CHECK == 0) {
+      const int k = j + 1;
+      std::vector<double> G1(65 * k), G2(65 * k), Gs(k), g(4), yy;
+      HIPCHK(ctx, hipMemcpyAsync(G1.data(), ctx->kry_H1, sizeof(double) * 65 * k, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipMemcpyAsync(G2.data(), ctx->kry_H2, sizeof(double) * 65 * k, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipMemcpyAsync(Gs.data(), ctx->kry_Hs, sizeof(double) * k, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipMemcpyAsync(g.data(), ctx->d_scalar, sizeof(double) * 3, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      double res = HUGE_VAL;
+      const double beta = std::sqrt(g[1]);
+      hessenberg_lsq(k, G1.data(), G2.data(), Gs.data(), beta, yy, &res);
+      if (res <= target * beta) {
+        m = k;
+        break;
+      }
+    }
   }
   LAUNCHCHK(ctx);
   // least squares min |beta e1 - H y| on the host (Givens rotations)
@@ -2792,8 +2881,9 @@ static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max
     k_multi_dot_reduce<<<k, 256, 0, ctx->stream>>>(DOT_BLOCKS, k, ctx->kry_part, out, 0);
   };
   auto precond = [&](const T2 *in, T2 *out) {
-    if (f32 && ctx->pc_amg) {
-      amg_apply(ctx, am, (float)omega, (const float2 *)in, (float2 *)out);
+    if (ctx->pc_amg) {
+      if (f32) amg_apply(ctx, am, (float)omega, (const float2 *)in, (float2 *)out);
+      else amg_apply_f64(ctx, am, (float)omega, (const double2 *)in, (double2 *)out);
       return;
     }
     k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(out, n, T2{T(0), T(0)});
@@ -2901,12 +2991,14 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
     if ((r = amg_setup(ctx))) return r;
     amg_refresh(ctx);
     LAUNCHCHK(ctx);
+    if ((r = amg_capture(ctx))) return r;
   }
   if (o->method == 1) {
     // BiCGStab-IR: FP64 residual, inner BiCGStab in FP32 (mixed) or FP64
     if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], nullptr);
     k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
-    if ((r = krylov_alloc(ctx, std::max(mr, 8), mixed))) return r;
+    const bool k32 = mixed && !ctx->pc_amg;   // the multigrid keeps an FP64 Krylov method
+    if ((r = krylov_alloc(ctx, std::max(mr, 8), k32))) return r;
     if (!ctx->bic_S && (r = dalloc(ctx, &ctx->bic_S, 16))) return r;
     const int ks = o->precond_sweeps > 0 ? o->precond_sweeps : 2;
     if ((r = residual_fp64(ctx, &rel, mixed))) return r;
@@ -2916,9 +3008,9 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
     while (rel > o->rel_tol && cp / 5 < o->max_sweeps) {
       if (rel > 0.99 * prev) break;
       prev = rel;
-      const double reduce = std::max(0.1 * o->rel_tol / rel, mixed ? 1e-4 : 1e-13);
-      r = mixed ? bicg_inner<float>(ctx, ks, omega, reduce, o->max_sweeps, &cp, &st->matvecs)
-                : bicg_inner<double>(ctx, ks, omega, reduce, o->max_sweeps, &cp, &st->matvecs);
+      const double reduce = std::max(0.1 * o->rel_tol / rel, k32 ? 1e-4 : 1e-13);
+      r = k32 ? bicg_inner<float>(ctx, ks, omega, reduce, o->max_sweeps, &cp, &st->matvecs)
+              : bicg_inner<double>(ctx, ks, omega, reduce, o->max_sweeps, &cp, &st->matvecs);
       st->sweeps = (int32_t)(cp / 5);
       if (r) return r;
       if ((r = residual_fp64(ctx, &rel, mixed))) return r;
@@ -2935,7 +3027,8 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
     // rows), which the corrections keep: start the velocities there as well
     if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], nullptr);
     k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
-    if ((r = krylov_alloc(ctx, mr, mixed))) return r;
+    const bool k32 = mixed && !ctx->pc_amg;   // the multigrid keeps an FP64 Krylov method
+    if ((r = krylov_alloc(ctx, mr, k32))) return r;
     const int ks = o->precond_sweeps > 0 ? o->precond_sweeps : 2;
     if ((r = residual_fp64(ctx, &rel, true))) return r;
     st->refinements = 1;
@@ -2946,8 +3039,10 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
       prev = rel;
       int mv = 0;
       const int passes = o->orth_passes > 0 ? o->orth_passes : 2;
-      r = mixed ? gmres_cycle<float>(ctx, mr, ks, omega, passes, &cp, &mv)
-                : gmres_cycle<double>(ctx, mr, ks, omega, passes, &cp, &mv);
+      // end the cycle once its own residual estimate is half of what the tolerance asks
+      const double target = 0.5 * o->rel_tol / rel;
+      r = k32 ? gmres_cycle<float>(ctx, mr, ks, omega, passes, &cp, &mv, target)
+              : gmres_cycle<double>(ctx, mr, ks, omega, passes, &cp, &mv, target);
       st->sweeps = (int32_t)(cp / 5);
       if (r) return r;
       st->matvecs += mv;
