@@ -74,7 +74,7 @@ def main():
     ap.add_argument("--precond", type=int, default=0,
                     help="SOR sweeps per preconditioner application (sor: default 5) or smoothing sweeps "
                          "before and after the coarse correction (multigrid: default 2)")
-    ap.add_argument("--pc", default="sor", choices=["sor", "multigrid"],
+    ap.add_argument("--pc", default="multigrid", choices=["sor", "multigrid"],
                     help="preconditioner of the mixed_gmres path")
     ap.add_argument("--orth-passes", type=int, default=1, help="Gram-Schmidt passes (1 or 2)")
     ap.add_argument("--full-sweeps", action="store_true",

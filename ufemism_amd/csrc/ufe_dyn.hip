@@ -257,7 +257,6 @@ struct ufe_ctx {
   int amg_nu = 2;                   // smoothing sweeps before and after the coarse correction
   double *amg_W = nullptr, *amg_inv = nullptr;
   float2 *amg_y = nullptr;          // fine-level A z of the restriction
-  float2 *amg_v32 = nullptr, *amg_z32 = nullptr;   // FP32 copies around an FP64 Krylov method
   hipGraphExec_t amg_graph = nullptr;   // the coarse levels of one application, captured
   int amg_graph_nu = 0;
   double amg_setup_seconds = 0.0;
@@ -359,7 +358,7 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->amg.clear();
   ctx->amg_ready = ctx->amg_direct = false;
   ctx->amg_W = ctx->amg_inv = nullptr;
-  ctx->amg_y = ctx->amg_v32 = ctx->amg_z32 = nullptr;
+  ctx->amg_y = nullptr;
   if (ctx->amg_graph) (void)hipGraphExecDestroy(ctx->amg_graph);
   ctx->amg_graph = nullptr;
   ctx->amg_graph_nu = 0;
@@ -2291,8 +2290,6 @@ static int amg_setup(ufe_ctx *ctx) {
     HIPCHK(ctx, hipMemset(V.sc, 0, 8 * sizeof(double)));
   }
   if ((r = dalloc(ctx, &ctx->amg_y, (size_t)nA))) return r;
-  if ((r = dalloc(ctx, &ctx->amg_v32, (size_t)nA))) return r;
-  if ((r = dalloc(ctx, &ctx->amg_z32, (size_t)nA))) return r;
   AmgLevel &Lc = ctx->amg[L - 1];
   ctx->amg_direct = Lc.n <= AMG_DIRECT;
   if (ctx->amg_direct) {
@@ -2332,7 +2329,7 @@ static void amg_refresh(ufe_ctx *ctx) {
   }
   if (ctx->amg_direct) {
     AmgLevel &V = ctx->amg[L - 1];
-    k_amg_dense_inv<<<1, 256, 0, ctx->stream>>>(V.n, V.ptr, V.col, V.cf64, V.S64, V.ev64, ctx->amg_W, ctx->amg_inv);
+    k_amg_dense_inv<<<1, AMG_INV_THREADS, 0, ctx->stream>>>(V.n, V.ptr, V.col, V.cf64, V.S64, V.ev64, ctx->amg_W, ctx->amg_inv);
   }
 }
 
@@ -2437,14 +2434,6 @@ static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float
   if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), z);
   k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), z);
   for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, true, false, true);
-}
-
-// The same for an FP64 Krylov method: the cycle runs in FP32 on rounded copies
-static void amg_apply_f64(ufe_ctx *ctx, const AaAcMesh &am, float omega, const double2 *v, double2 *z) {
-  const int n = ctx->nA;
-  k_to_f32_pair<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, v, ctx->amg_v32);
-  amg_apply(ctx, am, omega, ctx->amg_v32, ctx->amg_z32);
-  k_to_f64_pair<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->amg_z32, z);
 }
 
 // colour passes of one multigrid application (for the sweep count)
@@ -2782,9 +2771,8 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
   }
   for (int j = 0; j < m; j++) {
     T2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;
-    if (ctx->pc_amg) {
-      if (f32) amg_apply(ctx, am, (float)omega, (const float2 *)vj, (float2 *)zj);
-      else amg_apply_f64(ctx, am, (float)omega, (const double2 *)vj, (double2 *)zj);
+    if (f32 && ctx->pc_amg) {
+      amg_apply(ctx, am, (float)omega, (const float2 *)vj, (float2 *)zj);
       *cpass += amg_passes(ctx);
     } else {
       k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
@@ -2881,9 +2869,8 @@ static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max
     k_multi_dot_reduce<<<k, 256, 0, ctx->stream>>>(DOT_BLOCKS, k, ctx->kry_part, out, 0);
   };
   auto precond = [&](const T2 *in, T2 *out) {
-    if (ctx->pc_amg) {
-      if (f32) amg_apply(ctx, am, (float)omega, (const float2 *)in, (float2 *)out);
-      else amg_apply_f64(ctx, am, (float)omega, (const double2 *)in, (double2 *)out);
+    if (f32 && ctx->pc_amg) {
+      amg_apply(ctx, am, (float)omega, (const float2 *)in, (float2 *)out);
       return;
     }
     k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(out, n, T2{T(0), T(0)});
@@ -2997,7 +2984,7 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
     // BiCGStab-IR: FP64 residual, inner BiCGStab in FP32 (mixed) or FP64
     if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], nullptr);
     k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
-    const bool k32 = mixed && !ctx->pc_amg;   // the multigrid keeps an FP64 Krylov method
+    const bool k32 = mixed;   // FP32 Krylov vectors
     if ((r = krylov_alloc(ctx, std::max(mr, 8), k32))) return r;
     if (!ctx->bic_S && (r = dalloc(ctx, &ctx->bic_S, 16))) return r;
     const int ks = o->precond_sweeps > 0 ? o->precond_sweeps : 2;
@@ -3027,7 +3014,7 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
     // rows), which the corrections keep: start the velocities there as well
     if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], nullptr);
     k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
-    const bool k32 = mixed && !ctx->pc_amg;   // the multigrid keeps an FP64 Krylov method
+    const bool k32 = mixed;   // FP32 Krylov vectors
     if ((r = krylov_alloc(ctx, mr, k32))) return r;
     const int ks = o->precond_sweeps > 0 ? o->precond_sweeps : 2;
     if ((r = residual_fp64(ctx, &rel, true))) return r;
