@@ -70,7 +70,8 @@ def main():
     ap.add_argument("--omega-mixed", type=float, default=0.0, help="omega of the GMRES preconditioner")
     ap.add_argument("--check", type=int, default=256, help="residual checks of the plain SOR path")
     ap.add_argument("--max-sweeps", type=int, default=400000)
-    ap.add_argument("--gmres", type=int, default=40, help="GMRES restart length")
+    ap.add_argument("--gmres", type=int, default=0,
+                    help="GMRES restart length (default 20 with the multigrid preconditioner, 40 with SOR)")
     ap.add_argument("--precond", type=int, default=0,
                     help="SOR sweeps per preconditioner application (sor: default 5) or smoothing sweeps "
                          "before and after the coarse correction (multigrid: default 2)")
@@ -109,6 +110,8 @@ def main():
     args.pc_type = 1 if args.pc == "multigrid" else 0     # UFE_PRECOND_MULTIGRID / UFE_PRECOND_SOR
     args.sor_sweeps = args.precond if (args.precond > 0 and args.pc == "sor") else 5
     args.pc_sweeps = args.precond if args.precond > 0 else (2 if args.pc == "multigrid" else 5)
+    args.gmres_sor = args.gmres if args.gmres > 0 else 40
+    args.gmres_pc = args.gmres if args.gmres > 0 else (20 if args.pc == "multigrid" else 40)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -171,7 +174,7 @@ def main():
                                         omega=omega, check_interval=args.check)
         mixed = path == "mixed_gmres"
         return ctx.ssa_linear_solve(args.tol, args.max_sweeps, precision=abi.UFE_MIXED if mixed else abi.UFE_FP64,
-                                    omega=omega_mixed, gmres=args.gmres,
+                                    omega=omega_mixed, gmres=args.gmres_pc if mixed else args.gmres_sor,
                                     precond_sweeps=args.pc_sweeps if mixed else args.sor_sweeps,
                                     orth_passes=args.orth_passes,
                                     method=1 if args.krylov == "bicgstab" else 0,
@@ -243,7 +246,7 @@ def main():
                         f"FP64 relative residual {args.tol:g}",
             "nV": sizes["nV"], "nVAaAc": nA, "mean_degree": round(sb["mean_degree"], 4),
             "path": head, "omega": omega, "omega_inner": omega_mixed,
-            "krylov": args.krylov, "gmres_restart": args.gmres, "preconditioner": args.pc,
+            "krylov": args.krylov, "gmres_restart": args.gmres_pc, "preconditioner": args.pc,
             "precond_sweeps": args.pc_sweeps,
             "orth_passes": args.orth_passes,
             "value_definition": "nVAaAc * (FP64 reference-SOR sweeps to the tolerance) / "
@@ -426,7 +429,7 @@ def bench_precision(args, ctx, step, wl):
             for f in ("U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac", "N_AaAc"):
                 ctx.fill(f, 0.0)
             mixed = mode == abi.UFE_MIXED
-            ctx.set_params(p, mode, linear_rel_tol=tol, gmres_restart=args.gmres,
+            ctx.set_params(p, mode, linear_rel_tol=tol, gmres_restart=args.gmres_pc if mixed else args.gmres_sor,
                            precond_sweeps=args.pc_sweeps if mixed else args.sor_sweeps,
                            precond_type=args.pc_type if mixed else abi.UFE_PRECOND_SOR)
             ctx.synchronize()
@@ -552,7 +555,7 @@ def bench_batch(args, wls):
         ctx.synchronize()
         ta = time.perf_counter()
         st, rel = ctx.batch_solve_SSA(args.batch_picard, rel_tol=args.tol, precision=prec,
-                                      gmres_restart=args.gmres, precond_sweeps=args.sor_sweeps, use_graph=graph)
+                                      gmres_restart=args.gmres_sor, precond_sweeps=args.sor_sweeps, use_graph=graph)
         ctx.synchronize()
         el = time.perf_counter() - ta
         return st, rel, el, B.member_values("U_SSA_AaAc"), B.member_values("V_SSA_AaAc")

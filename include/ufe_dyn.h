@@ -99,7 +99,7 @@ typedef struct ufe_params {
    * FP32 sweeps and Krylov vectors for UFE_MIXED, FP64 for UFE_FP64_KRYLOV,
    * with FP64 residuals and iterative refinement either way. */
   double SSA_linear_rel_tol;        /* <= 0: 1e-6 */
-  int32_t SSA_gmres_restart;        /* <= 0: 40 */
+  int32_t SSA_gmres_restart;        /* <= 0: 40 (SOR) or 20 (multigrid) */
   int32_t SSA_precond_sweeps;       /* <= 0: 5 */
   /* ice_physical_properties (src/general_ice_model_data_module.f90:298-462),
    * run at the end of update_general_ice_model_data (:91) when non-zero:

@@ -253,7 +253,7 @@ def _ptr(a):
 
 
 def params_struct(p: DynParams, precision=UFE_FP64, sweep_batch=0, linear_rel_tol=1e-6,
-                  gmres_restart=40, precond_sweeps=5, precond_type=0) -> Params:
+                  gmres_restart=0, precond_sweeps=0, precond_type=0) -> Params:
     s = Params()
     s.nZ = p.nZ
     for k, z in enumerate(p.zeta):

@@ -2578,7 +2578,8 @@ int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) {
       lo.rel_tol = P.SSA_linear_rel_tol > 0.0 ? P.SSA_linear_rel_tol : 1e-6;
       lo.max_sweeps = 1 << 30;
       lo.precision = P.precision == UFE_MIXED ? UFE_MIXED : UFE_FP64;
-      lo.gmres_restart = P.SSA_gmres_restart > 0 ? P.SSA_gmres_restart : 40;
+      lo.gmres_restart = P.SSA_gmres_restart > 0 ? P.SSA_gmres_restart
+                                                   : (P.SSA_precond_type == UFE_PRECOND_MULTIGRID ? 20 : 40);
       lo.precond_type = P.SSA_precond_type;
       lo.precond_sweeps = P.SSA_precond_sweeps > 0 ? P.SSA_precond_sweeps
                                                    : (P.SSA_precond_type == UFE_PRECOND_MULTIGRID ? 2 : 5);

@@ -41,8 +41,9 @@ MODULE ice_dynamics_module
     ! UFE_FP64_KRYLOV solve each linearised system to SSA_linear_rel_tol by GMRES-IR
     INTEGER  :: precision = 0
     REAL(dp) :: SSA_linear_rel_tol = 1E-6_dp
-    INTEGER  :: SSA_gmres_restart = 40
-    INTEGER  :: SSA_precond_sweeps = 5
+    ! 0: the library defaults (40 and 5 with SOR sweeps, 20 and 2 with multigrid)
+    INTEGER  :: SSA_gmres_restart = 0
+    INTEGER  :: SSA_precond_sweeps = 0
     ! preconditioner of those solves (UFE_MIXED): 0 SOR sweeps, 1 aggregation multigrid
     INTEGER  :: SSA_precond_type = 0
     ! ice_physical_properties at the end of update_general_ice_model_data:
