@@ -8,7 +8,7 @@ solves (UFE_PRECOND_MULTIGRID) on the 20k marine system:
 * a second solve of the same system gives the same bits (no atomics, fixed
   reduction order);
 * solve_SSA's Picard loop with SSA_precond_type = multigrid agrees with the
-  FP64 Krylov step;
+  FP64 Krylov step, on the marine sheet and on a grounded dome;
 * FP64 precision or plain SOR with the multigrid preconditioner is refused.
 """
 import numpy as np
@@ -127,3 +127,34 @@ def test_picard_loop_with_multigrid():
             assert e <= 1e-7, (k, e)
     finally:
         ctx.close()
+
+
+def test_multigrid_on_a_grounded_dome():
+    """A fully grounded dome with ice-free margins (large sliding terms around
+    it): the multigrid-preconditioned Picard loop agrees with the FP64 Krylov
+    step."""
+    print("[multigrid ssa] dome8k, mixed+multigrid vs fp64 Krylov at 1e-10", flush=True)
+    ctx = abi.Context(0)
+    try:
+        wl = geometry.dome(8000, seed=1, picard=10)
+        step = Step(ctx, wl)
+        out = {}
+        for mode, pc in ((abi.UFE_FP64_KRYLOV, abi.UFE_PRECOND_SOR), (abi.UFE_MIXED, abi.UFE_PRECOND_MULTIGRID)):
+            step.reset()
+            for f in ("U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac", "N_AaAc"):
+                ctx.fill(f, 0.0)
+            ctx.set_params(wl.params, mode, linear_rel_tol=1e-10, precond_type=pc)
+            r = step.run()
+            assert r["stats"].status == 0, r["stats"].status
+            out[mode] = r
+            print(f"  mode {mode} precond {pc}: picard {r['picard']} sweeps {r['sweeps']}", flush=True)
+        info = ctx.multigrid_info()
+        assert info["levels"] >= 2 and info["direct"]
+        a, b = out[abi.UFE_FP64_KRYLOV], out[abi.UFE_MIXED]
+        for k in ("U_SSA", "V_SSA", "Hi"):
+            e = _rel(b[k], a[k])
+            print(f"  {k}: rel L2 {e:.2e}", flush=True)
+            assert e <= 1e-7, (k, e)
+    finally:
+        ctx.close()
+
