@@ -530,9 +530,10 @@ def bench_batch(args, wls):
     """Batched ensemble: independent SSA solves (K Picard steps each) on seeded
     meshes with sizes log-uniform in [batch-min, batch-max], dome or marine,
     A uniform in [1e-17, 1e-16], log-normal friction; every linearised system to
-    the relative residual --tol in mixed precision, one hipGraph launch per GMRES
-    cycle of all members, the sweep's launch shape autotuned; error per member
-    against the same batch in FP64."""
+    the relative residual --tol in mixed precision (with the --pc preconditioner,
+    the members' union aggregated as one mesh for multigrid), one hipGraph launch
+    per GMRES cycle of all members, the sweep's launch shape autotuned; error per
+    member against the same batch in FP64 with SOR sweeps as preconditioner."""
     import numpy as np
     from ufemism_amd import abi, batch
 
@@ -554,8 +555,12 @@ def bench_batch(args, wls):
         ctx.update_general_ice_model_data()
         ctx.synchronize()
         ta = time.perf_counter()
+        mixed = prec == abi.UFE_MIXED       # the multigrid preconditioner needs the mixed path
         st, rel = ctx.batch_solve_SSA(args.batch_picard, rel_tol=args.tol, precision=prec,
-                                      gmres_restart=args.gmres_sor, precond_sweeps=args.sor_sweeps, use_graph=graph)
+                                      gmres_restart=args.gmres_pc if mixed else args.gmres_sor,
+                                      precond_sweeps=args.pc_sweeps if mixed else args.sor_sweeps,
+                                      use_graph=graph,
+                                      precond_type=args.pc_type if mixed else abi.UFE_PRECOND_SOR)
         ctx.synchronize()
         el = time.perf_counter() - ta
         return st, rel, el, B.member_values("U_SSA_AaAc"), B.member_values("V_SSA_AaAc")
@@ -575,6 +580,7 @@ def bench_batch(args, wls):
                     f"A ~ U[1e-17, 1e-16], log-normal friction; linear tolerance {args.tol:g}",
         "members": len(wls), "vertices": int(sum(w.mesh.nV for w in wls)), "nVAaAc": int(rows),
         "member_nV_min": int(min(w.mesh.nV for w in wls)), "member_nV_max": int(max(w.mesh.nV for w in wls)),
+        "preconditioner": args.pc,
         "seconds": round(el, 4), "fp64_seconds": round(el0, 4), "speedup_vs_fp64": round(el0 / el, 3),
         "gmres_cycles": st.cycles, "sweeps_per_member": st.sweeps, "converged": st.converged,
         "aggregate_Mvertex_sweeps_per_s": round(rows * st.sweeps / el / 1e6, 1),

@@ -437,6 +437,9 @@ typedef struct ufe_batch_opts {
   double omega;                     /* <= 0: params.SSA_SOR_omega */
   int32_t max_cycles;               /* GMRES cycles (all members in lockstep; <= 0: unbounded) */
   int32_t use_graph;                /* 1: each cycle of all members is one hipGraph launch */
+  int32_t precond_type;             /* UFE_PRECOND_SOR, or UFE_PRECOND_MULTIGRID (UFE_MIXED; the union
+                                       of the members is aggregated as one mesh, precond_sweeps
+                                       smoothing sweeps, default 2) */
 } ufe_batch_opts;
 
 typedef struct ufe_batch_stats {

@@ -101,3 +101,25 @@ def test_autotuned_shape_changes_no_bits(ens):
         for b in range(len(wls)):
             assert np.array_equal(U0[b], U2[b]) and np.array_equal(V0[b], V2[b]), (shape, b)
     ctx.sweep_tuning_set(t.threads_per_block, t.schedule, t.loads)
+
+
+def test_batch_multigrid_matches_fp64(ens):
+    """The union of the members aggregated as one hierarchy: every member
+    agrees with the FP64 batch, and the batch's graph replays (the coarse
+    levels recorded inline) equal eager launches (the coarse levels replayed
+    from the multigrid's own graph) bit for bit."""
+    ctx, B, wls = ens
+    U0, V0, _ = _solve(ctx, B, precision=abi.UFE_FP64)
+    kw = dict(precision=abi.UFE_MIXED, precond_type=abi.UFE_PRECOND_MULTIGRID, precond_sweeps=2, gmres_restart=20)
+    Ug, Vg, sg = _solve(ctx, B, use_graph=True, **kw)
+    Ue, Ve, se = _solve(ctx, B, use_graph=False, **kw)
+    print(f"  multigrid batch: cycles {sg.cycles} sweeps {sg.sweeps} {sg.seconds:.3f}s; "
+          f"hierarchy {ctx.multigrid_info()['points']}", flush=True)
+    assert sg.cycles == se.cycles
+    for b in range(len(wls)):
+        e = max(_rel(Ug[b], U0[b]), _rel(Vg[b], V0[b]))
+        print(f"  member {b}: multigrid vs fp64 rel L2 {e:.2e}", flush=True)
+        assert e <= 1e-7, (b, e)
+        assert np.array_equal(Ug[b], Ue[b]) and np.array_equal(Vg[b], Ve[b]), b
+    with pytest.raises(abi.UFEError):
+        ctx.batch_linear_solve(precision=abi.UFE_FP64, precond_type=abi.UFE_PRECOND_MULTIGRID)

@@ -182,7 +182,7 @@ class AdjointStats(ct.Structure):
 class BatchOpts(ct.Structure):
     _fields_ = [("rel_tol", ct.c_double), ("precision", ct.c_int32), ("gmres_restart", ct.c_int32),
                 ("precond_sweeps", ct.c_int32), ("omega", ct.c_double), ("max_cycles", ct.c_int32),
-                ("use_graph", ct.c_int32)]
+                ("use_graph", ct.c_int32), ("precond_type", ct.c_int32)]
 
 
 class BatchStats(ct.Structure):
@@ -568,13 +568,15 @@ class Context:
         self._chk(self.lib.ufe_sweep_tuning_set(self.h, int(threads_per_block), int(schedule), int(loads)),
                   "sweep_tuning_set")
 
-    def _batch_opts(self, rel_tol, precision, gmres_restart, precond_sweeps, omega, max_cycles, use_graph):
+    def _batch_opts(self, rel_tol, precision, gmres_restart, precond_sweeps, omega, max_cycles, use_graph,
+                    precond_type=UFE_PRECOND_SOR):
         return BatchOpts(float(rel_tol), int(precision), int(gmres_restart), int(precond_sweeps),
-                         float(omega), int(max_cycles), int(bool(use_graph)))
+                         float(omega), int(max_cycles), int(bool(use_graph)), int(precond_type))
 
     def batch_linear_solve(self, rel_tol=1e-6, precision=UFE_MIXED, gmres_restart=40, precond_sweeps=5,
-                           omega=0.0, max_cycles=0, use_graph=True):
-        o = self._batch_opts(rel_tol, precision, gmres_restart, precond_sweeps, omega, max_cycles, use_graph)
+                           omega=0.0, max_cycles=0, use_graph=True, precond_type=UFE_PRECOND_SOR):
+        o = self._batch_opts(rel_tol, precision, gmres_restart, precond_sweeps, omega, max_cycles, use_graph,
+                             precond_type)
         st = BatchStats()
         rel = np.zeros(len(self._batch_sizes[0]))
         self._chk(self.lib.ufe_batch_linear_solve(self.h, ct.byref(o), _ptr(rel), ct.byref(st)),
@@ -582,8 +584,9 @@ class Context:
         return st, rel
 
     def batch_solve_SSA(self, picard_iterations, rel_tol=1e-6, precision=UFE_MIXED, gmres_restart=40,
-                        precond_sweeps=5, omega=0.0, max_cycles=0, use_graph=True):
-        o = self._batch_opts(rel_tol, precision, gmres_restart, precond_sweeps, omega, max_cycles, use_graph)
+                        precond_sweeps=5, omega=0.0, max_cycles=0, use_graph=True, precond_type=UFE_PRECOND_SOR):
+        o = self._batch_opts(rel_tol, precision, gmres_restart, precond_sweeps, omega, max_cycles, use_graph,
+                             precond_type)
         st = BatchStats()
         rel = np.zeros(len(self._batch_sizes[0]))
         self._chk(self.lib.ufe_batch_solve_SSA(self.h, int(picard_iterations), ct.byref(o), _ptr(rel),

@@ -238,6 +238,7 @@ struct ufe_ctx {
   std::vector<int64_t> b_rows;       // SOR rows per member
   // batch Krylov work space and the captured cycle
   int bk_m = 0, bk_ks = 0, bk_prec = -1, bk_bs = 0, bk_sched = -1, bk_loads = -1;
+  int bk_key = 0;                   // precision and preconditioner of the captured cycle
   double bk_omega = 0.0;
   void *bk_V = nullptr, *bk_Z = nullptr, *bk_w = nullptr, *bk_x = nullptr, *bk_R = nullptr;
   double *bk_part = nullptr, *bk_H = nullptr, *bk_Hs = nullptr, *bk_y = nullptr, *bk_nrm = nullptr,
@@ -349,7 +350,7 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->d_ch_start = ctx->d_ch_count = ctx->d_ch_member = ctx->d_mch_ptr = nullptr;
   ctx->h_mch_ptr.clear();
   ctx->b_rows.clear();
-  ctx->bk_m = ctx->bk_ks = ctx->bk_bs = 0;
+  ctx->bk_m = ctx->bk_ks = ctx->bk_bs = ctx->bk_key = 0;
   ctx->bk_prec = ctx->bk_sched = -1;
   ctx->bk_V = ctx->bk_Z = ctx->bk_w = ctx->bk_x = ctx->bk_R = nullptr;
   ctx->bk_part = ctx->bk_H = ctx->bk_Hs = ctx->bk_y = ctx->bk_nrm = ctx->bk_scale = ctx->bk_rel = nullptr;
@@ -2428,8 +2429,10 @@ static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float
   k_matvec<float><<<nblk(n), BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32,
                                                       active_ranges(ctx), z, ctx->amg_y);
   k_amg_gather<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(C.n, C.mptr, C.mem, v, ctx->amg_y, C.b);
-  if (ctx->amg_graph) (void)hipGraphLaunch(ctx->amg_graph, ctx->stream);
-  else amg_solve_level(ctx, 1);
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  (void)hipStreamIsCapturing(ctx->stream, &cap);
+  if (ctx->amg_graph && cap == hipStreamCaptureStatusNone) (void)hipGraphLaunch(ctx->amg_graph, ctx->stream);
+  else amg_solve_level(ctx, 1);     // eager, or recorded into the caller's graph
   k_amg_prolong<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, z);
   if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), z);
   k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), z);
@@ -3457,12 +3460,16 @@ static void batch_enqueue_cycle(ufe_ctx *ctx, int m, int ks, double omega, doubl
   k_bscale<T, T><<<nb, BLOCK, 0, ctx->stream>>>(n, (const T2 *)ctx->bk_R, ctx->bk_scale, ctx->d_mid, V);
   for (int j = 0; j < m; j++) {
     T2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;
-    k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
-    for (int k = 0; k < ks; k++) {
-      const bool skip = k > 0 && !ctx->repeat_turn;
-      const bool neu = !ctx->neumann_last || k == ks - 1;
-      if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1, skip, neu);
-      else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1, skip, neu);
+    if (f32 && ctx->pc_amg) {
+      amg_apply(ctx, am, (float)omega, (const float2 *)vj, (float2 *)zj);
+    } else {
+      k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
+      for (int k = 0; k < ks; k++) {
+        const bool skip = k > 0 && !ctx->repeat_turn;
+        const bool neu = !ctx->neumann_last || k == ks - 1;
+        if (f32) enqueue_sweep_fp32(ctx, am, (float)omega, (float2 *)zj, (const float2 *)vj, k & 1, skip, neu);
+        else enqueue_sweep_fp64_vec(ctx, am, omega, (double2 *)zj, (const double2 *)vj, k & 1, skip, neu);
+      }
     }
     if (f32)
       k_matvec<float><<<nb, BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32, ar,
@@ -3487,9 +3494,10 @@ static void batch_enqueue_cycle(ufe_ctx *ctx, int m, int ks, double omega, doubl
 }
 
 static int batch_capture(ufe_ctx *ctx, bool f32, int m, int ks, double omega, double tol) {
+  const int key = (f32 ? 1 : 2) + (ctx->pc_amg ? 4 * ctx->amg_nu : 0);   // precision and preconditioner
   if (ctx->bk_graph && ctx->bk_ks == ks && ctx->bk_omega == omega && ctx->bk_tol == tol &&
       ctx->bk_bs == ctx->tune_bs && ctx->bk_sched == ctx->tune_sched && ctx->bk_loads == ctx->tune_loads &&
-      ctx->bk_m == m)
+      ctx->bk_m == m && ctx->bk_key == key)
     return UFE_OK;
   if (ctx->bk_graph) (void)hipGraphExecDestroy(ctx->bk_graph);
   ctx->bk_graph = nullptr;
@@ -3515,6 +3523,7 @@ static int batch_capture(ufe_ctx *ctx, bool f32, int m, int ks, double omega, do
     return UFE_HIP_ERROR;
   }
   ctx->bk_graph = ex;
+  ctx->bk_key = key;
   ctx->bk_ks = ks;
   ctx->bk_omega = omega;
   ctx->bk_tol = tol;
@@ -3537,6 +3546,12 @@ int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *o, double *member
     return UFE_BAD_INPUT;
   }
   const bool f32 = o->precision == UFE_MIXED;
+  if (o->precond_type != UFE_PRECOND_SOR && !(o->precond_type == UFE_PRECOND_MULTIGRID && f32)) {
+    ctx->err = "ufe_batch_linear_solve: precond_type UFE_PRECOND_SOR, or UFE_PRECOND_MULTIGRID with UFE_MIXED";
+    return UFE_BAD_INPUT;
+  }
+  ctx->pc_amg = o->precond_type == UFE_PRECOND_MULTIGRID;
+  ctx->amg_nu = o->precond_sweeps > 0 ? o->precond_sweeps : 2;
   ctx->repeat_turn = false;          // batch preconditioner: turn colour once, Neumann at the end
   ctx->neumann_last = true;
   const int m = o->gmres_restart, ks = o->precond_sweeps > 0 ? o->precond_sweeps : 4;
@@ -3546,6 +3561,12 @@ int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *o, double *member
   if ((r = batch_alloc(ctx, m, f32))) return r;
   auto t0 = std::chrono::steady_clock::now();
   if (f32) k_to_f32_pair<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_pair[1], ctx->d_EV32);
+  if (ctx->pc_amg) {                  // the union of the members as one hierarchy
+    if ((r = amg_setup(ctx))) return r;
+    amg_refresh(ctx);
+    LAUNCHCHK(ctx);
+    if ((r = amg_capture(ctx))) return r;
+  }
   if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], nullptr);
   k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
   if (f32) batch_residual<float>(ctx, tol);
@@ -3577,7 +3598,7 @@ int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *o, double *member
       LAUNCHCHK(ctx);
     }
     st->cycles++;
-    cpass += (int64_t)m * precond_passes(ctx, ks);
+    cpass += (int64_t)m * (ctx->pc_amg ? amg_passes(ctx) : precond_passes(ctx, ks));
     st->sweeps = (int32_t)(cpass / 5);
   }
   st->converged = conv;
