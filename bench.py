@@ -608,8 +608,11 @@ def bench_adjoint(args, ctx):
     step = Step(ctx, wl)
     ctx.update_general_ice_model_data()
     log(f"[adjoint] mesh nV={wl.mesh.nV} nVAaAc={ctx.sizes['nVAaAc']} set up in {time.time() - t0:.1f}s")
+    mg = args.pc == "multigrid"       # FP64 Krylov around the FP32 cycle, both directions
     prob = SSAProblem(ctx, AdjointOptions(picard_iterations=args.adj_picard, lin_rel_tol=args.adj_tol,
-                                          checkpoint_every=args.adj_every))
+                                          checkpoint_every=args.adj_every, precond_type=args.pc_type,
+                                          gmres_restart=args.gmres_pc if mg else 40,
+                                          precond_sweeps=args.pc_sweeps if mg else 5))
     dev = prob.dev
     VA = step.ma["VAaAc"]
     log_tau0 = torch.log(torch.tensor(wl.tau_c_override(VA), dtype=torch.float64, device=dev))
@@ -637,6 +640,14 @@ def bench_adjoint(args, ctx):
     fs, bs = prob.forward_stats, prob.backward_stats
     log(f"[adjoint] forward {tf:.3f}s backward {tb:.3f}s ratio {tb / tf:.2f}")
     h = 1e-6
+    # The central differences are evaluated with SOR-preconditioned forwards.
+    # The step changes u by ~1e-6 relative while each linear solve leaves an
+    # error of ~1e-7 at its residual; that error cancels between the two sides
+    # only when both take the same iteration path, as SOR's fixed sweeps from
+    # the same start do (the multigrid K-cycle and the early cycle exits do not).
+    import dataclasses
+    fd_tol = args.adj_tol
+    prob.opts = dataclasses.replace(prob.opts, precond_type=0, gmres_restart=40, precond_sweeps=5)
     gen = torch.Generator(device="cpu").manual_seed(args.seed)
     errs = []
     for k in range(args.adj_fd):
@@ -659,7 +670,9 @@ def bench_adjoint(args, ctx):
         "forward_sweeps": fs.forward_sweeps, "adjoint_sweeps": bs.adjoint_sweeps,
         "recompute_sweeps": bs.recompute_sweeps,
         "max_forward_residual": fs.max_forward_residual, "max_adjoint_residual": bs.max_adjoint_residual,
-        "J": J, "fd_directions": args.adj_fd, "fd_step": h,
+        "preconditioner": args.pc,
+        "J": J, "fd_directions": args.adj_fd, "fd_step": h, "fd_lin_rel_tol": fd_tol,
+        "fd_forward_preconditioner": "sor",
         "fd_max_rel_error": max(errs) if errs else None,
         "fd_rel_errors": [float(f"{e:.3e}") for e in errs],
     }

@@ -108,7 +108,7 @@ typedef struct ufe_params {
    * UFE_PHYS_BENCHMARK_FIXED (2): the fixed values of the EISMINT/Halfar/Bueler
    * benchmarks (:323-348). */
   int32_t physical_properties;
-  /* UFE_MIXED only: the preconditioner of the GMRES-IR linear solves,
+  /* UFE_MIXED or UFE_FP64_KRYLOV: the preconditioner of the GMRES-IR linear solves,
    * UFE_PRECOND_SOR (SSA_precond_sweeps SOR sweeps, default 5) or
    * UFE_PRECOND_MULTIGRID (aggregation multigrid, SSA_precond_sweeps SOR
    * smoothing sweeps before and after the coarse correction, default 2). */
@@ -208,10 +208,11 @@ typedef struct ufe_linsolve_opts {
                                        preconditioner application only (the vector handed to the
                                        Krylov method satisfies it); 1: after every sweep */
   int32_t precond_type;             /* UFE_PRECOND_SOR: precond_sweeps SOR sweeps from zero;
-                                       UFE_PRECOND_MULTIGRID (UFE_MIXED, a Krylov method): one
-                                       aggregation-multigrid K-cycle, precond_sweeps (<= 0: 2) FP32
-                                       SOR sweeps before and after the coarse correction; set up on
-                                       the first use per mesh (ufe_multigrid_info) */
+                                       UFE_PRECOND_MULTIGRID (a Krylov method): one aggregation-
+                                       multigrid K-cycle in FP32, precond_sweeps (<= 0: 2) FP32 SOR
+                                       sweeps before and after the coarse correction, around FP32
+                                       (UFE_MIXED) or FP64 Krylov vectors; set up on the first use
+                                       per mesh (ufe_multigrid_info) */
 } ufe_linsolve_opts;
 
 /* Statistics of one linearised solve. */
@@ -377,6 +378,10 @@ typedef struct ufe_adjoint_opts {
   int32_t precond_sweeps;           /* SOR sweeps per preconditioner application */
   double omega;                     /* <= 0: params.SSA_SOR_omega */
   int32_t max_sweeps;               /* per linear solve */
+  int32_t precond_type;             /* UFE_PRECOND_SOR, or UFE_PRECOND_MULTIGRID for the forward solves
+                                       and the adjoint solves (the transposed hierarchy: every coarse
+                                       operator transposed, FP64 transposed SOR smoothing on the fine
+                                       level); precond_sweeps then counts smoothing sweeps (<= 0: 2) */
 } ufe_adjoint_opts;
 
 typedef struct ufe_adjoint_stats {

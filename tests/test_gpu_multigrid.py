@@ -9,7 +9,8 @@ solves (UFE_PRECOND_MULTIGRID) on the 20k marine system:
   reduction order);
 * solve_SSA's Picard loop with SSA_precond_type = multigrid agrees with the
   FP64 Krylov step, on the marine sheet and on a grounded dome;
-* FP64 precision or plain SOR with the multigrid preconditioner is refused.
+* FP64 Krylov vectors around the FP32 cycle reach 1e-11; plain SOR with the
+  multigrid preconditioner is refused.
 """
 import numpy as np
 import pytest
@@ -92,13 +93,21 @@ def test_multigrid_solve_is_repeatable(ctx, system):
     assert np.array_equal(a[1], b[1])
 
 
-def test_multigrid_needs_mixed_krylov(ctx, system):
-    with pytest.raises(abi.UFEError):
-        ctx.ssa_linear_solve(1e-6, 100, precision=abi.UFE_FP64, gmres=40,
-                             precond_type=abi.UFE_PRECOND_MULTIGRID)
-    with pytest.raises(abi.UFEError):
-        ctx.ssa_linear_solve(1e-6, 100, precision=abi.UFE_MIXED, gmres=0,
-                             precond_type=abi.UFE_PRECOND_MULTIGRID)
+def test_multigrid_needs_a_krylov_method(ctx, system):
+    for prec in (abi.UFE_FP64, abi.UFE_MIXED):
+        with pytest.raises(abi.UFEError):
+            ctx.ssa_linear_solve(1e-6, 100, precision=prec, gmres=0, precond_type=abi.UFE_PRECOND_MULTIGRID)
+
+
+def test_multigrid_with_fp64_krylov(ctx, system):
+    """FP64 Krylov vectors around the FP32 cycle reach 1e-11 directly."""
+    ref = system
+    st, uv = _solve(ctx, 1e-11, precision=abi.UFE_FP64, gmres=20, precond_sweeps=2,
+                    precond_type=abi.UFE_PRECOND_MULTIGRID)
+    print(f"  fp64 krylov + multigrid: matvecs {st.matvecs} refinements {st.refinements} rel {st.rel_residual:.1e}",
+          flush=True)
+    assert st.status == 0 and st.rel_residual <= 1e-11
+    assert _rel(uv, ref) <= 1e-6          # two solutions to a 1e-11 residual: within the conditioning
 
 
 def test_picard_loop_with_multigrid():

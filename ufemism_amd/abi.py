@@ -167,7 +167,7 @@ class AdjointOpts(ct.Structure):
     _fields_ = [("lin_rel_tol", ct.c_double), ("picard_iterations", ct.c_int32),
                 ("checkpoint_every", ct.c_int32), ("precision", ct.c_int32),
                 ("gmres_restart", ct.c_int32), ("precond_sweeps", ct.c_int32),
-                ("omega", ct.c_double), ("max_sweeps", ct.c_int32)]
+                ("omega", ct.c_double), ("max_sweeps", ct.c_int32), ("precond_type", ct.c_int32)]
 
 
 class AdjointStats(ct.Structure):
@@ -524,10 +524,11 @@ class Context:
 
     def adjoint_forward(self, picard_iterations, lin_rel_tol=1e-12, checkpoint_every=1,
                         precision=UFE_FP64, gmres_restart=40, precond_sweeps=5, omega=0.0,
-                        max_sweeps=1_000_000, allow=(UFE_OK, UFE_SOR_NOT_CONVERGED)) -> AdjointStats:
+                        max_sweeps=1_000_000, allow=(UFE_OK, UFE_SOR_NOT_CONVERGED),
+                        precond_type=UFE_PRECOND_SOR) -> AdjointStats:
         o = AdjointOpts(float(lin_rel_tol), int(picard_iterations), int(checkpoint_every),
                         int(precision), int(gmres_restart), int(precond_sweeps), float(omega),
-                        int(max_sweeps))
+                        int(max_sweeps), int(precond_type))
         st = AdjointStats()
         r = self.lib.ufe_ssa_adjoint_forward(self.h, ct.byref(o), ct.byref(st))
         if r not in allow:

@@ -35,6 +35,7 @@ class AdjointOptions:
     precond_sweeps: int = 5
     omega: float = 0.0
     max_sweeps: int = 1_000_000
+    precond_type: int = abi.UFE_PRECOND_SOR   # UFE_PRECOND_MULTIGRID: both directions multigrid-preconditioned
 
 
 class SSAProblem:
@@ -82,7 +83,7 @@ class SSAProblem:
         self.generation += 1
         self.forward_stats = self.ctx.adjoint_forward(
             o.picard_iterations, o.lin_rel_tol, o.checkpoint_every, o.precision, o.gmres_restart,
-            o.precond_sweeps, o.omega, o.max_sweeps)
+            o.precond_sweeps, o.omega, o.max_sweeps, precond_type=o.precond_type)
         U = torch.empty(self.nA, dtype=torch.float64, device=self.dev)
         V = torch.empty_like(U)
         self.ctx.get_device("U_SSA_AaAc", U.data_ptr())

@@ -122,6 +122,8 @@ struct AmgLevel {
   int64_t nnz = 0;
   int32_t *ptr = nullptr, *col = nullptr;   // off-diagonal couplings, CSR over points
   float2 *cf = nullptr;
+  int32_t *ptrT = nullptr, *colT = nullptr; // the transposed couplings (adjoint solves)
+  float2 *cfT = nullptr;
   float *S = nullptr;               // u-v coupling of each point with itself
   double2 *fd = nullptr;            // diagonal pair added to the members' diagonals (fixed per mesh)
   double2 *ev64 = nullptr;          // diagonal pair of the current system
@@ -260,6 +262,11 @@ struct ufe_ctx {
   float2 *amg_y = nullptr;          // fine-level A z of the restriction
   hipGraphExec_t amg_graph = nullptr;   // the coarse levels of one application, captured
   int amg_graph_nu = 0;
+  bool amg_T = false;               // cycle the transposed hierarchy (adjoint solves)
+  hipGraphExec_t amg_graph_T = nullptr;
+  int amg_graph_T_nu = 0;
+  double2 *amg_y64 = nullptr;       // FP64 fine-level A z (adjoint)
+  float2 *amg_v32 = nullptr, *amg_z32 = nullptr;   // FP32 copies around an FP64 Krylov method
   double amg_setup_seconds = 0.0;
 };
 
@@ -363,6 +370,12 @@ static void free_mesh(ufe_ctx *ctx) {
   if (ctx->amg_graph) (void)hipGraphExecDestroy(ctx->amg_graph);
   ctx->amg_graph = nullptr;
   ctx->amg_graph_nu = 0;
+  if (ctx->amg_graph_T) (void)hipGraphExecDestroy(ctx->amg_graph_T);
+  ctx->amg_graph_T = nullptr;
+  ctx->amg_graph_T_nu = 0;
+  ctx->amg_T = false;
+  ctx->amg_y64 = nullptr;
+  ctx->amg_v32 = ctx->amg_z32 = nullptr;
 }
 
 static size_t grid_len(const ufe_ctx *ctx, int grid) {
@@ -557,8 +570,8 @@ int ufe_set_params(ufe_ctx *ctx, const ufe_params *p) {
     return UFE_BAD_INPUT;
   }
   if (p->SSA_precond_type != UFE_PRECOND_SOR &&
-      !(p->SSA_precond_type == UFE_PRECOND_MULTIGRID && p->precision == UFE_MIXED)) {
-    ctx->err = "SSA_precond_type must be UFE_PRECOND_SOR, or UFE_PRECOND_MULTIGRID with precision UFE_MIXED";
+      !(p->SSA_precond_type == UFE_PRECOND_MULTIGRID && p->precision != UFE_FP64)) {
+    ctx->err = "SSA_precond_type must be UFE_PRECOND_SOR, or UFE_PRECOND_MULTIGRID with UFE_MIXED or UFE_FP64_KRYLOV";
     return UFE_BAD_INPUT;
   }
   ctx->params = *p;
@@ -2261,6 +2274,22 @@ static int amg_setup(ufe_ctx *ctx) {
     if ((r = dupload(ctx, &V.ptr, C.ptr.data(), C.ptr.size()))) return r;
     if ((r = dupload(ctx, &V.col, colv.data(), colv.size()))) return r;
     if ((r = dupload(ctx, &V.cf, cf32.data(), cf32.size()))) return r;
+    {   // the transposed couplings: the coefficient of I in J's row becomes J's in I's
+      std::vector<int32_t> pT(C.n + 1, 0), cT(std::max<size_t>(C.col.size(), 1), 0);
+      std::vector<float2> fT(std::max<size_t>(C.col.size(), 1), make_float2(0.0f, 0.0f));
+      for (size_t k = 0; k < C.col.size(); k++) pT[C.col[k] + 1]++;
+      for (int i = 0; i < C.n; i++) pT[i + 1] += pT[i];
+      std::vector<int32_t> cur(pT.begin(), pT.end() - 1);
+      for (int i = 0; i < C.n; i++)
+        for (int k = C.ptr[i]; k < C.ptr[i + 1]; k++) {
+          const int q = cur[C.col[k]]++;
+          cT[q] = i;
+          fT[q] = cf32[k];
+        }
+      if ((r = dupload(ctx, &V.ptrT, pT.data(), pT.size()))) return r;
+      if ((r = dupload(ctx, &V.colT, cT.data(), cT.size()))) return r;
+      if ((r = dupload(ctx, &V.cfT, fT.data(), fT.size()))) return r;
+    }
     if ((r = dupload(ctx, &V.S, S32.data(), S32.size()))) return r;
     // the part of the diagonal that does not come from the level above: at level
     // 1 the fine fixed parts (Neumann self-couplings) as well, since the fine
@@ -2291,6 +2320,9 @@ static int amg_setup(ufe_ctx *ctx) {
     HIPCHK(ctx, hipMemset(V.sc, 0, 8 * sizeof(double)));
   }
   if ((r = dalloc(ctx, &ctx->amg_y, (size_t)nA))) return r;
+  if ((r = dalloc(ctx, &ctx->amg_y64, (size_t)nA))) return r;
+  if ((r = dalloc(ctx, &ctx->amg_v32, (size_t)nA))) return r;
+  if ((r = dalloc(ctx, &ctx->amg_z32, (size_t)nA))) return r;
   AmgLevel &Lc = ctx->amg[L - 1];
   ctx->amg_direct = Lc.n <= AMG_DIRECT;
   if (ctx->amg_direct) {
@@ -2309,12 +2341,12 @@ static int amg_setup(ufe_ctx *ctx) {
   return UFE_OK;
 }
 
-static AmgOp amg_op(const AmgLevel &V) {
+static AmgOp amg_op(const ufe_ctx *ctx, const AmgLevel &V) {
   AmgOp A;
   A.n = V.n;
-  A.ptr = V.ptr;
-  A.col = V.col;
-  A.cf = V.cf;
+  A.ptr = ctx->amg_T ? V.ptrT : V.ptr;
+  A.col = ctx->amg_T ? V.colT : V.col;
+  A.cf = ctx->amg_T ? V.cfT : V.cf;
   A.S = V.S;
   A.ev = V.ev32;
   return A;
@@ -2340,11 +2372,11 @@ static void amg_solve_level(ufe_ctx *ctx, int l);
 // 2 nu sweeps alternate between t and x, starting in t, so they end in x.
 static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
   AmgLevel &V = ctx->amg[l];
-  const AmgOp A = amg_op(V);
+  const AmgOp A = amg_op(ctx, V);
   const int nb = nblk(V.n), last = (int)ctx->amg.size() - 1;
   if (l == last) {
     if (ctx->amg_direct) {
-      k_amg_dense_apply<<<1, 2 * AMG_DIRECT, 0, ctx->stream>>>(V.n, ctx->amg_inv, b, x);
+      k_amg_dense_apply<<<1, 2 * AMG_DIRECT, 0, ctx->stream>>>(V.n, ctx->amg_inv, b, x, ctx->amg_T ? 1 : 0);
       return;
     }
     k_amg_bj<<<nb, BLOCK, 0, ctx->stream>>>(A, b, nullptr, V.t, AMG_BJ_OMEGA);
@@ -2379,7 +2411,7 @@ static void amg_solve_level(ufe_ctx *ctx, int l) {
   AmgLevel &V = ctx->amg[l];
   amg_cycle(ctx, l, V.b, V.e1);
   if (!V.kstep) return;
-  const AmgOp A = amg_op(V);
+  const AmgOp A = amg_op(ctx, V);
   const int nd = std::min(AMG_DOT_BLOCKS, nblk(V.n, 256));
   k_amg_matvec_dot<<<nd, 256, 0, ctx->stream>>>(A, V.e1, V.w1, V.b, nullptr, V.part);
   k_amg_kfin<<<1, 256, 0, ctx->stream>>>(nd, V.part, V.sc, 1);
@@ -2393,9 +2425,11 @@ static void amg_solve_level(ufe_ctx *ctx, int l) {
 // only touch fixed buffers, so they are captured once per mesh and smoothing
 // count and replayed: about 170 small kernels become one graph launch.
 static int amg_capture(ufe_ctx *ctx) {
-  if (ctx->amg_graph && ctx->amg_graph_nu == ctx->amg_nu) return UFE_OK;
-  if (ctx->amg_graph) (void)hipGraphExecDestroy(ctx->amg_graph);
-  ctx->amg_graph = nullptr;
+  hipGraphExec_t &G = ctx->amg_T ? ctx->amg_graph_T : ctx->amg_graph;
+  int &Gnu = ctx->amg_T ? ctx->amg_graph_T_nu : ctx->amg_graph_nu;
+  if (G && Gnu == ctx->amg_nu) return UFE_OK;
+  if (G) (void)hipGraphExecDestroy(G);
+  G = nullptr;
   hipGraph_t g = nullptr;
   HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   amg_solve_level(ctx, 1);
@@ -2413,8 +2447,8 @@ static int amg_capture(ufe_ctx *ctx) {
     ctx->err = std::string("multigrid graph instantiate: ") + hipGetErrorString(e3);
     return UFE_HIP_ERROR;
   }
-  ctx->amg_graph = ex;
-  ctx->amg_graph_nu = ctx->amg_nu;
+  G = ex;
+  Gnu = ctx->amg_nu;
   return UFE_OK;
 }
 
@@ -2437,6 +2471,42 @@ static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float
   if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), z);
   k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), z);
   for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, true, false, true);
+}
+
+// The same for an FP64 Krylov method: the cycle runs in FP32 on rounded copies
+static void amg_apply_f64(ufe_ctx *ctx, const AaAcMesh &am, float omega, const double2 *v, double2 *z) {
+  const int n = ctx->nA;
+  k_to_f32_pair<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, v, ctx->amg_v32);
+  amg_apply(ctx, am, omega, ctx->amg_v32, ctx->amg_z32);
+  k_to_f64_pair<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->amg_z32, z);
+}
+
+static void apply_AT(ufe_ctx *ctx, const AaAcMesh &mT, const ActiveRanges &ar, const double2 *x, double2 *y);
+
+// The transposed operator P^T A^T R^T of the adjoint solves: nu forward FP64
+// transposed SOR sweeps from zero (edge rows stay zero), the coarse correction
+// on the transposed hierarchy (ctx->amg_T set by the caller), nu backward
+// sweeps.
+static void amg_apply_T(ufe_ctx *ctx, const AaAcMesh &mT, double omega, const double2 *v, double2 *z) {
+  const int n = ctx->nA, nu = std::max(1, ctx->amg_nu);
+  auto sweep = [&](bool backward) {
+    for (int i = 0; i < 5; i++) {
+      const int c = backward ? 4 - i : i;
+      const int m = ctx->colour_active[c];
+      if (m == 0) continue;
+      k_sor_colour<<<xcd_grid(m), BLOCK, 0, ctx->stream>>>(mT, z, ctx->d_pair[1], v, ctx->colour_start[c], m,
+                                                         omega, nullptr, nullptr);
+    }
+  };
+  k_fill<double2><<<nblk(n), BLOCK, 0, ctx->stream>>>(z, n, make_double2(0.0, 0.0));
+  for (int k = 0; k < nu; k++) sweep(false);
+  AmgLevel &C = ctx->amg[1];
+  apply_AT(ctx, mT, active_ranges(ctx), z, ctx->amg_y64);
+  k_amg_gather64<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(C.n, C.mptr, C.mem, v, ctx->amg_y64, C.b);
+  if (ctx->amg_graph_T) (void)hipGraphLaunch(ctx->amg_graph_T, ctx->stream);
+  else amg_solve_level(ctx, 1);
+  k_amg_prolong64<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, z);
+  for (int k = 0; k < nu; k++) sweep(true);
 }
 
 // colour passes of one multigrid application (for the sweep count)
@@ -2775,8 +2845,9 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
   }
   for (int j = 0; j < m; j++) {
     T2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;
-    if (f32 && ctx->pc_amg) {
-      amg_apply(ctx, am, (float)omega, (const float2 *)vj, (float2 *)zj);
+    if (ctx->pc_amg) {
+      if (f32) amg_apply(ctx, am, (float)omega, (const float2 *)vj, (float2 *)zj);
+      else amg_apply_f64(ctx, am, (float)omega, (const double2 *)vj, (double2 *)zj);
       *cpass += amg_passes(ctx);
     } else {
       k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(zj, n, T2{T(0), T(0)});
@@ -2873,8 +2944,9 @@ static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max
     k_multi_dot_reduce<<<k, 256, 0, ctx->stream>>>(DOT_BLOCKS, k, ctx->kry_part, out, 0);
   };
   auto precond = [&](const T2 *in, T2 *out) {
-    if (f32 && ctx->pc_amg) {
-      amg_apply(ctx, am, (float)omega, (const float2 *)in, (float2 *)out);
+    if (ctx->pc_amg) {
+      if (f32) amg_apply(ctx, am, (float)omega, (const float2 *)in, (float2 *)out);
+      else amg_apply_f64(ctx, am, (float)omega, (const double2 *)in, (double2 *)out);
       return;
     }
     k_fill<T2><<<nb, BLOCK, 0, ctx->stream>>>(out, n, T2{T(0), T(0)});
@@ -2968,17 +3040,18 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
     return UFE_BAD_INPUT;
   }
   ctx->pc_amg = o->precond_type == UFE_PRECOND_MULTIGRID;
-  if (ctx->pc_amg && (!mixed || (o->method == 0 && mr == 0))) {
+  if (ctx->pc_amg && o->method == 0 && mr == 0) {
     ctx->pc_amg = false;
-    ctx->err = "the multigrid preconditioner needs precision UFE_MIXED and a Krylov method";
+    ctx->err = "the multigrid preconditioner needs a Krylov method (gmres_restart > 0 or BiCGStab)";
     return UFE_BAD_INPUT;
   }
   ctx->amg_nu = o->precond_sweeps > 0 ? o->precond_sweeps : 2;
-  if (mixed) {  // FP32 centre coefficients of the current system
+  if (mixed || ctx->pc_amg) {  // FP32 centre coefficients of the current system (sweeps, FP32 cycle)
     k_to_f32_pair<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_pair[1], ctx->d_EV32);
     LAUNCHCHK(ctx);
   }
   if (ctx->pc_amg) {
+    ctx->amg_T = false;
     if ((r = amg_setup(ctx))) return r;
     amg_refresh(ctx);
     LAUNCHCHK(ctx);
@@ -3180,6 +3253,26 @@ static int adj_solve(ufe_ctx *ctx, double tol, int m, int ks, double omega, int 
   const ActiveRanges ar = active_ranges(ctx);
   double2 *V = ctx->adj_V, *Z = ctx->adj_Z, *w = ctx->adj_w, *rr = ctx->adj_r;
   int used = 0;
+  const bool mg = ctx->adj_lin.precond_type == UFE_PRECOND_MULTIGRID;
+  if (mg) {                           // the transposed hierarchy for this system's diagonals
+    int r;
+    ctx->amg_T = false;
+    if ((r = amg_setup(ctx))) return r;
+    ctx->amg_T = true;
+    ctx->amg_nu = ks;
+    k_to_f32_pair<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->d_pair[1], ctx->d_EV32);
+    amg_refresh(ctx);
+    LAUNCHCHK(ctx);
+    r = amg_capture(ctx);
+    if (r) {
+      ctx->amg_T = false;
+      return r;
+    }
+  }
+  struct ResetT {                     // back to the forward orientation on every exit
+    ufe_ctx *c;
+    ~ResetT() { c->amg_T = false; }
+  } reset_t{ctx};
   double rel = 0.0, prev = HUGE_VAL;
   std::vector<double> H1(65 * 64), Hs(65), sc(4);
   while (true) {
@@ -3201,8 +3294,9 @@ static int adj_solve(ufe_ctx *ctx, double tol, int m, int ks, double omega, int 
     k_normalize<double><<<nb, BLOCK, 0, ctx->stream>>>(n, rr, ctx->adj_nrm + 1, V, ctx->adj_nrm + 4);
     for (int j = 0; j < m; j++) {
       double2 *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n;
-      precond_T(ctx, mT, omega, ks, vj, zj);
-      used += ks;
+      if (mg) amg_apply_T(ctx, mT, omega, vj, zj);
+      else precond_T(ctx, mT, omega, ks, vj, zj);
+      used += mg ? 2 * ks : ks;
       apply_AT(ctx, mT, ar, zj, w);
       (*matvecs)++;
       double *h1 = ctx->adj_H1 + (int64_t)j * 65;
@@ -3236,7 +3330,8 @@ int ufe_ssa_adjoint_forward(ufe_ctx *ctx, const ufe_adjoint_opts *o, ufe_adjoint
   std::memset(st, 0, sizeof(*st));
   if (o->picard_iterations < 1 || o->picard_iterations > 100000 || o->checkpoint_every < 1 ||
       o->gmres_restart < 1 || o->gmres_restart > 64 || !(o->lin_rel_tol > 0.0) ||
-      (o->precision != UFE_FP64 && o->precision != UFE_MIXED)) {
+      (o->precision != UFE_FP64 && o->precision != UFE_MIXED) ||
+      (o->precond_type != UFE_PRECOND_SOR && o->precond_type != UFE_PRECOND_MULTIGRID)) {
     ctx->err = "ufe_ssa_adjoint_forward: picard_iterations >= 1, checkpoint_every >= 1, gmres_restart in 1..64, "
                "lin_rel_tol > 0 and precision UFE_FP64 or UFE_MIXED are required";
     return UFE_BAD_INPUT;
@@ -3256,6 +3351,7 @@ int ufe_ssa_adjoint_forward(ufe_ctx *ctx, const ufe_adjoint_opts *o, ufe_adjoint
   L.omega = o->omega;
   L.gmres_restart = o->gmres_restart;
   L.precond_sweeps = o->precond_sweeps;
+  L.precond_type = o->precond_type;
   L.orth_passes = 1;
   ctx->adj_ready = false;
   auto t0 = std::chrono::steady_clock::now();
@@ -3562,6 +3658,7 @@ int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *o, double *member
   auto t0 = std::chrono::steady_clock::now();
   if (f32) k_to_f32_pair<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_pair[1], ctx->d_EV32);
   if (ctx->pc_amg) {                  // the union of the members as one hierarchy
+    ctx->amg_T = false;
     if ((r = amg_setup(ctx))) return r;
     amg_refresh(ctx);
     LAUNCHCHK(ctx);
