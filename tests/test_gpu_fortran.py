@@ -77,3 +77,35 @@ def test_fortran_host_step_matches_oracle(tmp_path):
         print(f"  {k}: rel L2 {e:.2e}", flush=True)
         assert e <= 1e-9, k
     assert np.allclose(out["dt_crit"], ice.critical_timesteps(wl.params), rtol=1e-9)
+
+
+def test_fortran_host_mixed_multigrid(tmp_path):
+    """C%precision = UFE_MIXED and C%SSA_precond_type = multigrid through the
+    Fortran configuration: the step equals the same step driven through the
+    Python boundary with the same options."""
+    exe = build.FORTRAN_EXE
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built (flang)")
+    from ufemism_amd import abi
+    from ufemism_amd.dynamics import Step
+    print("[fortran] dome 3k step, mixed + multigrid", flush=True)
+    wl = geometry.dome(3000, seed=13, picard=6)
+    write_input(tmp_path / "in.bin", wl)
+    r = subprocess.run([exe, str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), "mixed_multigrid"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    out = read_output(tmp_path / "out.bin", wl.mesh.nV)
+    ctx = abi.Context(0)
+    try:
+        step = Step(ctx, wl)
+        ctx.set_params(wl.params, abi.UFE_MIXED, linear_rel_tol=1e-10, precond_type=abi.UFE_PRECOND_MULTIGRID)
+        ref = step.run()
+        print(f"  fortran sweeps {out['sweeps']} python sweeps {ref['sweeps']}", flush=True)
+        for k in ("U_SSA", "V_SSA", "Hi"):
+            e = np.linalg.norm(out[k] - ref[k]) / max(np.linalg.norm(ref[k]), 1e-300)
+            print(f"  {k}: rel L2 {e:.2e}", flush=True)
+            assert e <= 1e-7, k
+        assert ctx.multigrid_info()["levels"] >= 2
+    finally:
+        ctx.close()

@@ -12,13 +12,17 @@ PROGRAM ufe_step_driver
   TYPE(type_ice_model) :: ice
   TYPE(type_SMB_model) :: SMB
   TYPE(type_BMB_model) :: BMB
-  CHARACTER(LEN=1024) :: fin, fout
+  CHARACTER(LEN=1024) :: fin, fout, mode
   INTEGER :: u, nV, nTri, nC_mem, nZ, k, nmax_out, nmax_in
   INTEGER, ALLOCATABLE :: mask_noice(:)
   REAL(dp) :: A_flow, tau_c, dt, hdr(4), pr(5), dt_crit(3)
 
   CALL GET_COMMAND_ARGUMENT(1, fin)
   CALL GET_COMMAND_ARGUMENT(2, fout)
+  ! optional third argument: 'mixed_multigrid' solves every linearised SSA
+  ! system to 1e-10 by mixed GMRES-IR with the multigrid preconditioner
+  mode = ''
+  IF (COMMAND_ARGUMENT_COUNT() >= 3) CALL GET_COMMAND_ARGUMENT(3, mode)
   OPEN(NEWUNIT=u, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
   READ(u) nV, nTri, nC_mem, nZ
   READ(u) hdr
@@ -45,6 +49,11 @@ PROGRAM ufe_step_driver
   READ(u) nmax_out, nmax_in
   C%SSA_max_outer_loops = nmax_out
   C%SSA_max_inner_loops = nmax_in
+  IF (TRIM(mode) == 'mixed_multigrid') THEN
+    C%precision = 1
+    C%SSA_precond_type = 1
+    C%SSA_linear_rel_tol = 1E-10_dp
+  END IF
 
   CALL initialise_mesh_dynamics(mesh)
   CALL allocate_ice_model(mesh, ice)
