@@ -2465,8 +2465,15 @@ static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float
   k_amg_gather<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(C.n, C.mptr, C.mem, v, ctx->amg_y, C.b);
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   (void)hipStreamIsCapturing(ctx->stream, &cap);
-  if (ctx->amg_graph && cap == hipStreamCaptureStatusNone) (void)hipGraphLaunch(ctx->amg_graph, ctx->stream);
-  else amg_solve_level(ctx, 1);     // eager, or recorded into the caller's graph
+  // eager launches when the stream is being captured (recorded into the
+  // caller's graph) or the graph launch is refused (the same kernels; the
+  // refusal is cleared so that the launch checks after the cycle pass)
+  bool launched = false;
+  if (ctx->amg_graph && cap == hipStreamCaptureStatusNone) {
+    launched = hipGraphLaunch(ctx->amg_graph, ctx->stream) == hipSuccess;
+    if (!launched) (void)hipGetLastError();
+  }
+  if (!launched) amg_solve_level(ctx, 1);
   k_amg_prolong<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, z);
   if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), z);
   k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), z);
@@ -2503,8 +2510,12 @@ static void amg_apply_T(ufe_ctx *ctx, const AaAcMesh &mT, double omega, const do
   AmgLevel &C = ctx->amg[1];
   apply_AT(ctx, mT, active_ranges(ctx), z, ctx->amg_y64);
   k_amg_gather64<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(C.n, C.mptr, C.mem, v, ctx->amg_y64, C.b);
-  if (ctx->amg_graph_T) (void)hipGraphLaunch(ctx->amg_graph_T, ctx->stream);
-  else amg_solve_level(ctx, 1);
+  bool launched = false;
+  if (ctx->amg_graph_T) {
+    launched = hipGraphLaunch(ctx->amg_graph_T, ctx->stream) == hipSuccess;
+    if (!launched) (void)hipGetLastError();
+  }
+  if (!launched) amg_solve_level(ctx, 1);   // the same kernels, eager
   k_amg_prolong64<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, z);
   for (int k = 0; k < nu; k++) sweep(true);
 }
