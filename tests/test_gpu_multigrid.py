@@ -167,3 +167,31 @@ def test_multigrid_on_a_grounded_dome():
     finally:
         ctx.close()
 
+
+
+@pytest.mark.parametrize("n", [150, 300])
+def test_multigrid_on_tiny_meshes(n):
+    """Meshes so small that level 1 is (nearly) the dense coarsest level: the
+    two-level and three-level hierarchies solve the Picard loop like FP64 SOR
+    preconditioning does."""
+    ctx = abi.Context(0)
+    try:
+        wl = geometry.dome(n, seed=2, picard=4)
+        step = Step(ctx, wl)
+        out = {}
+        for mode, pc in ((abi.UFE_FP64_KRYLOV, abi.UFE_PRECOND_SOR), (abi.UFE_MIXED, abi.UFE_PRECOND_MULTIGRID)):
+            step.reset()
+            for f in ("U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac", "N_AaAc"):
+                ctx.fill(f, 0.0)
+            ctx.set_params(wl.params, mode, linear_rel_tol=1e-10, precond_type=pc)
+            r = step.run()
+            assert r["stats"].status == 0, r["stats"].status
+            out[mode] = r
+        info = ctx.multigrid_info()
+        print(f"  dome nV={wl.mesh.nV}: hierarchy {info['points']} direct {info['direct']}", flush=True)
+        assert info["direct"] and info["levels"] >= 2
+        a, b = out[abi.UFE_FP64_KRYLOV], out[abi.UFE_MIXED]
+        for k in ("U_SSA", "V_SSA", "Hi"):
+            assert _rel(b[k], a[k]) <= 1e-7, (k, _rel(b[k], a[k]))
+    finally:
+        ctx.close()
