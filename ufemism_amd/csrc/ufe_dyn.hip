@@ -20,6 +20,7 @@
 #include <hip/hip_ext.h>
 
 #include "ufe_kernels.inc"
+#include "amg_host.h"
 
 namespace ufe {
 
@@ -2051,89 +2052,6 @@ constexpr float AMG_BJ_OMEGA = 0.8f;     // damping of the coarse-level block Ja
 constexpr int AMG_COARSE_SWEEPS = 20;    // coarsest level too large for the dense inverse
 constexpr int AMG_MAX_LEVELS = 16;
 
-struct AmgHostOp {              // a point operator: off-diagonal pairs, fixed diagonal pairs, u-v coupling
-  int n = 0;
-  std::vector<int32_t> ptr, col;
-  std::vector<double2> cf, fd;
-  std::vector<double> S;
-};
-
-// Greedy aggregation: a point whose neighbours are all free takes them into a
-// new aggregate; every point left joins the aggregate of its first aggregated
-// neighbour (or forms its own).
-static std::vector<int32_t> amg_aggregate(const AmgHostOp &A, int *na) {
-  std::vector<int32_t> agg(A.n, -1);
-  int k = 0;
-  for (int i = 0; i < A.n; i++) {
-    if (agg[i] >= 0) continue;
-    bool free = true;
-    for (int q = A.ptr[i]; q < A.ptr[i + 1] && free; q++) free = agg[A.col[q]] < 0;
-    if (!free) continue;
-    agg[i] = k;
-    for (int q = A.ptr[i]; q < A.ptr[i + 1]; q++) agg[A.col[q]] = k;
-    k++;
-  }
-  for (int i = 0; i < A.n; i++) {
-    if (agg[i] >= 0) continue;
-    int a = -1;
-    for (int q = A.ptr[i]; q < A.ptr[i + 1] && a < 0; q++) a = agg[A.col[q]];
-    agg[i] = a >= 0 ? a : k++;
-  }
-  *na = k;
-  return agg;
-}
-
-// Galerkin product P^T A P for piecewise-constant P, and the member lists.
-// C.fd holds the whole fixed diagonal part (the members' fixed parts plus the
-// couplings inside each aggregate); intra the couplings inside alone, which is
-// what a level adds to the sum of its members' full diagonals.
-static void amg_galerkin(const AmgHostOp &A, const std::vector<int32_t> &agg, int na, AmgHostOp &C,
-                         std::vector<int32_t> &mptr, std::vector<int32_t> &mem, std::vector<double2> &intra) {
-  mptr.assign(na + 1, 0);
-  for (int i = 0; i < A.n; i++) mptr[agg[i] + 1]++;
-  for (int I = 0; I < na; I++) mptr[I + 1] += mptr[I];
-  mem.resize(A.n);
-  {
-    std::vector<int32_t> cur(mptr.begin(), mptr.end() - 1);
-    for (int i = 0; i < A.n; i++) mem[cur[agg[i]]++] = i;
-  }
-  C.n = na;
-  C.ptr.assign(1, 0);
-  C.col.clear();
-  C.cf.clear();
-  C.fd.assign(na, make_double2(0.0, 0.0));
-  C.S.assign(na, 0.0);
-  intra.assign(na, make_double2(0.0, 0.0));
-  std::vector<int64_t> mark(na, -1);
-  for (int I = 0; I < na; I++) {
-    const int64_t row0 = (int64_t)C.col.size();
-    for (int k = mptr[I]; k < mptr[I + 1]; k++) {
-      const int i = mem[k];
-      C.fd[I].x += A.fd[i].x;
-      C.fd[I].y += A.fd[i].y;
-      C.S[I] += A.S[i];
-      for (int q = A.ptr[i]; q < A.ptr[i + 1]; q++) {
-        const int J = agg[A.col[q]];
-        const double2 c = A.cf[q];
-        if (J == I) {
-          C.fd[I].x += c.x;
-          C.fd[I].y += c.y;
-          intra[I].x += c.x;
-          intra[I].y += c.y;
-        } else if (mark[J] >= row0) {
-          C.cf[mark[J]].x += c.x;
-          C.cf[mark[J]].y += c.y;
-        } else {
-          mark[J] = (int64_t)C.col.size();
-          C.col.push_back(J);
-          C.cf.push_back(c);
-        }
-      }
-    }
-    C.ptr.push_back((int32_t)C.col.size());
-  }
-}
-
 static int amg_setup(ufe_ctx *ctx) {
   if (ctx->amg_ready) return UFE_OK;
   const auto t0 = std::chrono::steady_clock::now();
@@ -2208,7 +2126,7 @@ static int amg_setup(ufe_ctx *ctx) {
     AmgHostOp &A = ops[0];
     A.n = n0;
     A.ptr.assign(1, 0);
-    A.fd.assign(n0, make_double2(0.0, 0.0));
+    A.fd.assign(n0, AmgPair{0.0, 0.0});
     A.S.resize(n0);
     std::vector<int64_t> mark(n0, -1);
     for (int i = 0; i < n0; i++) {
@@ -2224,7 +2142,7 @@ static int amg_setup(ufe_ctx *ctx) {
         } else {
           mark[j] = (int64_t)A.col.size();
           A.col.push_back(j);
-          A.cf.push_back(make_double2(cx, cy));
+          A.cf.push_back(AmgPair{cx, cy});
         }
       };
       for (int s = 0; s < deg[p]; s++) {
@@ -2241,14 +2159,14 @@ static int amg_setup(ufe_ctx *ctx) {
   }
   // coarsen until the dense inverse fits (at least one coarse level)
   std::vector<std::vector<int32_t>> aggs, mptrs, mems;
-  std::vector<std::vector<double2>> intras;
+  std::vector<std::vector<AmgPair>> intras;
   while ((int)ops.size() < AMG_MAX_LEVELS && (ops.size() == 1 || ops.back().n > AMG_DIRECT)) {
     int na = 0;
     std::vector<int32_t> agg = amg_aggregate(ops.back(), &na);
     if (ops.size() > 1 && na > (int)(0.85 * ops.back().n)) break;   // coarsening stalled
     AmgHostOp C;
     std::vector<int32_t> mptr, mem;
-    std::vector<double2> intra;
+    std::vector<AmgPair> intra;
     amg_galerkin(ops.back(), agg, na, C, mptr, mem, intra);
     intras.push_back(std::move(intra));
     aggs.push_back(std::move(agg));
@@ -2274,18 +2192,14 @@ static int amg_setup(ufe_ctx *ctx) {
     if ((r = dupload(ctx, &V.ptr, C.ptr.data(), C.ptr.size()))) return r;
     if ((r = dupload(ctx, &V.col, colv.data(), colv.size()))) return r;
     if ((r = dupload(ctx, &V.cf, cf32.data(), cf32.size()))) return r;
-    {   // the transposed couplings: the coefficient of I in J's row becomes J's in I's
-      std::vector<int32_t> pT(C.n + 1, 0), cT(std::max<size_t>(C.col.size(), 1), 0);
-      std::vector<float2> fT(std::max<size_t>(C.col.size(), 1), make_float2(0.0f, 0.0f));
-      for (size_t k = 0; k < C.col.size(); k++) pT[C.col[k] + 1]++;
-      for (int i = 0; i < C.n; i++) pT[i + 1] += pT[i];
-      std::vector<int32_t> cur(pT.begin(), pT.end() - 1);
-      for (int i = 0; i < C.n; i++)
-        for (int k = C.ptr[i]; k < C.ptr[i + 1]; k++) {
-          const int q = cur[C.col[k]]++;
-          cT[q] = i;
-          fT[q] = cf32[k];
-        }
+    {   // the transposed couplings (adjoint solves)
+      std::vector<int32_t> pT, cT;
+      std::vector<float2> fT;
+      amg_transpose(C.n, C.ptr, C.col, cf32, pT, cT, fT);
+      if (cT.empty()) {
+        cT.push_back(0);
+        fT.push_back(make_float2(0.0f, 0.0f));
+      }
       if ((r = dupload(ctx, &V.ptrT, pT.data(), pT.size()))) return r;
       if ((r = dupload(ctx, &V.colT, cT.data(), cT.size()))) return r;
       if ((r = dupload(ctx, &V.cfT, fT.data(), fT.size()))) return r;
@@ -2295,8 +2209,9 @@ static int amg_setup(ufe_ctx *ctx) {
     // 1 the fine fixed parts (Neumann self-couplings) as well, since the fine
     // diagonal pairs EV do not hold them; below, the couplings inside each
     // aggregate only (the members' diagonals already hold their fixed parts)
-    const std::vector<double2> &own = l == 1 ? C.fd : intras[l - 1];
-    if ((r = dupload(ctx, &V.fd, own.data(), own.size()))) return r;
+    const std::vector<AmgPair> &own = l == 1 ? C.fd : intras[l - 1];
+    static_assert(sizeof(AmgPair) == sizeof(double2), "AmgPair must have the layout of double2");
+    if ((r = dupload(ctx, &V.fd, reinterpret_cast<const double2 *>(own.data()), own.size()))) return r;
     if ((r = dalloc(ctx, &V.ev64, (size_t)C.n))) return r;
     if ((r = dalloc(ctx, &V.ev32, (size_t)C.n))) return r;
     // members and aggregate map: level 1 in fine rows, deeper levels in points
@@ -2327,9 +2242,9 @@ static int amg_setup(ufe_ctx *ctx) {
   ctx->amg_direct = Lc.n <= AMG_DIRECT;
   if (ctx->amg_direct) {
     const AmgHostOp &C = ops[L - 1];
-    std::vector<double2> cf = C.cf;
-    if (cf.empty()) cf.push_back(make_double2(0.0, 0.0));
-    if ((r = dupload(ctx, &Lc.cf64, cf.data(), cf.size()))) return r;
+    std::vector<AmgPair> cf = C.cf;
+    if (cf.empty()) cf.push_back(AmgPair{0.0, 0.0});
+    if ((r = dupload(ctx, &Lc.cf64, reinterpret_cast<const double2 *>(cf.data()), cf.size()))) return r;
     if ((r = dupload(ctx, &Lc.S64, C.S.data(), C.S.size()))) return r;
     const size_t N = 2 * (size_t)Lc.n;
     if ((r = dalloc(ctx, &ctx->amg_W, N * 2 * N))) return r;
