@@ -86,7 +86,7 @@ typedef struct ufe_params {
   double SSA_SOR_omega;
   int32_t SSA_max_inner_loops;
   int32_t use_analytical_GL_flux;   /* calculate_GL_flux (:848-949) prescribes the GL Ac velocities,
-                                       which SOR skips (:624-631); reference SOR (UFE_FP64) only */
+                                       which SOR skips (:624-631) and the Krylov modes hold fixed */
   int32_t ssa_forced_zero;          /* EISMINT/Halfar/Bueler: SSA velocities zero (:431-449) */
   int32_t zero_Hi_at_edges;         /* ice-thickness edge condition (:189-229) */
   int32_t use_tau_c_override;       /* tau_c := field "tau_c_override_AaAc" after basal_yield_stress */

@@ -10,8 +10,12 @@ solves (UFE_PRECOND_MULTIGRID) on the 20k marine system:
 * solve_SSA's Picard loop with SSA_precond_type = multigrid agrees with the
   FP64 Krylov step, on the marine sheet and on a grounded dome;
 * FP64 Krylov vectors around the FP32 cycle reach 1e-11; plain SOR with the
-  multigrid preconditioner is refused.
+  multigrid preconditioner is refused;
+* with the grounding-line flux on, the Krylov modes keep the prescribed rows
+  and agree with each other and with the reference SOR step.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -167,6 +171,54 @@ def test_multigrid_on_a_grounded_dome():
     finally:
         ctx.close()
 
+
+def test_gl_flux_with_the_krylov_modes():
+    """use_analytical_GL_flux with the Krylov solves of solve_SSA: the rows that
+    calculate_GL_flux prescribes keep those values bit for bit (the reference
+    SOR skips them, src/ice_dynamics_module.f90:627-631); FP64 Krylov, mixed
+    GMRES-IR with SOR sweeps and mixed GMRES-IR with the multigrid agree to
+    1e-7; and all three agree with the reference SOR step run to
+    max|dU| < 1e-4 m/yr."""
+    print("[gl krylov] marine20k with the GL flux: reference SOR, fp64 Krylov, mixed sor, mixed multigrid",
+          flush=True)
+    ctx = abi.Context(0)
+    try:
+        wl = geometry.marine(20000, seed=5, picard=10)
+        wl.A_flow = 1e-18
+        wl.params.use_analytical_GL_flux = True
+        step = Step(ctx, wl)
+        tight = dataclasses.replace(wl.params, SSA_max_residual_UV=1e-4, SSA_max_inner_loops=1_000_000)
+        runs = (("sor", tight, abi.UFE_FP64, abi.UFE_PRECOND_SOR),
+                ("fp64_krylov", wl.params, abi.UFE_FP64_KRYLOV, abi.UFE_PRECOND_SOR),
+                ("mixed_sor", wl.params, abi.UFE_MIXED, abi.UFE_PRECOND_SOR),
+                ("mixed_multigrid", wl.params, abi.UFE_MIXED, abi.UFE_PRECOND_MULTIGRID))
+        out, gl = {}, {}
+        for name, params, mode, pc in runs:
+            step.reset()
+            ctx.set_params(params, mode, linear_rel_tol=1e-10, precond_type=pc)
+            r = step.run()
+            assert r["stats"].status == 0, (name, r["stats"].status)
+            mask = ctx.get("mask_gl_Ac") == 1
+            out[name] = r
+            gl[name] = (mask, r["Ux_SSA_Ac"][mask], r["Uy_SSA_Ac"][mask])
+            print(f"  {name}: picard {r['picard']} sweeps {sum(r['sweeps'])} GL rows {int(mask.sum())}", flush=True)
+        m0, ux0, uy0 = gl["sor"]
+        assert m0.sum() > 20
+        for name in gl:
+            m, ux, uy = gl[name]
+            assert np.array_equal(m, m0) and np.array_equal(ux, ux0) and np.array_equal(uy, uy0), name
+        a = out["fp64_krylov"]
+        for name in ("mixed_sor", "mixed_multigrid"):
+            for k in ("U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac", "Hi"):
+                e = _rel(out[name][k], a[k])
+                print(f"  {name} {k}: vs fp64 Krylov rel L2 {e:.2e}", flush=True)
+                assert e <= 1e-7, (name, k, e)
+        for k in ("U_SSA", "V_SSA", "Hi"):
+            e = _rel(a[k], out["sor"][k])
+            print(f"  fp64 Krylov {k}: vs reference SOR rel L2 {e:.2e}", flush=True)
+            assert e <= 1e-3, (k, e)
+    finally:
+        ctx.close()
 
 
 @pytest.mark.parametrize("n", [150, 300])

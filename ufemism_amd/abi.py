@@ -339,7 +339,7 @@ class Context:
 
     def set_params(self, p: DynParams, precision=UFE_FP64, sweep_batch=0, **krylov):
         """krylov: linear_rel_tol, gmres_restart, precond_sweeps, precond_type (UFE_MIXED /
-        UFE_FP64_KRYLOV; precond_type UFE_PRECOND_MULTIGRID with UFE_MIXED only)."""
+        UFE_FP64_KRYLOV; both take precond_type UFE_PRECOND_SOR or UFE_PRECOND_MULTIGRID)."""
         self._params = params_struct(p, precision, sweep_batch, **krylov)
         self._chk(self.lib.ufe_set_params(self.h, ct.byref(self._params)), "ufe_set_params")
 

@@ -2010,6 +2010,7 @@ static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, flo
   q.cU = ctx->d_cU32;
   q.Sxy = ctx->d_Sxy32;
   q.EV = ctx->d_EV32;
+  q.skip = ctx->gl_active ? ctx->d_skip : nullptr;
   if (!D) D = ctx->d_D32;
   if (!R) R = ctx->d_R32;
   for (int k = skip_first ? 1 : 0; k < 5; k++) {
@@ -2032,7 +2033,7 @@ static void enqueue_sweep_fp64_vec(ufe_ctx *ctx, const AaAcMesh &m, double omega
     if (n == 0) continue;
     timed_kernel(ctx, ctx->bytes64[c], n, k_sor_colour, dim3(xcd_grid(n)), dim3(BLOCK), m, D,
                  (const double2 *)ctx->d_pair[1], R, ctx->colour_start[c], n, omega, (SorFlags *)nullptr,
-                 (const uint8_t *)nullptr);
+                 (const uint8_t *)(ctx->gl_active ? ctx->d_skip : nullptr));
   }
   if (!neumann) return;
   if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D, nullptr);
@@ -2313,7 +2314,7 @@ static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
   k_amg_residual<<<nb, BLOCK, 0, ctx->stream>>>(A, cur, b, V.r);
   k_amg_gather<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(C.n, C.mptr, C.mem, V.r, nullptr, C.b);
   amg_solve_level(ctx, l + 1);
-  k_amg_prolong<<<nb, BLOCK, 0, ctx->stream>>>(V.n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, cur);
+  k_amg_prolong<<<nb, BLOCK, 0, ctx->stream>>>(V.n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, cur, nullptr);
   for (int s = 0; s < nu; s++) {
     k_amg_bj<<<nb, BLOCK, 0, ctx->stream>>>(A, b, cur, oth, AMG_BJ_OMEGA);
     std::swap(cur, oth);
@@ -2389,7 +2390,8 @@ static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float
     if (!launched) (void)hipGetLastError();
   }
   if (!launched) amg_solve_level(ctx, 1);
-  k_amg_prolong<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, z);
+  k_amg_prolong<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, z,
+                                                   ctx->gl_active ? ctx->d_skip : nullptr);
   if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), z);
   k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), z);
   for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, true, false, true);
@@ -2520,10 +2522,6 @@ int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) {
   if (!st) st = &local;
   std::memset(st, 0, sizeof(*st));
   const ufe_params &P = ctx->params;
-  if (P.use_analytical_GL_flux && P.precision != UFE_FP64) {
-    ctx->err = "use_analytical_GL_flux runs with the reference SOR (precision UFE_FP64)";
-    return st->status = UFE_NOT_SUPPORTED;
-  }
   auto t0 = std::chrono::steady_clock::now();
   // gating (:430-465)
   HIPCHK(ctx, hipMemsetAsync(ctx->d_count, 0, sizeof(unsigned long long), ctx->stream));
@@ -2540,7 +2538,10 @@ int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) {
     return UFE_OK;
   }
   if ((r = ssa_gather(ctx))) return r;
-  // calculate_GL_flux (:471) prescribes the GL Ac velocities; SOR then skips those rows
+  // calculate_GL_flux (:471) prescribes the GL Ac velocities; SOR then skips
+  // those rows (:627-631).  The Krylov solves treat them as fixed: their
+  // residual and correction are zero, and every preconditioner leaves them
+  // at zero, so the corrections never move the prescribed values.
   struct GLGuard {
     ufe_ctx *c;
     ~GLGuard() { c->gl_active = false; }
@@ -2634,6 +2635,7 @@ int ufe_ssa_prepare(ufe_ctx *ctx) {
 // FP64 relative residual; with store=true also hands r to the FP32 sweeps.
 static int residual_fp64(ufe_ctx *ctx, double *rel, bool store = false) {
   const AaAcMesh m = aaac_mesh(ctx);
+  const uint8_t *skip = ctx->gl_active ? ctx->d_skip : nullptr;
   int nb_tot = 0;
   for (int c = 0; c < 5; c++) {
     const int n = ctx->colour_active[c];
@@ -2641,10 +2643,11 @@ static int residual_fp64(ufe_ctx *ctx, double *rel, bool store = false) {
     const int nb = nblk(n);
     if (store)
       k_residual_store<<<nb, BLOCK, 0, ctx->stream>>>(m, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
-                                                       ctx->colour_start[c], n, ctx->d_R32, ctx->d_part + 2 * nb_tot);
+                                                       ctx->colour_start[c], n, ctx->d_R32, ctx->d_part + 2 * nb_tot,
+                                                       skip);
     else
       k_ssa_residual<<<nb, BLOCK, 0, ctx->stream>>>(m, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
-                                                     ctx->colour_start[c], n, ctx->d_part + 2 * nb_tot);
+                                                     ctx->colour_start[c], n, ctx->d_part + 2 * nb_tot, skip);
     nb_tot += nb;
   }
   k_ssa_rn<<<1, 1024, 0, ctx->stream>>>(nb_tot, ctx->d_part, ctx->d_scalar);
@@ -2667,6 +2670,7 @@ static ActiveRanges active_ranges(ufe_ctx *ctx) {
     a.start[c] = ctx->colour_start[c];
     a.count[c] = ctx->colour_active[c];
   }
+  a.skip = ctx->gl_active ? ctx->d_skip : nullptr;
   return a;
 }
 
