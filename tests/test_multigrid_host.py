@@ -10,7 +10,9 @@ unequal couplings in the two directions:
 * the fixed diagonal parts add up (members' parts plus the couplings inside
   the aggregate) and the u-v couplings are the members' sums;
 * the transposed couplings give y.(A x) = x.(A^T y);
-* a second level coarsens the first in the same way.
+* a second level coarsens the first in the same way;
+* on a grid large enough for the threaded Galerkin product, 1, 3 and 7 host
+  threads give the same coarse operator bit for bit.
 """
 import os
 import shutil
@@ -140,12 +142,43 @@ static void check_level(const AmgHostOp &A, std::mt19937 &g, int level, AmgHostO
               *std::max_element(size.begin(), size.end()));
 }
 
+static bool same(const AmgHostOp &a, const AmgHostOp &b) {
+  auto eq = [](const std::vector<AmgPair> &x, const std::vector<AmgPair> &y) {
+    if (x.size() != y.size()) return false;
+    for (size_t k = 0; k < x.size(); k++)
+      if (x[k].x != y[k].x || x[k].y != y[k].y) return false;
+    return true;
+  };
+  return a.n == b.n && a.ptr == b.ptr && a.col == b.col && a.S == b.S && eq(a.cf, b.cf) && eq(a.fd, b.fd);
+}
+
 int main() {
   std::mt19937 g(12345);
   for (int rep = 0; rep < 3; rep++) {
     AmgHostOp A = grid(37 + 11 * rep, 29 + 5 * rep, g), C1, C2;
     check_level(A, g, 1, C1);
     check_level(C1, g, 2, C2);
+  }
+  // a grid large enough for the threaded Galerkin product: the identities hold
+  // and the coarse operator is the same, bit for bit, for 1, 3 and 7 parts
+  {
+    AmgHostOp A = grid(331, 277, g), C1;
+    check_level(A, g, 1, C1);
+    int na = 0;
+    const std::vector<int32_t> agg = amg_aggregate(A, &na);
+    AmgHostOp ref;
+    std::vector<int32_t> mptr0, mem0;
+    std::vector<AmgPair> intra0;
+    amg_galerkin(A, agg, na, ref, mptr0, mem0, intra0, 1);
+    for (int parts : {3, 7}) {
+      AmgHostOp C;
+      std::vector<int32_t> mptr, mem;
+      std::vector<AmgPair> intra;
+      amg_galerkin(A, agg, na, C, mptr, mem, intra, parts);
+      bool ok = same(C, ref) && mptr == mptr0 && mem == mem0 && intra.size() == intra0.size();
+      for (size_t k = 0; ok && k < intra.size(); k++) ok = intra[k].x == intra0[k].x && intra[k].y == intra0[k].y;
+      CHECK(ok, "Galerkin product with %d parts differs from one part", parts);
+    }
   }
   std::printf(fails ? "FAILED %d\n" : "OK\n", fails);
   return fails ? 1 : 0;
@@ -161,7 +194,7 @@ def test_multigrid_host_logic(tmp_path):
     src.write_text(DRIVER.replace('#include <vector>\n#include "amg_host.h"',
                                   '#include <vector>\n#include <algorithm>\n#include "amg_host.h"'))
     exe = tmp_path / "amg_host_check"
-    subprocess.run([cxx, "-O2", "-std=c++17", "-Wall", "-I", CSRC, "-o", str(exe), str(src)], check=True,
+    subprocess.run([cxx, "-O2", "-std=c++17", "-Wall", "-pthread", "-I", CSRC, "-o", str(exe), str(src)], check=True,
                    capture_output=True, text=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     print(r.stdout)

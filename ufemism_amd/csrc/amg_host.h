@@ -3,10 +3,55 @@
 // Plain C++ (no HIP types) so that the CPU tests compile it on their own.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
+#include <thread>
 #include <vector>
 
 namespace ufe {
+
+// Host threads for a set-up pass over n rows: one below 65,536 rows, else the
+// hardware's count (at most 16).
+inline int amg_threads(int64_t n) {
+  if (n < 65536) return 1;
+  const unsigned h = std::thread::hardware_concurrency();
+  return h ? (int)std::min(h, 16u) : 1;
+}
+
+// body(t, begin, end) on `parts` contiguous ranges of [0, n), one thread each
+template <typename F>
+inline void amg_parts(int n, int parts, F body) {
+  if (parts <= 1) {
+    body(0, 0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(parts);
+  for (int t = 0; t < parts; t++)
+    th.emplace_back(body, t, (int)((int64_t)n * t / parts), (int)((int64_t)n * (t + 1) / parts));
+  for (auto &x : th) x.join();
+}
+
+// Rows built in parts: part t holds the rows of its range one after another,
+// rowlen[i] the length of row i.  Joins them into CSR ptr, col, cf.
+template <typename T>
+inline void amg_join_rows(const std::vector<int32_t> &rowlen, std::vector<std::vector<int32_t>> &pc,
+                          std::vector<std::vector<T>> &pf, std::vector<int32_t> &ptr, std::vector<int32_t> &col,
+                          std::vector<T> &cf) {
+  const int n = (int)rowlen.size();
+  ptr.assign(n + 1, 0);
+  for (int i = 0; i < n; i++) ptr[i + 1] = ptr[i] + rowlen[i];
+  col.clear();
+  cf.clear();
+  col.reserve(ptr[n]);
+  cf.reserve(ptr[n]);
+  for (size_t t = 0; t < pc.size(); t++) {
+    col.insert(col.end(), pc[t].begin(), pc[t].end());
+    cf.insert(cf.end(), pf[t].begin(), pf[t].end());
+    std::vector<int32_t>().swap(pc[t]);
+    std::vector<T>().swap(pf[t]);
+  }
+}
 
 struct AmgPair {                // {u-u, v-v} coefficient pair; the layout of a double2
   double x, y;
@@ -48,8 +93,11 @@ inline std::vector<int32_t> amg_aggregate(const AmgHostOp &A, int *na) {
 // C.fd holds the whole fixed diagonal part (the members' fixed parts plus the
 // couplings inside each aggregate); intra the couplings inside alone, which is
 // what a level adds to the sum of its members' full diagonals.
+// parts: host threads for the coarse rows (0: amg_threads); the result does
+// not depend on it.
 inline void amg_galerkin(const AmgHostOp &A, const std::vector<int32_t> &agg, int na, AmgHostOp &C,
-                         std::vector<int32_t> &mptr, std::vector<int32_t> &mem, std::vector<AmgPair> &intra) {
+                         std::vector<int32_t> &mptr, std::vector<int32_t> &mem, std::vector<AmgPair> &intra,
+                         int parts = 0) {
   mptr.assign(na + 1, 0);
   for (int i = 0; i < A.n; i++) mptr[agg[i] + 1]++;
   for (int I = 0; I < na; I++) mptr[I + 1] += mptr[I];
@@ -59,40 +107,51 @@ inline void amg_galerkin(const AmgHostOp &A, const std::vector<int32_t> &agg, in
     for (int i = 0; i < A.n; i++) mem[cur[agg[i]]++] = i;
   }
   C.n = na;
-  C.ptr.assign(1, 0);
-  C.col.clear();
-  C.cf.clear();
   C.fd.assign(na, AmgPair{0.0, 0.0});
   C.S.assign(na, 0.0);
   intra.assign(na, AmgPair{0.0, 0.0});
-  std::vector<int64_t> mark(na, -1);
-  for (int I = 0; I < na; I++) {
-    const int64_t row0 = (int64_t)C.col.size();
-    for (int k = mptr[I]; k < mptr[I + 1]; k++) {
-      const int i = mem[k];
-      C.fd[I].x += A.fd[i].x;
-      C.fd[I].y += A.fd[i].y;
-      C.S[I] += A.S[i];
-      for (int q = A.ptr[i]; q < A.ptr[i + 1]; q++) {
-        const int J = agg[A.col[q]];
-        const AmgPair c = A.cf[q];
-        if (J == I) {
-          C.fd[I].x += c.x;
-          C.fd[I].y += c.y;
-          intra[I].x += c.x;
-          intra[I].y += c.y;
-        } else if (mark[J] >= row0) {
-          C.cf[mark[J]].x += c.x;
-          C.cf[mark[J]].y += c.y;
-        } else {
-          mark[J] = (int64_t)C.col.size();
-          C.col.push_back(J);
-          C.cf.push_back(c);
+  // coarse rows in parts; within a row the entries keep the order of first
+  // appearance and the sums the members' order, whatever the number of parts
+  if (parts <= 0) parts = amg_threads(A.n);
+  parts = std::max(1, std::min(parts, na));
+  std::vector<int32_t> rowlen(na);
+  std::vector<std::vector<int32_t>> pc(parts);
+  std::vector<std::vector<AmgPair>> pf(parts);
+  amg_parts(na, parts, [&](int t, int I0, int I1) {
+    std::vector<int32_t> col;            // local, not pc[t]: the parts' vector headers share cache lines
+    std::vector<AmgPair> cf;
+    std::vector<int32_t> mark(na, -1);   // position in this part's col of J in the current row
+    for (int I = I0; I < I1; I++) {
+      const int32_t row0 = (int32_t)col.size();
+      for (int k = mptr[I]; k < mptr[I + 1]; k++) {
+        const int i = mem[k];
+        C.fd[I].x += A.fd[i].x;
+        C.fd[I].y += A.fd[i].y;
+        C.S[I] += A.S[i];
+        for (int q = A.ptr[i]; q < A.ptr[i + 1]; q++) {
+          const int J = agg[A.col[q]];
+          const AmgPair c = A.cf[q];
+          if (J == I) {
+            C.fd[I].x += c.x;
+            C.fd[I].y += c.y;
+            intra[I].x += c.x;
+            intra[I].y += c.y;
+          } else if (mark[J] >= row0) {
+            cf[mark[J]].x += c.x;
+            cf[mark[J]].y += c.y;
+          } else {
+            mark[J] = (int32_t)col.size();
+            col.push_back(J);
+            cf.push_back(c);
+          }
         }
       }
+      rowlen[I] = (int32_t)col.size() - row0;
     }
-    C.ptr.push_back((int32_t)C.col.size());
-  }
+    pc[t] = std::move(col);
+    pf[t] = std::move(cf);
+  });
+  amg_join_rows(rowlen, pc, pf, C.ptr, C.col, C.cf);
 }
 
 // The transposed couplings of C: the coefficient of I in J's row becomes J's

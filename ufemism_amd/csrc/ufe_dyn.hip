@@ -2122,41 +2122,55 @@ static int amg_setup(ufe_ctx *ctx) {
     exl.push_back(std::move(e));
   }
   // the operator on the fine points with the Neumann rows eliminated
+  // (rows in parts on host threads; a row's entries keep the order of first
+  // appearance, found by a search of the row so far, which is short)
   std::vector<AmgHostOp> ops(1);
   {
     AmgHostOp &A = ops[0];
     A.n = n0;
-    A.ptr.assign(1, 0);
     A.fd.assign(n0, AmgPair{0.0, 0.0});
     A.S.resize(n0);
-    std::vector<int64_t> mark(n0, -1);
-    for (int i = 0; i < n0; i++) {
-      const int p = rows[i];
-      const int64_t base = cptr[p / SELL] + (p % SELL), row0 = (int64_t)A.col.size();
-      auto add = [&](int j, double cx, double cy) {
-        if (j == i) {
-          A.fd[i].x += cx;
-          A.fd[i].y += cy;
-        } else if (mark[j] >= row0) {
-          A.cf[mark[j]].x += cx;
-          A.cf[mark[j]].y += cy;
-        } else {
-          mark[j] = (int64_t)A.col.size();
-          A.col.push_back(j);
-          A.cf.push_back(AmgPair{cx, cy});
+    const int parts = amg_threads(n0);
+    std::vector<int32_t> rowlen(n0);
+    std::vector<std::vector<int32_t>> pc(parts);
+    std::vector<std::vector<AmgPair>> pf(parts);
+    amg_parts(n0, parts, [&](int t, int i0, int i1) {
+      std::vector<int32_t> lc;
+      std::vector<AmgPair> lf;
+      for (int i = i0; i < i1; i++) {
+        const int p = rows[i];
+        const int64_t base = cptr[p / SELL] + (p % SELL);
+        const size_t row0 = lc.size();
+        auto add = [&](int j, double cx, double cy) {
+          if (j == i) {
+            A.fd[i].x += cx;
+            A.fd[i].y += cy;
+            return;
+          }
+          for (size_t k = row0; k < lc.size(); k++)
+            if (lc[k] == j) {
+              lf[k].x += cx;
+              lf[k].y += cy;
+              return;
+            }
+          lc.push_back(j);
+          lf.push_back(AmgPair{cx, cy});
+        };
+        for (int s = 0; s < deg[p]; s++) {
+          const int64_t e = base + (int64_t)s * SELL;
+          const int q = col[e];
+          const double2 c = cU[e];
+          if (pid[q] >= 0) add(pid[q], c.x, c.y);
+          else if (exi[q] >= 0)
+            for (auto &pr : exl[exi[q]]) add(pr.first, c.x * pr.second, c.y * pr.second);
         }
-      };
-      for (int s = 0; s < deg[p]; s++) {
-        const int64_t e = base + (int64_t)s * SELL;
-        const int q = col[e];
-        const double2 c = cU[e];
-        if (pid[q] >= 0) add(pid[q], c.x, c.y);
-        else if (exi[q] >= 0)
-          for (auto &pr : exl[exi[q]]) add(pr.first, c.x * pr.second, c.y * pr.second);
+        rowlen[i] = (int32_t)(lc.size() - row0);
+        A.S[i] = 3.0 * Sxy[p];
       }
-      A.ptr.push_back((int32_t)A.col.size());
-      A.S[i] = 3.0 * Sxy[p];
-    }
+      pc[t] = std::move(lc);
+      pf[t] = std::move(lf);
+    });
+    amg_join_rows(rowlen, pc, pf, A.ptr, A.col, A.cf);
   }
   // coarsen until the dense inverse fits (at least one coarse level)
   std::vector<std::vector<int32_t>> aggs, mptrs, mems;
