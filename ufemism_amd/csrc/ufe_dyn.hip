@@ -3281,7 +3281,7 @@ int ufe_ssa_adjoint_forward(ufe_ctx *ctx, const ufe_adjoint_opts *o, ufe_adjoint
     return UFE_BAD_INPUT;
   }
   if (ctx->params.use_analytical_GL_flux) {
-    ctx->err = "use_analytical_GL_flux is not supported";
+    ctx->err = "use_analytical_GL_flux is not supported by the adjoint forward";
     return st->status = UFE_NOT_SUPPORTED;
   }
   const int K = o->picard_iterations, every = o->checkpoint_every;
