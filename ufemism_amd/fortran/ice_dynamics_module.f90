@@ -44,7 +44,7 @@ MODULE ice_dynamics_module
     ! 0: the library defaults (40 and 5 with SOR sweeps, 20 and 2 with multigrid)
     INTEGER  :: SSA_gmres_restart = 0
     INTEGER  :: SSA_precond_sweeps = 0
-    ! preconditioner of those solves (UFE_MIXED): 0 SOR sweeps, 1 aggregation multigrid
+    ! preconditioner of those solves: 0 SOR sweeps, 1 aggregation multigrid
     INTEGER  :: SSA_precond_type = 0
     ! ice_physical_properties at the end of update_general_ice_model_data:
     ! 0 the A_flow fields are inputs, 1 Arrhenius law from Ti, 2 fixed benchmark values
