@@ -12,7 +12,10 @@ unequal couplings in the two directions:
 * the transposed couplings give y.(A x) = x.(A^T y);
 * a second level coarsens the first in the same way;
 * on a grid large enough for the threaded Galerkin product, 1, 3 and 7 host
-  threads give the same coarse operator bit for bit.
+  threads give the same coarse operator bit for bit;
+* the fine operator with the Neumann rows eliminated (a grid with an edge
+  ring and four corners in sliced-ELLPACK storage) equals the full rows
+  applied to the expanded vector, with 1 and 5 host threads.
 """
 import os
 import shutil
@@ -142,6 +145,103 @@ static void check_level(const AmgHostOp &A, std::mt19937 &g, int level, AmgHostO
               *std::max_element(size.begin(), size.end()));
 }
 
+// The fine operator of a grid with a Neumann ring: a sliced-ELLPACK matrix (slice
+// 4) over all points, SOR rows inside, edge rows the mean of their inner
+// neighbours, the four corners the mean of their (edge) neighbours.  For a
+// random fine vector x, expanded to every point, the full rows equal the
+// eliminated operator's rows (off-diagonal and fixed diagonal parts).
+static void check_fine_operator(int nx, int ny, std::mt19937 &g, int parts) {
+  std::uniform_real_distribution<double> u(0.1, 1.0), s(-1.0, 1.0);
+  const int N = nx * ny, sell = 4;
+  auto edge = [&](int p) { int i = p % nx, j = p / nx; return i == 0 || j == 0 || i == nx - 1 || j == ny - 1; };
+  auto corner = [&](int p) { int i = p % nx, j = p / nx; return (i == 0 || i == nx - 1) && (j == 0 || j == ny - 1); };
+  std::vector<std::vector<int>> nb(N);
+  for (int p = 0; p < N; p++) {
+    const int i = p % nx, j = p / nx;
+    for (int dj = -1; dj <= 1; dj++)
+      for (int di = -1; di <= 1; di++) {
+        if ((di == 0 && dj == 0) || (di != 0 && dj != 0 && (i + j) % 3)) continue;
+        const int ii = i + di, jj = j + dj;
+        if (ii >= 0 && jj >= 0 && ii < nx && jj < ny) nb[p].push_back(jj * nx + ii);
+      }
+  }
+  std::vector<int64_t> cptr(1, 0);
+  std::vector<uint8_t> deg(N);
+  for (int c = 0; c * sell < N; c++) {
+    int w = 0;
+    for (int p = c * sell; p < std::min(N, (c + 1) * sell); p++) w = std::max(w, (int)nb[p].size());
+    cptr.push_back(cptr.back() + (int64_t)w * sell);
+  }
+  std::vector<int32_t> col(cptr.back(), 0);
+  std::vector<AmgPair> cU(cptr.back(), AmgPair{0.0, 0.0});
+  std::vector<double> Sxy(N);
+  for (int p = 0; p < N; p++) {
+    deg[p] = (uint8_t)nb[p].size();
+    for (size_t k = 0; k < nb[p].size(); k++) {
+      const int64_t e = cptr[p / sell] + p % sell + (int64_t)k * sell;
+      col[e] = nb[p][k];
+      cU[e] = AmgPair{u(g), u(g)};
+    }
+    Sxy[p] = s(g);
+  }
+  std::vector<int32_t> rows, pid(N, -1), erow, eptr{0}, ecol, crow, cptr2{0}, ccol;
+  for (int p = 0; p < N; p++)
+    if (!edge(p)) { pid[p] = (int32_t)rows.size(); rows.push_back(p); }
+  for (int p = 0; p < N; p++) {
+    if (!edge(p) || corner(p)) continue;
+    erow.push_back(p);
+    for (int q : nb[p]) if (!edge(q)) ecol.push_back(q);
+    eptr.push_back((int32_t)ecol.size());
+  }
+  for (int p = 0; p < N; p++) {
+    if (!corner(p)) continue;
+    crow.push_back(p);
+    for (int q : nb[p]) ccol.push_back(q);
+    cptr2.push_back((int32_t)ccol.size());
+  }
+  std::vector<int32_t> exi;
+  std::vector<AmgExpansion> exl;
+  amg_neumann_expansions(pid, erow, eptr, ecol, crow, cptr2, ccol, exi, exl);
+  AmgHostOp A, A1;
+  amg_fine_operator(rows, sell, cptr, deg, col, cU.data(), Sxy, pid, exi, exl, A, parts);
+  amg_fine_operator(rows, sell, cptr, deg, col, cU.data(), Sxy, pid, exi, exl, A1, 1);
+  CHECK(A.n == (int)rows.size() && A.ptr == A1.ptr && A.col == A1.col, "fine operator with %d parts: structure", parts);
+  for (size_t k = 0; k < A.cf.size() && k < A1.cf.size(); k++)
+    CHECK(A.cf[k].x == A1.cf[k].x && A.cf[k].y == A1.cf[k].y, "fine operator with %d parts: entry %zu", parts, k);
+  int self = 0;
+  for (int i = 0; i < A.n; i++) self += A.fd[i].x != 0.0;
+  CHECK(self > 0, "no self-couplings came back through the Neumann rows");
+  std::vector<AmgPair> x(A.n), X(N, AmgPair{0.0, 0.0});
+  for (int i = 0; i < A.n; i++) { x[i] = AmgPair{s(g), s(g)}; X[rows[i]] = x[i]; }
+  for (size_t i = 0; i < erow.size(); i++) {
+    double a = 0.0, b = 0.0;
+    for (int k = eptr[i]; k < eptr[i + 1]; k++) { a += X[ecol[k]].x; b += X[ecol[k]].y; }
+    X[erow[i]] = AmgPair{a / (eptr[i + 1] - eptr[i]), b / (eptr[i + 1] - eptr[i])};
+  }
+  for (size_t i = 0; i < crow.size(); i++) {
+    double a = 0.0, b = 0.0;
+    for (int k = cptr2[i]; k < cptr2[i + 1]; k++) { a += X[ccol[k]].x; b += X[ccol[k]].y; }
+    X[crow[i]] = AmgPair{a / (cptr2[i + 1] - cptr2[i]), b / (cptr2[i + 1] - cptr2[i])};
+  }
+  double err = 0.0, scale = 0.0;
+  for (int i = 0; i < A.n; i++) {
+    const int p = rows[i];
+    double fu = 0.0, fv = 0.0;
+    for (int k = 0; k < deg[p]; k++) {
+      const int64_t e = cptr[p / sell] + p % sell + (int64_t)k * sell;
+      fu += cU[e].x * X[col[e]].x;
+      fv += cU[e].y * X[col[e]].y;
+    }
+    double eu = A.fd[i].x * x[i].x, ev = A.fd[i].y * x[i].y;
+    for (int k = A.ptr[i]; k < A.ptr[i + 1]; k++) { eu += A.cf[k].x * x[A.col[k]].x; ev += A.cf[k].y * x[A.col[k]].y; }
+    err = std::fmax(err, std::fmax(std::fabs(fu - eu), std::fabs(fv - ev)));
+    scale = std::fmax(scale, std::fmax(std::fabs(fu), std::fabs(fv)));
+    CHECK(A.S[i] == 3.0 * Sxy[p], "S of fine point %d", i);
+  }
+  CHECK(err <= 1e-12 * scale, "fine operator %dx%d: eliminated rows differ by %.3e of %.3e", nx, ny, err, scale);
+  std::printf("fine operator %dx%d: %d points, %zu couplings, %d with self-couplings\n", nx, ny, A.n, A.col.size(), self);
+}
+
 static bool same(const AmgHostOp &a, const AmgHostOp &b) {
   auto eq = [](const std::vector<AmgPair> &x, const std::vector<AmgPair> &y) {
     if (x.size() != y.size()) return false;
@@ -159,6 +259,8 @@ int main() {
     check_level(A, g, 1, C1);
     check_level(C1, g, 2, C2);
   }
+  check_fine_operator(13, 9, g, 1);
+  check_fine_operator(301, 257, g, 5);
   // a grid large enough for the threaded Galerkin product: the identities hold
   // and the coarse operator is the same, bit for bit, for 1, 3 and 7 parts
   {

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <thread>
+#include <utility>
 #include <vector>
 
 namespace ufe {
@@ -63,6 +64,104 @@ struct AmgHostOp {              // a point operator: off-diagonal pairs, fixed d
   std::vector<AmgPair> cf, fd;
   std::vector<double> S;
 };
+
+// A point's value as a weighted sum of fine points (a Neumann row eliminated)
+using AmgExpansion = std::vector<std::pair<int32_t, double>>;
+
+// The Neumann rows as weighted sums of fine points.  pid[p] is the fine point
+// of row p (-1 when p is not an SOR row).  Edge row erow[i] is the mean of its
+// listed neighbours eptr/ecol (SOR rows); corner crow[i] is the mean of its
+// listed neighbours cptr/ccol, edge rows among them expanded.  exi[p] becomes
+// p's index in exl (-1 for rows that are neither).
+inline void amg_neumann_expansions(const std::vector<int32_t> &pid, const std::vector<int32_t> &erow,
+                                   const std::vector<int32_t> &eptr, const std::vector<int32_t> &ecol,
+                                   const std::vector<int32_t> &crow, const std::vector<int32_t> &cptr,
+                                   const std::vector<int32_t> &ccol, std::vector<int32_t> &exi,
+                                   std::vector<AmgExpansion> &exl) {
+  exi.assign(pid.size(), -1);
+  exl.clear();
+  for (size_t i = 0; i < erow.size(); i++) {
+    const double w = 1.0 / (double)(eptr[i + 1] - eptr[i]);
+    AmgExpansion e;
+    for (int k = eptr[i]; k < eptr[i + 1]; k++)
+      if (pid[ecol[k]] >= 0) e.push_back({pid[ecol[k]], w});
+    exi[erow[i]] = (int32_t)exl.size();
+    exl.push_back(std::move(e));
+  }
+  for (size_t i = 0; i < crow.size(); i++) {
+    const double w = 1.0 / (double)(cptr[i + 1] - cptr[i]);
+    AmgExpansion e;
+    for (int k = cptr[i]; k < cptr[i + 1]; k++) {
+      const int q = ccol[k];
+      if (pid[q] >= 0) e.push_back({pid[q], w});
+      else if (exi[q] >= 0)
+        for (auto &pr : exl[exi[q]]) e.push_back({pr.first, pr.second * w});
+    }
+    exi[crow[i]] = (int32_t)exl.size();
+    exl.push_back(std::move(e));
+  }
+}
+
+// The operator on the fine points with the Neumann rows eliminated.  Fine
+// point i is row rows[i] of a sliced-ELLPACK matrix: slot s of row p holds
+// column col[e] and pair cU[e] at e = cptr[p / sell] + p % sell + s * sell,
+// for s < deg[p].  A coupling to a Neumann row is spread over its expansion,
+// a coupling that comes back to the row itself goes into the fixed diagonal
+// part fd, and S = 3 Sxy.  Rows are built in parts on host threads (parts 0:
+// amg_threads); a row's entries keep the order of first appearance, found by a
+// search of the row so far (short), so the result does not depend on parts.
+inline void amg_fine_operator(const std::vector<int32_t> &rows, int sell, const std::vector<int64_t> &cptr,
+                              const std::vector<uint8_t> &deg, const std::vector<int32_t> &col, const AmgPair *cU,
+                              const std::vector<double> &Sxy, const std::vector<int32_t> &pid,
+                              const std::vector<int32_t> &exi, const std::vector<AmgExpansion> &exl, AmgHostOp &A,
+                              int parts = 0) {
+  const int n0 = (int)rows.size();
+  A.n = n0;
+  A.fd.assign(n0, AmgPair{0.0, 0.0});
+  A.S.resize(n0);
+  if (parts <= 0) parts = amg_threads(n0);
+  parts = std::max(1, std::min(parts, std::max(n0, 1)));
+  std::vector<int32_t> rowlen(n0);
+  std::vector<std::vector<int32_t>> pc(parts);
+  std::vector<std::vector<AmgPair>> pf(parts);
+  amg_parts(n0, parts, [&](int t, int i0, int i1) {
+    std::vector<int32_t> lc;             // local, not pc[t]: the parts' vector headers share cache lines
+    std::vector<AmgPair> lf;
+    for (int i = i0; i < i1; i++) {
+      const int p = rows[i];
+      const int64_t base = cptr[p / sell] + (p % sell);
+      const size_t row0 = lc.size();
+      auto add = [&](int j, double cx, double cy) {
+        if (j == i) {
+          A.fd[i].x += cx;
+          A.fd[i].y += cy;
+          return;
+        }
+        for (size_t k = row0; k < lc.size(); k++)
+          if (lc[k] == j) {
+            lf[k].x += cx;
+            lf[k].y += cy;
+            return;
+          }
+        lc.push_back(j);
+        lf.push_back(AmgPair{cx, cy});
+      };
+      for (int s = 0; s < deg[p]; s++) {
+        const int64_t e = base + (int64_t)s * sell;
+        const int q = col[e];
+        const AmgPair c = cU[e];
+        if (pid[q] >= 0) add(pid[q], c.x, c.y);
+        else if (exi[q] >= 0)
+          for (auto &pr : exl[exi[q]]) add(pr.first, c.x * pr.second, c.y * pr.second);
+      }
+      rowlen[i] = (int32_t)(lc.size() - row0);
+      A.S[i] = 3.0 * Sxy[p];
+    }
+    pc[t] = std::move(lc);
+    pf[t] = std::move(lf);
+  });
+  amg_join_rows(rowlen, pc, pf, A.ptr, A.col, A.cf);
+}
 
 // Greedy aggregation: a point whose neighbours are all free takes them into a
 // new aggregate; every point left joins the aggregate of its first aggregated

@@ -2097,81 +2097,15 @@ static int amg_setup(ufe_ctx *ctx) {
     ctx->err = "no SOR rows to aggregate";
     return UFE_BAD_INPUT;
   }
-  // the Neumann rows as weighted sums of fine points: edges average SOR rows,
-  // corners average their neighbours (edge rows expanded)
-  std::vector<int32_t> exi(nA, -1);
-  std::vector<std::vector<std::pair<int32_t, double>>> exl;
-  for (int i = 0; i < ctx->n_neu; i++) {
-    const double w = 1.0 / (double)(nptr[i + 1] - nptr[i]);
-    std::vector<std::pair<int32_t, double>> e;
-    for (int k = nptr[i]; k < nptr[i + 1]; k++)
-      if (pid[ncol[k]] >= 0) e.push_back({pid[ncol[k]], w});
-    exi[nrow[i]] = (int32_t)exl.size();
-    exl.push_back(std::move(e));
-  }
-  for (int i = 0; i < ctx->n_cor; i++) {
-    const double w = 1.0 / (double)(cptr2[i + 1] - cptr2[i]);
-    std::vector<std::pair<int32_t, double>> e;
-    for (int k = cptr2[i]; k < cptr2[i + 1]; k++) {
-      const int q = ccol[k];
-      if (pid[q] >= 0) e.push_back({pid[q], w});
-      else if (exi[q] >= 0)
-        for (auto &pr : exl[exi[q]]) e.push_back({pr.first, pr.second * w});
-    }
-    exi[crow[i]] = (int32_t)exl.size();
-    exl.push_back(std::move(e));
-  }
-  // the operator on the fine points with the Neumann rows eliminated
-  // (rows in parts on host threads; a row's entries keep the order of first
-  // appearance, found by a search of the row so far, which is short)
+  // the Neumann rows as weighted sums of fine points, then the operator on the
+  // fine points with them eliminated
+  std::vector<int32_t> exi;
+  std::vector<AmgExpansion> exl;
+  amg_neumann_expansions(pid, nrow, nptr, ncol, crow, cptr2, ccol, exi, exl);
+  static_assert(sizeof(AmgPair) == sizeof(double2), "AmgPair must have the layout of double2");
   std::vector<AmgHostOp> ops(1);
-  {
-    AmgHostOp &A = ops[0];
-    A.n = n0;
-    A.fd.assign(n0, AmgPair{0.0, 0.0});
-    A.S.resize(n0);
-    const int parts = amg_threads(n0);
-    std::vector<int32_t> rowlen(n0);
-    std::vector<std::vector<int32_t>> pc(parts);
-    std::vector<std::vector<AmgPair>> pf(parts);
-    amg_parts(n0, parts, [&](int t, int i0, int i1) {
-      std::vector<int32_t> lc;
-      std::vector<AmgPair> lf;
-      for (int i = i0; i < i1; i++) {
-        const int p = rows[i];
-        const int64_t base = cptr[p / SELL] + (p % SELL);
-        const size_t row0 = lc.size();
-        auto add = [&](int j, double cx, double cy) {
-          if (j == i) {
-            A.fd[i].x += cx;
-            A.fd[i].y += cy;
-            return;
-          }
-          for (size_t k = row0; k < lc.size(); k++)
-            if (lc[k] == j) {
-              lf[k].x += cx;
-              lf[k].y += cy;
-              return;
-            }
-          lc.push_back(j);
-          lf.push_back(AmgPair{cx, cy});
-        };
-        for (int s = 0; s < deg[p]; s++) {
-          const int64_t e = base + (int64_t)s * SELL;
-          const int q = col[e];
-          const double2 c = cU[e];
-          if (pid[q] >= 0) add(pid[q], c.x, c.y);
-          else if (exi[q] >= 0)
-            for (auto &pr : exl[exi[q]]) add(pr.first, c.x * pr.second, c.y * pr.second);
-        }
-        rowlen[i] = (int32_t)(lc.size() - row0);
-        A.S[i] = 3.0 * Sxy[p];
-      }
-      pc[t] = std::move(lc);
-      pf[t] = std::move(lf);
-    });
-    amg_join_rows(rowlen, pc, pf, A.ptr, A.col, A.cf);
-  }
+  amg_fine_operator(rows, SELL, cptr, deg, col, reinterpret_cast<const AmgPair *>(cU.data()), Sxy, pid, exi, exl,
+                    ops[0]);
   // coarsen until the dense inverse fits (at least one coarse level)
   std::vector<std::vector<int32_t>> aggs, mptrs, mems;
   std::vector<std::vector<AmgPair>> intras;
@@ -2225,7 +2159,6 @@ static int amg_setup(ufe_ctx *ctx) {
     // diagonal pairs EV do not hold them; below, the couplings inside each
     // aggregate only (the members' diagonals already hold their fixed parts)
     const std::vector<AmgPair> &own = l == 1 ? C.fd : intras[l - 1];
-    static_assert(sizeof(AmgPair) == sizeof(double2), "AmgPair must have the layout of double2");
     if ((r = dupload(ctx, &V.fd, reinterpret_cast<const double2 *>(own.data()), own.size()))) return r;
     if ((r = dalloc(ctx, &V.ev64, (size_t)C.n))) return r;
     if ((r = dalloc(ctx, &V.ev32, (size_t)C.n))) return r;
