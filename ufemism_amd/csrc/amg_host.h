@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <system_error>
 #include <thread>
 #include <utility>
 #include <vector>
@@ -19,7 +20,9 @@ inline int amg_threads(int64_t n) {
   return h ? (int)std::min(h, 16u) : 1;
 }
 
-// body(t, begin, end) on `parts` contiguous ranges of [0, n), one thread each
+// body(t, begin, end) on `parts` contiguous ranges of [0, n), one thread each;
+// a part whose thread cannot be started runs on the calling thread (no
+// exception leaves, since callers sit behind a C interface)
 template <typename F>
 inline void amg_parts(int n, int parts, F body) {
   if (parts <= 1) {
@@ -28,8 +31,14 @@ inline void amg_parts(int n, int parts, F body) {
   }
   std::vector<std::thread> th;
   th.reserve(parts);
-  for (int t = 0; t < parts; t++)
-    th.emplace_back(body, t, (int)((int64_t)n * t / parts), (int)((int64_t)n * (t + 1) / parts));
+  for (int t = 0; t < parts; t++) {
+    const int b = (int)((int64_t)n * t / parts), e = (int)((int64_t)n * (t + 1) / parts);
+    try {
+      th.emplace_back(body, t, b, e);
+    } catch (const std::system_error &) {
+      body(t, b, e);
+    }
+  }
   for (auto &x : th) x.join();
 }
 
