@@ -35,7 +35,8 @@ enum {
   UFE_COLOURING_FAILED = 5,    /* src/mesh_five_colour_module.f90:168-171, 270-273, 329-341 */
   UFE_BAD_INPUT = 6,
   UFE_HIP_ERROR = 7,
-  UFE_NOT_SUPPORTED = 8
+  UFE_NOT_SUPPORTED = 8,
+  UFE_HOST_ERROR = 9           /* host memory or threads could not be had (no exception leaves the library) */
 };
 
 enum { UFE_FP64 = 0, UFE_MIXED = 1, UFE_FP64_KRYLOV = 2 };

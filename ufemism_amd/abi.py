@@ -19,13 +19,13 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "lib
 
 UFE_OK, UFE_SOR_NOT_CONVERGED, UFE_SSA_RESET, UFE_SSA_UNSTABLE = 0, 1, 2, 3
 UFE_COLOURING_IDENTIFY, UFE_COLOURING_FAILED, UFE_BAD_INPUT = 4, 5, 6
-UFE_HIP_ERROR, UFE_NOT_SUPPORTED = 7, 8
+UFE_HIP_ERROR, UFE_NOT_SUPPORTED, UFE_HOST_ERROR = 7, 8, 9
 UFE_FP64, UFE_MIXED, UFE_FP64_KRYLOV = 0, 1, 2
 UFE_PRECOND_SOR, UFE_PRECOND_MULTIGRID = 0, 1
 
 STATUS_NAMES = {0: "OK", 1: "SOR_NOT_CONVERGED", 2: "SSA_RESET", 3: "SSA_UNSTABLE",
                 4: "COLOURING_IDENTIFY", 5: "COLOURING_FAILED", 6: "BAD_INPUT",
-                7: "HIP_ERROR", 8: "NOT_SUPPORTED"}
+                7: "HIP_ERROR", 8: "NOT_SUPPORTED", 9: "HOST_ERROR"}
 
 # every symbol include/ufe_dyn.h declares
 EXPORTS = [

@@ -512,13 +512,20 @@ static ufe_params default_params() {
 // ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
+// An exception from host code (memory or threads exhausted) ends the entry
+// point it reached with UFE_HOST_ERROR; none leaves through the C interface.
+static int host_failure(ufe_ctx *ctx, const char *what) {
+  if (ctx) ctx->err = std::string("host failure: ") + (what ? what : "");
+  return UFE_HOST_ERROR;
+}
+
 extern "C" {
 
 int ufe_version(void) { return 1; }
 
 const char *ufe_last_error(ufe_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int ufe_ctx_create(int device, ufe_ctx **out) {
+int ufe_ctx_create(int device, ufe_ctx **out) try {
   if (!out) return UFE_BAD_INPUT;
   *out = nullptr;
   int n = 0;
@@ -534,6 +541,10 @@ int ufe_ctx_create(int device, ufe_ctx **out) {
   }
   *out = ctx;
   return UFE_OK;
+} catch (const std::exception &e) {
+  return host_failure(nullptr, e.what());
+} catch (...) {
+  return host_failure(nullptr, "unknown exception");
 }
 
 int ufe_ctx_destroy(ufe_ctx *ctx) {
@@ -705,7 +716,7 @@ static int gpu_neighbour_functions(ufe_ctx *ctx, int np, int nCm, const double *
   return UFE_OK;
 }
 
-int ufe_mesh_build(ufe_ctx *ctx, ufe_mesh_desc *m) {
+int ufe_mesh_build(ufe_ctx *ctx, ufe_mesh_desc *m) try {
   if (!ctx) return UFE_BAD_INPUT;
   int r = check_primary(ctx, m);
   if (r) return r;
@@ -730,6 +741,10 @@ int ufe_mesh_build(ufe_ctx *ctx, ufe_mesh_desc *m) {
                                    m->edge_index_Ac, 0, m->Nx, m->Ny, m->Nxx, m->Nxy, m->Nyy)))
     return r;
   return five_colouring(m, ctx->err);
+} catch (const std::exception &e) {
+  return host_failure(ctx, e.what());
+} catch (...) {
+  return host_failure(ctx, "unknown exception");
 }
 
 // ---- primary-mesh geometry ---------------------------------------------------
@@ -777,7 +792,7 @@ static int check_geometry(ufe_ctx *ctx, const ufe_mesh_geometry_desc *g) {
   return UFE_OK;
 }
 
-int ufe_mesh_geometry(ufe_ctx *ctx, ufe_mesh_geometry_desc *g, ufe_mesh_geometry_stats *st) {
+int ufe_mesh_geometry(ufe_ctx *ctx, ufe_mesh_geometry_desc *g, ufe_mesh_geometry_stats *st) try {
   if (!ctx) return UFE_BAD_INPUT;
   int r = check_geometry(ctx, g);
   if (r) return r;
@@ -878,6 +893,10 @@ int ufe_mesh_geometry(ufe_ctx *ctx, ufe_mesh_geometry_desc *g, ufe_mesh_geometry
     return UFE_BAD_INPUT;
   }
   return UFE_OK;
+} catch (const std::exception &e) {
+  return host_failure(ctx, e.what());
+} catch (...) {
+  return host_failure(ctx, "unknown exception");
 }
 
 
@@ -942,7 +961,7 @@ static double rm_area_host(const double *a, const double *b, const double *c) {
 }
 
 int ufe_remap_build(ufe_ctx *dst, ufe_ctx *src, const ufe_mesh_geometry_desc *gs, const ufe_mesh_geometry_desc *gd,
-                    ufe_remap_stats *st) {
+                    ufe_remap_stats *st) try {
   if (!dst || !src || !dst->have_mesh || !src->have_mesh) return UFE_BAD_INPUT;
   if (dst->device != src->device) {
     dst->err = "ufe_remap_build: both contexts must use the same device";
@@ -1144,6 +1163,10 @@ int ufe_remap_build(ufe_ctx *dst, ufe_ctx *src, const ufe_mesh_geometry_desc *gs
   dst->rm_n = n;
   dst->rm_ready = true;
   return UFE_OK;
+} catch (const std::exception &e) {
+  return host_failure(dst, e.what());
+} catch (...) {
+  return host_failure(dst, "unknown exception");
 }
 
 int ufe_remap_field(ufe_ctx *dst, ufe_ctx *src, int method, const double *d_src, double *d_dst, int nz) {
@@ -1225,7 +1248,7 @@ int ufe_remap_export(ufe_ctx *dst, int *n_entries, int32_t *nn, int32_t *tvi, do
 }
 
 // ---- mesh upload -----------------------------------------------------------
-int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
+int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) try {
   if (!ctx) return UFE_BAD_INPUT;
   int r = check_primary(ctx, m);
   if (r) return r;
@@ -1667,6 +1690,10 @@ int ufe_mesh_upload(ufe_ctx *ctx, const ufe_mesh_desc *m) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->have_mesh = true;
   return UFE_OK;
+} catch (const std::exception &e) {
+  return host_failure(ctx, e.what());
+} catch (...) {
+  return host_failure(ctx, "unknown exception");
 }
 
 // ---- fields ----------------------------------------------------------------
@@ -2463,7 +2490,7 @@ static int zero_ssa_outputs(ufe_ctx *ctx) {
   return UFE_OK;
 }
 
-int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) {
+int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) try {
   if (!ctx || !ctx->have_mesh) return UFE_BAD_INPUT;
   ufe_ssa_stats local;
   if (!st) st = &local;
@@ -2566,6 +2593,10 @@ int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   st->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st->status;
+} catch (const std::exception &e) {
+  return host_failure(ctx, e.what());
+} catch (...) {
+  return host_failure(ctx, "unknown exception");
 }
 
 // ---- linearised solve at fixed viscosity (bench path) ------------------------
@@ -2895,7 +2926,7 @@ int ufe_multigrid_info_get(ufe_ctx *ctx, ufe_multigrid_info *out) {
   return UFE_OK;
 }
 
-int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_stats *st) {
+int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_stats *st) try {
   if (!ctx || !ctx->have_mesh || !st || !o) return UFE_BAD_INPUT;
   std::memset(st, 0, sizeof(*st));
   const AaAcMesh m = aaac_mesh(ctx);
@@ -3016,6 +3047,10 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
   st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   st->status = rel <= o->rel_tol ? UFE_OK : UFE_SOR_NOT_CONVERGED;
   return UFE_OK;
+} catch (const std::exception &e) {
+  return host_failure(ctx, e.what());
+} catch (...) {
+  return host_failure(ctx, "unknown exception");
 }
 
 
@@ -3202,7 +3237,7 @@ static int adj_solve(ufe_ctx *ctx, double tol, int m, int ks, double omega, int 
   return rel <= tol ? UFE_OK : UFE_SOR_NOT_CONVERGED;
 }
 
-int ufe_ssa_adjoint_forward(ufe_ctx *ctx, const ufe_adjoint_opts *o, ufe_adjoint_stats *st) {
+int ufe_ssa_adjoint_forward(ufe_ctx *ctx, const ufe_adjoint_opts *o, ufe_adjoint_stats *st) try {
   if (!ctx || !ctx->have_mesh || !o || !st) return UFE_BAD_INPUT;
   std::memset(st, 0, sizeof(*st));
   if (o->picard_iterations < 1 || o->picard_iterations > 100000 || o->checkpoint_every < 1 ||
@@ -3249,10 +3284,14 @@ int ufe_ssa_adjoint_forward(ufe_ctx *ctx, const ufe_adjoint_opts *o, ufe_adjoint
   ctx->adj_every = every;
   ctx->adj_ready = true;
   return st->status;
+} catch (const std::exception &e) {
+  return host_failure(ctx, e.what());
+} catch (...) {
+  return host_failure(ctx, "unknown exception");
 }
 
 int ufe_ssa_adjoint_backward(ufe_ctx *ctx, const double *dJ_dU, const double *dJ_dV, double *g_logtau,
-                             double *g_A, ufe_adjoint_stats *st) {
+                             double *g_A, ufe_adjoint_stats *st) try {
   if (!ctx || !ctx->have_mesh || !dJ_dU || !dJ_dV || !g_logtau || !g_A || !st) return UFE_BAD_INPUT;
   if (!ctx->adj_ready) {
     ctx->err = "ufe_ssa_adjoint_backward needs a preceding ufe_ssa_adjoint_forward";
@@ -3332,6 +3371,10 @@ int ufe_ssa_adjoint_backward(ufe_ctx *ctx, const double *dJ_dU, const double *dJ
   st->picard_iterations = K;
   st->backward_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st->status;
+} catch (const std::exception &e) {
+  return host_failure(ctx, e.what());
+} catch (...) {
+  return host_failure(ctx, "unknown exception");
 }
 
 
@@ -3339,7 +3382,7 @@ int ufe_ssa_adjoint_backward(ufe_ctx *ctx, const double *dJ_dU, const double *dJ
 // (no reference counterpart: many independent linearised SSA systems, each
 // src/ice_dynamics_module.f90:558-694, solved together)
 
-int ufe_batch_define(ufe_ctx *ctx, int nmembers, const int32_t *nV, const int32_t *nAc) {
+int ufe_batch_define(ufe_ctx *ctx, int nmembers, const int32_t *nV, const int32_t *nAc) try {
   if (!ctx || nmembers < 0 || nmembers > 4096 || (nmembers > 0 && (!nV || !nAc))) return UFE_BAD_INPUT;
   ctx->b_nV.clear();
   ctx->b_nAc.clear();
@@ -3352,6 +3395,10 @@ int ufe_batch_define(ufe_ctx *ctx, int nmembers, const int32_t *nV, const int32_
     ctx->b_nAc.push_back(nAc[b]);
   }
   return UFE_OK;
+} catch (const std::exception &e) {
+  return host_failure(ctx, e.what());
+} catch (...) {
+  return host_failure(ctx, "unknown exception");
 }
 
 static Chunks chunk_list(ufe_ctx *c) {
@@ -3506,7 +3553,7 @@ static int batch_capture(ufe_ctx *ctx, bool f32, int m, int ks, double omega, do
   return UFE_OK;
 }
 
-int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *o, double *member_rel, ufe_batch_stats *st) {
+int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *o, double *member_rel, ufe_batch_stats *st) try {
   if (!ctx || !ctx->have_mesh || !o || !st) return UFE_BAD_INPUT;
   std::memset(st, 0, sizeof(*st));
   if (ctx->nb == 0) {
@@ -3581,10 +3628,14 @@ int ufe_batch_linear_solve(ufe_ctx *ctx, const ufe_batch_opts *o, double *member
   st->status = conv == NB ? UFE_OK : UFE_SOR_NOT_CONVERGED;
   if (member_rel) std::memcpy(member_rel, ctx->h_bk_rel, sizeof(double) * NB);
   return UFE_OK;
+} catch (const std::exception &e) {
+  return host_failure(ctx, e.what());
+} catch (...) {
+  return host_failure(ctx, "unknown exception");
 }
 
 int ufe_batch_solve_SSA(ufe_ctx *ctx, int picard_iterations, const ufe_batch_opts *o, double *member_rel,
-                        ufe_batch_stats *st) {
+                        ufe_batch_stats *st) try {
   if (!ctx || !ctx->have_mesh || !o || !st || picard_iterations < 1) return UFE_BAD_INPUT;
   if (ctx->nb == 0) {
     ctx->err = "no batch: call ufe_batch_define before ufe_mesh_upload";
@@ -3610,11 +3661,15 @@ int ufe_batch_solve_SSA(ufe_ctx *ctx, int picard_iterations, const ufe_batch_opt
   tot.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   *st = tot;
   return UFE_OK;
+} catch (const std::exception &e) {
+  return host_failure(ctx, e.what());
+} catch (...) {
+  return host_failure(ctx, "unknown exception");
 }
 
 // Time `sweeps` FP32 sweeps (colour passes plus Neumann) of the uploaded mesh
 // (a single mesh or a batch) for every launch shape and keep the fastest.
-int ufe_sweep_autotune(ufe_ctx *ctx, int sweeps, ufe_sweep_tuning *out) {
+int ufe_sweep_autotune(ufe_ctx *ctx, int sweeps, ufe_sweep_tuning *out) try {
   if (!ctx || !ctx->have_mesh || !out || sweeps < 1) return UFE_BAD_INPUT;
   std::memset(out, 0, sizeof(*out));
   const AaAcMesh am = aaac_mesh(ctx);
@@ -3670,6 +3725,10 @@ int ufe_sweep_autotune(ufe_ctx *ctx, int sweeps, ufe_sweep_tuning *out) {
   ctx->tune_sched = sbest;
   ctx->tune_loads = lbest;
   return UFE_OK;
+} catch (const std::exception &e) {
+  return host_failure(ctx, e.what());
+} catch (...) {
+  return host_failure(ctx, "unknown exception");
 }
 
 int ufe_sweep_tuning_set(ufe_ctx *ctx, int threads_per_block, int schedule, int loads) {

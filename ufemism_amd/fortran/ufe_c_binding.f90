@@ -11,7 +11,7 @@ MODULE ufe_c_binding
   INTEGER(C_INT), PARAMETER :: UFE_OK = 0, UFE_SOR_NOT_CONVERGED = 1, UFE_SSA_RESET = 2, &
                                UFE_SSA_UNSTABLE = 3, UFE_COLOURING_IDENTIFY = 4, &
                                UFE_COLOURING_FAILED = 5, UFE_BAD_INPUT = 6, UFE_HIP_ERROR = 7, &
-                               UFE_NOT_SUPPORTED = 8
+                               UFE_NOT_SUPPORTED = 8, UFE_HOST_ERROR = 9
   INTEGER(C_INT), PARAMETER :: UFE_FP64 = 0, UFE_MIXED = 1, UFE_FP64_KRYLOV = 2
 
   TYPE, BIND(C) :: ufe_mesh_desc
