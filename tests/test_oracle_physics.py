@@ -138,19 +138,16 @@ def _small_ssa():
     return wl, om, ice
 
 
-def test_sor_fixed_point_is_the_direct_solution():
-    """SOR run to a tiny tolerance reaches the solution of the assembled
-    linear system (SSA stencil on SOR points, Neumann rows on edge points)."""
+def _assembled_ssa(om, ice, fixed=None):
+    """The linearised system of the last Picard iteration, assembled: the SSA
+    stencil on SOR points, Neumann rows on edge points (edge first, as the
+    reference tests it first), and identity rows holding the current values on
+    the points in `fixed` (not on edges)."""
     import scipy.sparse as sp
-    import scipy.sparse.linalg as spl
-    wl, om, ice = _small_ssa()
-    p = wl.params
-    p.SSA_max_outer_loops, p.SSA_max_residual_UV, p.SSA_max_inner_loops = 1, 1e-11, 200000
-    st = ice.solve_SSA(p)
-    assert st.status == 0 and st.sweeps[0] < 200000
-    n, nV = om.nVAaAc, om.mesh.nV
+    n = om.nVAaAc
     nc, C = om["nCAaAc"], om["CAaAc"] - 1
     edge = np.concatenate([om.mesh.edge_index > 0, om["edge_index_Ac"] > 0])
+    fixed = np.zeros(n, bool) if fixed is None else fixed
     I, J, X = [], [], []
     bu, bv = np.zeros(n), np.zeros(n)
     for i in range(n):
@@ -162,6 +159,10 @@ def test_sor_fixed_point_is_the_direct_solution():
                 for j in nb:
                     I.append(comp * n + i); J.append(comp * n + j); X.append(-1.0 / len(nb))
             continue
+        if fixed[i]:
+            I += [i, n + i]; J += [i, n + i]; X += [1.0, 1.0]
+            bu[i], bv[i] = ice["U_SSA_AaAc"][i], ice["V_SSA_AaAc"][i]
+            continue
         sxy = om["Nxy_AaAc"][i, k] + om["Nxy_AaAc"][i, :k].sum()
         for c in range(k):
             j = C[i, c]
@@ -170,11 +171,27 @@ def test_sor_fixed_point_is_the_direct_solution():
         I += [i, i, n + i, n + i]; J += [i, n + i, n + i, i]
         X += [ice["eu_i_AaAc"][i], 3 * sxy, ice["ev_i_AaAc"][i], 3 * sxy]
         bu[i], bv[i] = ice["RHSx_AaAc"][i], ice["RHSy_AaAc"][i]
-    A = sp.csr_matrix((X, (I, J)), shape=(2 * n, 2 * n))
-    x = spl.spsolve(A.tocsc(), np.concatenate([bu, bv]))
+    return sp.csr_matrix((X, (I, J)), shape=(2 * n, 2 * n)), np.concatenate([bu, bv])
+
+
+def _direct_error(om, ice, fixed=None):
+    import scipy.sparse.linalg as spl
+    A, b = _assembled_ssa(om, ice, fixed)
+    x = spl.spsolve(A.tocsc(), b)
+    n = om.nVAaAc
     u, v = ice["U_SSA_AaAc"], ice["V_SSA_AaAc"]
-    ref = np.sqrt(np.sum(x ** 2))
-    err = np.sqrt(np.sum((x[:n] - u) ** 2 + (x[n:] - v) ** 2)) / ref
+    return np.sqrt(np.sum((x[:n] - u) ** 2 + (x[n:] - v) ** 2)) / np.sqrt(np.sum(x ** 2))
+
+
+def test_sor_fixed_point_is_the_direct_solution():
+    """SOR run to a tiny tolerance reaches the solution of the assembled
+    linear system (SSA stencil on SOR points, Neumann rows on edge points)."""
+    wl, om, ice = _small_ssa()
+    p = wl.params
+    p.SSA_max_outer_loops, p.SSA_max_residual_UV, p.SSA_max_inner_loops = 1, 1e-11, 200000
+    st = ice.solve_SSA(p)
+    assert st.status == 0 and st.sweeps[0] < 200000
+    err = _direct_error(om, ice)
     assert err < 1e-8, err
 
 
@@ -343,3 +360,26 @@ def test_gl_flux_rows_are_prescribed_in_solve_SSA():
     assert st.picard_iterations == 3 and st.resets == 0
     assert np.array_equal(ice["Ux_SSA_Ac"][gl], ux0) and np.array_equal(ice["Uy_SSA_Ac"][gl], uy0)
     assert np.abs(ice["Ux_SSA_Ac"][~gl]).max() > 0
+
+
+def test_gl_rows_are_fixed_rows_of_the_direct_solution():
+    """With the GL flux on, SOR that skips the GL Ac rows (src/ice_dynamics_module.f90:
+    624-631) converges to the direct solution of the system in which those rows
+    are identity rows holding the prescribed values: the system that the Krylov
+    solves (residual and correction zero on those rows) also solve.  Without
+    the fixed rows the same solution is far from the direct one."""
+    wl, om, ice = _gl_setup(A=1e-21)
+    ice["A_flow_mean_Ac"][...] = 1e-21
+    p = DynParams(m_enh_sia=1.0, m_enh_ssa=1.0, SSA_max_outer_loops=1, use_tau_c_override=True,
+                  use_analytical_GL_flux=True, SSA_max_residual_UV=1e-10, SSA_max_inner_loops=400000)
+    st = ice.solve_SSA(p)
+    assert st.status == 0 and st.sweeps[0] < 400000 and st.resets == 0
+    nV = om.mesh.nV
+    fixed = np.zeros(om.nVAaAc, bool)
+    fixed[nV:] = ice["mask_gl_Ac"] == 1
+    edge = np.concatenate([om.mesh.edge_index > 0, om["edge_index_Ac"] > 0])
+    assert (fixed & ~edge).sum() > 20
+    err = _direct_error(om, ice, fixed)
+    assert err < 1e-8, err
+    assert _direct_error(om, ice) > 1e-4
+
