@@ -288,16 +288,37 @@ int main() {
 """
 
 
-def test_multigrid_host_logic(tmp_path):
+def _driver(tmp_path):
     cxx = shutil.which("g++")
     if cxx is None:
         pytest.skip("g++ not available")
     src = tmp_path / "amg_host_check.cpp"
     src.write_text(DRIVER.replace('#include <vector>\n#include "amg_host.h"',
                                   '#include <vector>\n#include <algorithm>\n#include "amg_host.h"'))
+    return cxx, src
+
+
+def test_multigrid_host_logic(tmp_path):
+    cxx, src = _driver(tmp_path)
     exe = tmp_path / "amg_host_check"
     subprocess.run([cxx, "-O2", "-std=c++17", "-Wall", "-pthread", "-I", CSRC, "-o", str(exe), str(src)], check=True,
                    capture_output=True, text=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     print(r.stdout)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("sanitizer", ["address,undefined", "thread"])
+def test_multigrid_host_logic_under_sanitizers(tmp_path, sanitizer):
+    """The same checks built with a sanitizer (host code, g++): no memory
+    error, undefined behaviour or data race in the threaded set-up."""
+    cxx, src = _driver(tmp_path)
+    exe = tmp_path / "amg_host_check_san"
+    b = subprocess.run([cxx, "-O1", "-g", "-std=c++17", "-pthread", "-fsanitize=" + sanitizer,
+                        "-fno-sanitize-recover=all", "-I", CSRC, "-o", str(exe), str(src)],
+                       capture_output=True, text=True)
+    if b.returncode != 0:
+        pytest.skip(f"g++ cannot build with -fsanitize={sanitizer}: {b.stderr[-300:]}")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-2000:] + r.stderr[-4000:]
+    assert "WARNING: ThreadSanitizer" not in r.stderr and "ERROR: AddressSanitizer" not in r.stderr, r.stderr[-4000:]
