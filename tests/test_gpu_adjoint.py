@@ -7,7 +7,6 @@ along 32 seeded random directions, and the gradient with respect to A along a
 few more.  Checkpointing every c-th Picard iterate must give the same gradient
 bit for bit as storing all of them.
 """
-import numpy as np
 import pytest
 
 from ufemism_amd import abi, geometry

@@ -5,7 +5,7 @@ reference's time-step selection."""
 import numpy as np
 import pytest
 
-from ufemism_amd import abi, geometry
+from ufemism_amd import geometry
 from ufemism_amd.dynamics import RegionClock, Step, TimeConfig, initial_fields, run_dynamics
 
 pytestmark = pytest.mark.gpu
