@@ -296,6 +296,17 @@ def main():
                 "kind": "port",
                 "sample": f"{n} FP64 SOR sweeps of the CPU oracle (OpenMP, {threads} threads) "
                           f"on the same {nA}-point linear system, {dt:.1f}s"}
+            try:   # the same on one thread, about a third of the time
+                d1, m1 = pyoracle.time_sor_sweeps(om, fields, wl.params, 2, 1)
+                ns1 = max(2, int(args.cpu_seconds / 3 / max(d1 / max(m1, 1), 1e-6)))
+                d1, m1 = pyoracle.time_sor_sweeps(om, fields, wl.params, ns1, 1)
+                out["cpu_baseline"]["single_thread"] = {
+                    "value": round(nA * m1 / d1 / 1e6, 3), "unit": "Mvertex-sweeps/s", "cores": 1,
+                    "sample": f"{m1} FP64 SOR sweeps of the CPU oracle on one thread, {d1:.1f}s"}
+            except Exception as e:
+                out["cpu_baseline"]["single_thread"] = {"value": None, "error": repr(e)}
+            finally:
+                pyoracle.set_threads(threads)
         except Exception as e:  # the CPU baseline is reported, never required
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
     if not args.no_step:
