@@ -12,7 +12,7 @@ from ufemism_amd.dynamics import Step
 
 pytestmark = pytest.mark.gpu
 
-TOL_FP64 = 1e-9   # relative L2, FP64 path
+TOL_FP64 = 1e-9   # relative L2 bound on velocities and thickness, FP64 path
 
 
 def _rel(a, b):
