@@ -1,8 +1,8 @@
 /*
  * ufe_oracle.c -- CPU restatement of the UFEMISM ice-dynamics hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the timed CPU baseline; see
- * ufe_oracle.h).  Each routine restates the reference routine cited above
+ * TEST INFRASTRUCTURE ONLY (what the tests compare the HIP library with,
+ * and the timed CPU baseline; see ufe_oracle.h).  Each routine restates the reference routine cited above
  * it, in the same array layout (column-major, 1-based indices), with the
  * same loop order and the same order of floating-point operations, so that
  * the per-row arithmetic matches the reference as written.  Compiled with

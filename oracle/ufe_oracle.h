@@ -1,8 +1,8 @@
 /*
  * ufe_oracle.h -- CPU oracle for the UFEMISM ice-dynamics hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This restatement is the checker that the HIP
- * library is compared against, and the timed CPU baseline.  Only tests/,
+ * TEST INFRASTRUCTURE ONLY.  The tests compare the HIP library with this
+ * restatement, and it is the timed CPU baseline.  Only tests/,
  * the smoke check and bench.py's CPU-baseline leg may load it; the product
  * library never links or calls it.
  *
