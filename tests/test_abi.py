@@ -75,3 +75,29 @@ def test_version_and_no_device_context():
     # creating a context needs a device; without one it reports an error
     r = lib.ufe_ctx_create(99, ct.byref(h))
     assert r != 0 and not h.value
+
+
+def test_constants_agree_across_the_three_bindings():
+    """Status codes, precision modes and preconditioner kinds: the C header's
+    enums, the Python binding's constants and the Fortran binding's
+    parameters hold the same values."""
+    text = open(os.path.join(ROOT, "include", "ufe_dyn.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    header = {}
+    for body in re.findall(r"enum\s*\{(.*?)\}", text, flags=re.S):
+        for name, val in re.findall(r"\b(UFE_[A-Z0-9_]+)\s*=\s*(\d+)", body):
+            header[name] = int(val)
+    assert {"UFE_OK", "UFE_HOST_ERROR", "UFE_MIXED", "UFE_PRECOND_MULTIGRID"} <= set(header)
+    for name, val in header.items():
+        assert hasattr(abi, name), name
+        assert getattr(abi, name) == val, (name, getattr(abi, name), val)
+    ftext = open(os.path.join(ROOT, "ufemism_amd", "fortran", "ufe_c_binding.f90")).read()
+    fortran = {n: int(v) for n, v in re.findall(r"\b(UFE_[A-Z0-9_]+)\s*=\s*(\d+)", ftext)}
+    assert {"UFE_OK", "UFE_HOST_ERROR"} <= set(fortran)
+    for name, val in fortran.items():
+        assert header.get(name) == val, (name, val, header.get(name))
+    assert set(abi.STATUS_NAMES) == {v for n, v in header.items()
+                                     if n in ("UFE_OK", "UFE_SOR_NOT_CONVERGED", "UFE_SSA_RESET",
+                                              "UFE_SSA_UNSTABLE", "UFE_COLOURING_IDENTIFY",
+                                              "UFE_COLOURING_FAILED", "UFE_BAD_INPUT", "UFE_HIP_ERROR",
+                                              "UFE_NOT_SUPPORTED", "UFE_HOST_ERROR")}
