@@ -101,3 +101,60 @@ def test_constants_agree_across_the_three_bindings():
                                               "UFE_SSA_UNSTABLE", "UFE_COLOURING_IDENTIFY",
                                               "UFE_COLOURING_FAILED", "UFE_BAD_INPUT", "UFE_HIP_ERROR",
                                               "UFE_NOT_SUPPORTED", "UFE_HOST_ERROR")}
+
+
+def test_fortran_bind_c_types_match_header(tmp_path):
+    """The Fortran host's BIND(C) mirrors (ufe_c_binding.f90) have the C structs'
+    size and every component at the C field's offset: a field added to one
+    side only shows here, on the CPU."""
+    import shutil
+    import subprocess
+    import pytest
+    flang = shutil.which(os.environ.get("FC", "flang"))
+    if flang is None:
+        pytest.skip("flang not available")
+    fsrc = os.path.join(ROOT, "ufemism_amd", "fortran", "ufe_c_binding.f90")
+    text = open(fsrc).read()
+    types = {}
+    for name, body in re.findall(r"TYPE,\s*BIND\(C\)\s*::\s*(\w+)(.*?)END TYPE", text, flags=re.S | re.I):
+        comps = []
+        for line in body.splitlines():
+            if "::" not in line:
+                continue
+            decl = re.sub(r"\(.*?\)", "", line.split("::", 1)[1])
+            comps += [c.strip() for c in decl.split(",") if c.strip()]
+        types[name] = comps
+    assert {"ufe_params", "ufe_ssa_stats", "ufe_mesh_desc"} <= set(types)
+    fl = ["PROGRAM probe", "  USE, INTRINSIC :: ISO_C_BINDING", "  USE ufe_c_binding", "  IMPLICIT NONE"]
+    for t in types:
+        fl.append(f"  TYPE({t}), TARGET :: v_{t}")
+    fl.append("  INTEGER(C_INTPTR_T) :: b")
+    for t, comps in types.items():
+        fl.append(f"  PRINT '(A,1X,A,1X,I0)', '{t}', 'size', C_SIZEOF(v_{t})")
+        fl.append(f"  b = TRANSFER(C_LOC(v_{t}), b)")
+        for c in comps:
+            fl.append(f"  PRINT '(A,1X,A,1X,I0)', '{t}', '{c}', TRANSFER(C_LOC(v_{t}%{c}), b) - b")
+    fl.append("END PROGRAM probe")
+    (tmp_path / "probe.f90").write_text("\n".join(fl) + "\n")
+    fexe = tmp_path / "fprobe"
+    r = subprocess.run([flang, "-module-dir", str(tmp_path), fsrc, str(tmp_path / "probe.f90"), "-o", str(fexe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    fout = subprocess.run([str(fexe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    cl = ['#include <stdio.h>', '#include <stddef.h>', '#include "ufe_dyn.h"', "int main(void){"]
+    for t, comps in types.items():
+        cl.append(f'printf("{t} size %zu\\n", sizeof({t}));')
+        for c in comps:
+            cl.append(f'printf("{t} {c} %zu\\n", offsetof({t}, {c}));')
+    cl.append("return 0;}")
+    (tmp_path / "cprobe.c").write_text("\n".join(cl))
+    cexe = tmp_path / "cprobe"
+    r = subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(tmp_path / "cprobe.c"), "-o", str(cexe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    cout = subprocess.run([str(cexe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    f = [ln.split() for ln in fout if ln.strip()]
+    c = [ln.split() for ln in cout if ln.strip()]
+    assert len(f) == len(c) > 60
+    for a, b in zip(f, c):
+        assert a[0] == b[0] and a[1].lower() == b[1].lower() and int(a[2]) == int(b[2]), (a, b)
