@@ -195,6 +195,180 @@ def test_sor_fixed_point_is_the_direct_solution():
     assert err < 1e-8, err
 
 
+def _linearised(om, ice, p):
+    """One call of the oracle's solve_SSA_linearised on the fields already set."""
+    ps = po.params_struct(p)
+    did, lm, ns = ct.c_int(0), ct.c_double(0), ct.c_int(0)
+    r = po.load().om_solve_SSA_linearised(ct.byref(om.s), ct.byref(ice.s), ct.byref(ps), ct.byref(did),
+                                          ct.byref(lm), ct.byref(ns))
+    return r, did.value, lm.value, ns.value
+
+
+def _bare_state(n=1000, seed=5):
+    m = meshgen.generate_mesh(geometry.dome_resolution(), n, seed=seed)
+    om = po.OracleMesh(m).build()
+    assert om.colour_status == 0
+    edge = np.concatenate([m.edge_index > 0, om["edge_index_Ac"] > 0])
+    return om, po.OracleIce(om), edge
+
+
+@pytest.mark.parametrize("H0", [1500.0, 0.05])
+def test_linearised_ssa_plug_flow_closed_form(H0):
+    """Uniform grounded ice with uniform eta, S and surface slope: the second-
+    derivative weights annihilate constants and Neumann rows keep them, so the
+    linearised SSA (src/ice_dynamics_module.f90:581-595, 638-644)
+        4u_xx + u_yy + 3v_xy - S u / (max(0.1,H) eta) = rho g ds/dx / eta
+    has the exact plug solution u = -rho g max(0.1,H) ds/dx / S and likewise
+    v with ds/dy.  Slopes of different size in x and y catch a u/v swap; the
+    thin-ice case catches a missing max(0.1, H); rho g is 910 * 9.81."""
+    om, ice, edge = _bare_state()
+    n = om.nVAaAc
+    gx, gy, S, eta = -2.0e-3, 7.0e-4, 5.0e3, 1.0e6
+    ice["Hi_AaAc"][:] = H0
+    ice["Hb_AaAc"][:] = 0.0                 # SL = Hb = 0: grounded at any thickness
+    ice["SL_AaAc"][:] = 0.0
+    ice["eta_AaAc"][:] = eta
+    ice["S_AaAc"][:] = S
+    ice["dHs_dx_shelf_AaAc"][:] = gx
+    ice["dHs_dy_shelf_AaAc"][:] = gy
+    ice["U_SSA_AaAc"][:] = 0.0
+    ice["V_SSA_AaAc"][:] = 0.0
+    u0 = -RHO_G * max(0.1, H0) * gx / S
+    v0 = -RHO_G * max(0.1, H0) * gy / S
+    p = DynParams(SSA_SOR_omega=1.3, SSA_max_inner_loops=100000, SSA_max_residual_UV=1e-12 * abs(u0))
+    r, did, _, sweeps = _linearised(om, ice, p)
+    assert r == 0 and did == 0 and sweeps < 100000
+    assert np.allclose(ice["U_SSA_AaAc"], u0, rtol=1e-9, atol=0)
+    assert np.allclose(ice["V_SSA_AaAc"], v0, rtol=1e-9, atol=0)
+    assert n == ice["U_SSA_AaAc"].size
+
+
+def _lhs_probe(om, ice, u, v):
+    """The oracle's left-hand sides at fixed (u, v): one sweep with omega = 0
+    updates nothing on SOR points, so LHSx/LHSy hold the discrete operator."""
+    ice["U_SSA_AaAc"][:] = u
+    ice["V_SSA_AaAc"][:] = v
+    _linearised(om, ice, DynParams(SSA_SOR_omega=0.0, SSA_max_inner_loops=1, SSA_max_residual_UV=0.0))
+    return ice["LHSx_AaAc"].copy(), ice["LHSy_AaAc"].copy()
+
+
+def _mixed_ice(om, seed=3):
+    """Non-uniform thickness, viscosity and friction; the points with x < 0
+    float (Hb = -1000 m, SL = 0, H = 500 m < 1000 * 1028/910 m), the others are
+    grounded on a bed at sea level."""
+    rng = np.random.default_rng(seed)
+    n = om.nVAaAc
+    x = om["VAaAc"][:, 0]
+    ice = po.OracleIce(om)
+    flo = x < 0
+    ice["Hi_AaAc"][:] = np.where(flo, 500.0, 1000.0 + 500.0 * rng.random(n))
+    ice["Hb_AaAc"][:] = np.where(flo, -1000.0, 0.0)
+    ice["SL_AaAc"][:] = 0.0
+    ice["eta_AaAc"][:] = 1e8 * (1.0 + rng.random(n))
+    ice["S_AaAc"][:] = 1e4 * (1.0 + rng.random(n))
+    ice["dHs_dx_shelf_AaAc"][:] = 1e-3 * rng.standard_normal(n)
+    ice["dHs_dy_shelf_AaAc"][:] = 1e-3 * rng.standard_normal(n)
+    return ice, flo
+
+
+@pytest.mark.parametrize("n", [1000, 4000])
+def test_linearised_ssa_operator_on_manufactured_quadratics(n):
+    """The discrete operator against the PDE on u = (x/L)^2, v = (y/L)^2:
+    4u_xx + u_yy = 8/L^2 and 4v_yy + v_xx = 8/L^2 (the swapped weights
+    4Nyy+Nxx on u would give 2/L^2), minus the friction term
+    S u / (max(0.1,H) eta) on grounded points only (floating points carry
+    none).  The second-derivative weights are consistent to first order, so
+    the median error over interior points is bounded and falls with the
+    resolution.  The right-hand sides are rho g ds/dx / eta pointwise."""
+    L = 800e3
+    errs = []
+    for nn in (n, 4 * n):
+        om, _, edge = _bare_state(nn, seed=7)
+        ice, flo = _mixed_ice(om)
+        X = om["VAaAc"]
+        u, v = (X[:, 0] / L) ** 2, (X[:, 1] / L) ** 2
+        lx, ly = _lhs_probe(om, ice, u, v)
+        beta = np.where(flo, 0.0, ice["S_AaAc"] / (np.maximum(0.1, ice["Hi_AaAc"]) * ice["eta_AaAc"]))
+        inner = ~edge
+        ex = np.median(np.abs(lx - (8 / L ** 2 - beta * u))[inner]) * L ** 2 / 8
+        ey = np.median(np.abs(ly - (8 / L ** 2 - beta * v))[inner]) * L ** 2 / 8
+        errs.append(max(ex, ey))
+        assert max(ex, ey) < 0.02, (nn, ex, ey)
+        # the friction term is of the same size as the curvature term here, so a
+        # friction term on floating points (or none on grounded ones) is seen
+        assert np.median((beta * u)[inner & ~flo]) * L ** 2 / 8 > 0.05
+        assert np.allclose(ice["RHSx_AaAc"], RHO_G * ice["dHs_dx_shelf_AaAc"] / ice["eta_AaAc"], rtol=1e-14)
+        assert np.allclose(ice["RHSy_AaAc"], RHO_G * ice["dHs_dy_shelf_AaAc"] / ice["eta_AaAc"], rtol=1e-14)
+    assert errs[1] < 0.5 * errs[0], errs
+
+
+def test_linearised_ssa_cross_term_is_the_reference_curvature_bug():
+    """u = 0, v = xy/L^2: the PDE's 3 v_xy = 3/L^2, but the reference takes the
+    cross curvature from get_mesh_curvatures_vertex_AaAc, which multiplies
+    every weight by the home value (src/mesh_ArakawaC_module.f90:647-656), so
+    the term is v_i times the weights' sum, which vanishes to rounding; the
+    coupling lands in the u-equation (LHSx), and LHSy sees none of it."""
+    L = 800e3
+    om, _, edge = _bare_state(2000, seed=8)
+    ice, flo = _mixed_ice(om)
+    X = om["VAaAc"]
+    v = X[:, 0] * X[:, 1] / L ** 2
+    lx, ly = _lhs_probe(om, ice, np.zeros(om.nVAaAc), v)
+    inner = ~edge
+    assert np.max(np.abs(lx[inner])) < 1e-6 * 3 / L ** 2
+    # the same v shifted by a constant changes LHSx only through that
+    # rounding-size weight sum, never by O(3/L^2)
+    lx2, _ = _lhs_probe(om, ice, np.zeros(om.nVAaAc), v + 1.0)
+    assert np.max(np.abs(lx2[inner] - lx[inner])) < 1e-6 * 3 / L ** 2
+    # v-equation: 4v_yy + v_xx = 0 for v = xy, so LHSy is the friction term alone
+    beta = np.where(flo, 0.0, ice["S_AaAc"] / (np.maximum(0.1, ice["Hi_AaAc"]) * ice["eta_AaAc"]))
+    scale = np.median((beta * np.abs(v))[inner & ~flo])
+    assert np.median(np.abs(ly + beta * v)[inner]) < 0.05 * scale
+
+
+def _rn_state():
+    wl, om, ice = _small_ssa()
+    x, y = om.mesh.V[:, 0], om.mesh.V[:, 1]
+    A = om["Aci"] - 1
+    xa = 0.5 * (om.mesh.V[A[:, 0], 0] + om.mesh.V[A[:, 1], 0])
+    ya = 0.5 * (om.mesh.V[A[:, 0], 1] + om.mesh.V[A[:, 1], 1])
+    vel = {"U_SSA": 100 * np.sin(x / 3e5) + 20, "V_SSA": 50 * np.cos(y / 2e5),
+           "Ux_SSA_Ac": 100 * np.sin(xa / 3e5) + 20, "Uy_SSA_Ac": 50 * np.cos(ya / 2e5)}
+    return wl, om, ice, vel
+
+
+def test_picard_rn_is_the_relative_l2_change_of_n():
+    """RN = sqrt(sum (N - N_prev)^2 / sum N^2) (src/ice_dynamics_module.f90:512-516):
+    with N_prev = N/2 it is exactly 1/2 (without the root 1/4, with the ratio
+    inverted 1); with a random N_prev it equals the numpy value."""
+    wl, om, ice, vel = _rn_state()
+    p = wl.params
+    p.SSA_max_outer_loops, p.SSA_max_inner_loops, p.SSA_RN_tol = 1, 1, 0.0
+    ice.set(**vel)
+    ice["N_AaAc"][:] = 0.0
+    ice.solve_SSA(p)
+    N1 = ice["N_AaAc"].copy()
+    assert np.all(N1 > 0)
+    ice.set(**vel)
+    ice["N_AaAc"][:] = 0.5 * N1
+    st = ice.solve_SSA(p)
+    assert np.array_equal(ice["N_AaAc"], N1)        # same velocities, same viscosity
+    assert abs(st.last_RN - 0.5) < 1e-14, st.last_RN
+    rng = np.random.default_rng(4)
+    Nr = N1 * np.exp(0.3 * rng.standard_normal(N1.size))
+    ice.set(**vel)
+    ice["N_AaAc"][:] = Nr
+    st = ice.solve_SSA(p)
+    ref = np.sqrt(np.sum((N1 - Nr) ** 2) / np.sum(N1 ** 2))
+    assert abs(st.last_RN - ref) < 1e-13 * ref
+    # converged on RN < SSA_RN_tol: no sweep is taken
+    ice.set(**vel)
+    ice["N_AaAc"][:] = 0.5 * N1
+    p.SSA_RN_tol = 0.5000001
+    st = ice.solve_SSA(p)
+    assert st.picard_converged == 1 and st.total_sweeps == 0
+
+
 def test_first_rn_is_one_and_failure_paths():
     wl, om, ice = _small_ssa()
     p = wl.params
