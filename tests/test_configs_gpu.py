@@ -1,0 +1,268 @@
+"""GPU parity at the benchmark configurations' own sizes.
+
+* marine 200k, plain FP64 SSA: the first Picard iterations of solve_SSA with
+  the reference SOR and stopping rules, sweep for sweep against the oracle;
+* marine 1M, mixed precision: the multigrid GMRES-IR solve reaches its FP64
+  residual as the oracle's own operator measures it, and its velocities are
+  within 1e-6 of the converged FP64 solution;
+* marine 200k adjoint: the gradient with respect to log tau_c against central
+  differences (step 1e-6) along 32 seeded directions;
+* the 64-member batch (5k-100k vertices): every member's solution satisfies
+  the oracle's linear system, and the smallest members equal the oracle's SOR
+  solution run from rest to a tight tolerance.
+
+The file sorts before the other GPU tests so that the batch meshes are
+generated in forked workers before this process opens a GPU context.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+import pytest
+
+from ufemism_amd import abi, batch, geometry
+from ufemism_amd.dynamics import Step, frozen_ssa_system
+
+pytestmark = pytest.mark.gpu
+
+SSA_FIELDS = ("Hi_AaAc", "Hb_AaAc", "SL_AaAc", "eta_AaAc", "S_AaAc", "dHs_dx_shelf_AaAc", "dHs_dy_shelf_AaAc")
+
+
+def _oracle():
+    from oracle import pyoracle
+    return pyoracle
+
+
+def _rel(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def _threads():
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def oracle_residual(om, fields, U, V, params):
+    """||b - A w|| / ||b|| of the linearised system over the SOR rows (edge rows
+    excluded), with A and b as the oracle assembles them
+    (src/ice_dynamics_module.f90:581-595, 638-644) from the given AaAc fields:
+    one oracle sweep with omega = 0 leaves w unchanged and stores LHS and RHS."""
+    import copy
+    po = _oracle()
+    ice = po.OracleIce(om)
+    for k, a in fields.items():
+        ice[k][...] = a
+    ice["U_SSA_AaAc"][...] = U
+    ice["V_SSA_AaAc"][...] = V
+    p = copy.deepcopy(params)
+    p.SSA_SOR_omega, p.SSA_max_inner_loops, p.SSA_max_residual_UV = 0.0, 1, 0.0
+    ps = po.params_struct(p)
+    did, lm, ns = ct.c_int(0), ct.c_double(0), ct.c_int(0)
+    po.load().om_solve_SSA_linearised(ct.byref(om.s), ct.byref(ice.s), ct.byref(ps), ct.byref(did),
+                                      ct.byref(lm), ct.byref(ns))
+    rows = np.concatenate([om.mesh.edge_index == 0, om["edge_index_Ac"] == 0])
+    rx = (ice["RHSx_AaAc"] - ice["LHSx_AaAc"])[rows]
+    ry = (ice["RHSy_AaAc"] - ice["LHSy_AaAc"])[rows]
+    bb = np.sum(ice["RHSx_AaAc"][rows] ** 2 + ice["RHSy_AaAc"][rows] ** 2)
+    return float(np.sqrt(np.sum(rx ** 2 + ry ** 2) / bb))
+
+
+# ---- C5: 64-member batch -----------------------------------------------------
+BATCH_PICARD = 3
+
+
+@pytest.fixture(scope="module")
+def ensemble64():
+    """64 seeded members, 5k-100k vertices log-uniform, dome or marine.  Forked
+    workers only while no GPU context exists in this process."""
+    workers = 1 if abi.contexts_opened() else min(8, _threads())
+    print(f"[C5] generating 64 members ({workers} workers)", flush=True)
+    wls = batch.ensemble(n=64, nv_min=5000, nv_max=100_000, seed=11, picard=BATCH_PICARD, workers=workers)
+    for wl in wls:
+        wl.params.SSA_RN_tol = 0.0
+    return wls
+
+
+def test_c5_batch_members_satisfy_the_oracle_system(ensemble64):
+    wls = ensemble64
+    po = _oracle()
+    po.set_threads(_threads())
+    ctx = abi.Context(0)
+    try:
+        B = batch.Batch(ctx, wls)
+        print(f"[C5] union nVAaAc={B.nA}, {sum(w.mesh.nV for w in wls)} vertices", flush=True)
+        sols = {}
+        for name, prec, pc, gm, sw in (("mixed", abi.UFE_MIXED, abi.UFE_PRECOND_MULTIGRID, 20, 2),
+                                       ("fp64", abi.UFE_FP64, abi.UFE_PRECOND_SOR, 40, 5)):
+            B.reset()
+            ctx.update_general_ice_model_data()
+            st, rel = ctx.batch_solve_SSA(BATCH_PICARD, rel_tol=1e-9, precision=prec, gmres_restart=gm,
+                                          precond_sweeps=sw, precond_type=pc)
+            print(f"  {name}: status {st.status} converged {st.converged} cycles {st.cycles} "
+                  f"max member rel {max(rel):.2e} {st.seconds:.3f}s", flush=True)
+            assert st.status == 0 and st.converged == len(wls)
+            assert max(rel) <= 1e-9
+            sols[name] = (B.member_values("U_SSA_AaAc"), B.member_values("V_SSA_AaAc"),
+                          {k: B.member_values(k) for k in SSA_FIELDS})
+        Um, Vm, _ = sols["mixed"]
+        Uf, Vf, fields = sols["fp64"]
+        sizes = np.array([w.mesh.nV for w in wls])
+        smallest = set(np.argsort(sizes)[:3].tolist())
+        worst_res, worst_mix = 0.0, 0.0
+        for b, wl in enumerate(wls):
+            om = po.OracleMesh(wl.mesh).build()
+            assert om.colour_status == 0
+            f = {k: fields[k][b] for k in SSA_FIELDS}
+            r = oracle_residual(om, f, Uf[b], Vf[b], wl.params)
+            e = np.sqrt(np.sum((Um[b] - Uf[b]) ** 2 + (Vm[b] - Vf[b]) ** 2) / np.sum(Uf[b] ** 2 + Vf[b] ** 2))
+            worst_res, worst_mix = max(worst_res, r), max(worst_mix, e)
+            # the library's residual and the oracle's agree to rounding of b - A w
+            assert r <= 1.01e-9, (b, r)
+            assert e <= 1e-6, (b, e)
+            if b in smallest:
+                # the oracle's own reference SOR from rest on the same system
+                ice = po.OracleIce(om)
+                for k, a in f.items():
+                    ice[k][...] = a
+                ice["U_SSA_AaAc"][...] = 0.0
+                ice["V_SSA_AaAc"][...] = 0.0
+                import copy
+                p = copy.deepcopy(wl.params)
+                p.SSA_max_inner_loops, p.SSA_max_residual_UV = 400000, 1e-11
+                ps = po.params_struct(p)
+                did, lm, ns = ct.c_int(0), ct.c_double(0), ct.c_int(0)
+                po.load().om_solve_SSA_linearised(ct.byref(om.s), ct.byref(ice.s), ct.byref(ps), ct.byref(did),
+                                                  ct.byref(lm), ct.byref(ns))
+                assert ns.value < 400000 and did.value == 0
+                d = np.sqrt(np.sum((Uf[b] - ice["U_SSA_AaAc"]) ** 2 + (Vf[b] - ice["V_SSA_AaAc"]) ** 2)
+                            / np.sum(ice["U_SSA_AaAc"] ** 2 + ice["V_SSA_AaAc"] ** 2))
+                print(f"  member {b} nV={wl.mesh.nV}: oracle SOR {ns.value} sweeps, batch FP64 vs oracle "
+                      f"rel L2 {d:.2e}", flush=True)
+                assert d <= 1e-7, (b, d)
+        print(f"  worst oracle residual of the FP64 members {worst_res:.2e}; worst mixed vs FP64 {worst_mix:.2e}",
+              flush=True)
+    finally:
+        ctx.close()
+
+
+# ---- C2: marine 200k, FP64 reference SSA ------------------------------------
+C2_PICARD = 10
+
+
+def test_c2_marine200k_ssa_matches_oracle(ctx):
+    """solve_SSA (src/ice_dynamics_module.f90:498-545) on the 200k marine
+    configuration, its first Picard iterations with the reference SOR and
+    stopping rules: equal sweep counts in every iteration and u, v within
+    1e-9 of the oracle; the GPU then runs the configuration's full 30."""
+    wl = geometry.marine(200_000, seed=2, picard=C2_PICARD, RN_tol=1e-6)
+    print(f"[C2] marine nV={wl.mesh.nV}, {C2_PICARD} Picard iterations", flush=True)
+    step = Step(ctx, wl)
+    out = step.run()
+    ref = _oracle().run_step(wl, threads=_threads())
+    print(f"  sweeps gpu={out['sweeps']} oracle={ref['sweeps']}", flush=True)
+    assert out["sweeps"] == ref["sweeps"]
+    for k in ("U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac", "Up_SSA_Ac", "Hi", "D_SIA_Ac"):
+        e = _rel(out[k], ref[k])
+        print(f"  {k}: rel L2 {e:.2e}", flush=True)
+        assert e <= 1e-9, (k, e)
+    wl.params.SSA_max_outer_loops = 30
+    ctx.set_params(wl.params, abi.UFE_FP64)
+    step.reset()
+    full = step.run()
+    print(f"  full run: {full['picard']} Picard, {sum(full['sweeps'])} sweeps, "
+          f"sweeps of the first {C2_PICARD} equal: {full['sweeps'][:C2_PICARD] == out['sweeps']}", flush=True)
+    assert full["sweeps"][:C2_PICARD] == out["sweeps"]
+    assert full["stats"].status in (abi.UFE_OK, abi.UFE_SOR_NOT_CONVERGED)
+
+
+# ---- C3: marine 1M, mixed precision ------------------------------------------
+def test_c3_mixed_1M_residual_and_error(ctx):
+    """The headline solve on the 1M marine mesh: mixed-precision GMRES-IR
+    with the multigrid preconditioner reaches the relative residual 1e-6 as
+    the oracle measures it on the same velocities, and at 1e-8 its velocities
+    are within 1e-6 (relative L2) of the FP64 solution solved to 1e-11."""
+    wl = geometry.marine(1_000_000, seed=11, picard=3)
+    print(f"[C3] marine nV={wl.mesh.nV}", flush=True)
+    step, st0 = frozen_ssa_system(ctx, wl, picard=3)
+    print(f"  warm-up Picard sweeps {list(st0.sweeps)[:st0.picard_iterations]}", flush=True)
+    fields = {k: ctx.get(k) for k in SSA_FIELDS}
+
+    def solve(prec, tol):
+        ctx.fill("U_SSA_AaAc", 0.0)
+        ctx.fill("V_SSA_AaAc", 0.0)
+        s = ctx.ssa_linear_solve(tol, 400000, precision=prec, gmres=20, precond_sweeps=2, orth_passes=1,
+                                 precond_type=abi.UFE_PRECOND_MULTIGRID)
+        assert s.status == 0 and s.rel_residual <= tol, (prec, tol, s.rel_residual)
+        return s, ctx.get("U_SSA_AaAc"), ctx.get("V_SSA_AaAc")
+
+    # FP64 GMRES around the FP32 cycle stagnates near 1e-12 at this size
+    s64, U64, V64 = solve(abi.UFE_FP64, 1e-11)
+    om = _oracle().OracleMesh(wl.mesh).build()
+    assert om.colour_status == 0
+    r64 = oracle_residual(om, fields, U64, V64, wl.params)
+    print(f"  fp64 to 1e-11: {s64.sweeps} sweeps, residual {s64.rel_residual:.2e}, oracle {r64:.2e}", flush=True)
+    assert r64 <= 1.01e-11
+    den = np.sqrt(np.sum(U64 ** 2 + V64 ** 2))
+    for tol in (1e-6, 1e-8):
+        s, U, V = solve(abi.UFE_MIXED, tol)
+        r = oracle_residual(om, fields, U, V, wl.params)
+        e = float(np.sqrt(np.sum((U - U64) ** 2 + (V - V64) ** 2)) / den)
+        print(f"  mixed to {tol:g}: {s.sweeps} sweeps, residual {s.rel_residual:.3e}, oracle {r:.3e}, "
+              f"velocity error vs FP64 {e:.2e}", flush=True)
+        assert abs(r - s.rel_residual) <= 1e-3 * s.rel_residual
+        if tol == 1e-8:
+            assert e <= 1e-6
+
+
+# ---- C4: adjoint on the 200k marine mesh -------------------------------------
+def test_c4_adjoint_200k_central_differences(ctx):
+    """Gradient of J = 1/2 |u - u_obs|^2 with respect to log tau_c on the 200k
+    marine mesh (u_obs from friction perturbed log-normally, sigma 0.3), after
+    5 Picard steps, against central differences (step 1e-6, FP64) along 32
+    seeded directions.  The gradient's forward and adjoint solves go to the
+    relative residual 1e-12 around the multigrid cycle.  The difference
+    quotients need J(+) - J(-) (about 2e-6 dJ) free of solver noise: their
+    forwards use FP64 GMRES around FP64 SOR sweeps to 1e-12, a map smooth in
+    the friction field (an FP32 preconditioner rounds each correction, which
+    leaves noise of the size of the tolerance in both J(+) and J(-))."""
+    torch = pytest.importorskip("torch")
+    from ufemism_amd.adjoint import AdjointOptions, SSAProblem, ssa_velocities
+    wl = geometry.marine(200_000, seed=12, picard=5)
+    print(f"[C4] marine nV={wl.mesh.nV}", flush=True)
+    step = Step(ctx, wl)
+    ctx.update_general_ice_model_data()
+    prob = SSAProblem(ctx, AdjointOptions(picard_iterations=5, lin_rel_tol=1e-12, gmres_restart=20,
+                                          precond_sweeps=2, precond_type=abi.UFE_PRECOND_MULTIGRID))
+    dev = prob.dev
+    VA = step.ma["VAaAc"]
+    log_tau0 = torch.log(torch.tensor(wl.tau_c_override(VA), dtype=torch.float64, device=dev))
+    A0 = torch.full((prob.nA,), wl.A_flow, dtype=torch.float64, device=dev)
+    g = torch.tensor(geometry.gaussian_field(VA, 50e3, 4243), dtype=torch.float64, device=dev)
+    U_obs, V_obs = prob.forward(log_tau0 + 0.3 * g, A0)
+    lt = log_tau0.clone().requires_grad_(True)
+    U, V = ssa_velocities(lt, A0, prob)
+    J = 0.5 * (((U - U_obs) ** 2).sum() + ((V - V_obs) ** 2).sum())
+    J.backward()
+    fs, bs = prob.forward_stats, prob.backward_stats
+    print(f"  forward {fs.forward_seconds:.3f}s backward {bs.backward_seconds:.3f}s "
+          f"ratio {bs.backward_seconds / fs.forward_seconds:.2f}; residuals {fs.max_forward_residual:.1e} "
+          f"{bs.max_adjoint_residual:.1e}", flush=True)
+    assert fs.max_forward_residual <= 1e-12 and bs.max_adjoint_residual <= 1e-12
+    assert bs.backward_seconds <= 3.0 * fs.forward_seconds
+    g_tau = lt.grad
+    import dataclasses
+    prob.opts = dataclasses.replace(prob.opts, precond_type=abi.UFE_PRECOND_SOR, gmres_restart=40, precond_sweeps=5)
+    h = 1e-6
+    gen = torch.Generator(device="cpu").manual_seed(31)
+    errs = []
+    for k in range(32):
+        d = torch.randn(prob.nA, generator=gen, dtype=torch.float64).to(dev)
+        ad = float((g_tau * d).sum())
+        Up, Vp = prob.forward(log_tau0 + h * d, A0)
+        Um, Vm = prob.forward(log_tau0 - h * d, A0)
+        fd = 0.5 * float(((Up - Um) * (Up + Um - 2 * U_obs)).sum() +
+                         ((Vp - Vm) * (Vp + Vm - 2 * V_obs)).sum()) / (2 * h)
+        errs.append(abs(fd - ad) / abs(ad))
+        print(f"  dir {k:2d}: adjoint {ad:+.9e} fd {fd:+.9e} rel {errs[-1]:.1e}", flush=True)
+    print(f"  worst {max(errs):.2e} median {np.median(errs):.2e}", flush=True)
+    assert max(errs) <= 1e-5

@@ -1,0 +1,170 @@
+"""Every per-operator export of the C-ABI (ufe_op_*) against the oracle's
+routine on the same seeded fields, and one dynamics step with the till yield
+stress (no friction override) against the oracle's step.
+
+The operators (src/mesh_derivatives_module.f90:315-597,
+src/mesh_ArakawaC_module.f90:542-844) are sums of products in a fixed order in
+both codes, so FP64 results agree to a few ulps of the largest term (bound
+1e-13 relative to the largest |value|) and are usually bit-identical.
+"""
+import numpy as np
+import pytest
+
+from ufemism_amd import geometry
+from ufemism_amd.dynamics import Step
+
+pytestmark = pytest.mark.gpu
+
+NZ = 15
+TOL = 1e-13
+
+
+def _oracle():
+    from oracle import pyoracle
+    return pyoracle
+
+
+@pytest.fixture(scope="module")
+def mesh_pair(ctx):
+    print("[ops] marine20k mesh on device and oracle", flush=True)
+    wl = geometry.marine(20000, seed=5, picard=1)
+    Step(ctx, wl)
+    om = _oracle().OracleMesh(wl.mesh).build()
+    return wl, om
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+
+
+def _empty(n):
+    import torch
+    return torch.full((n,), np.nan, dtype=torch.float64, device="cuda")
+
+
+def _sync():
+    """The library runs on its own stream: torch's fills and copies finish first."""
+    import torch
+    torch.cuda.synchronize()
+
+
+def _cmp(name, g, o):
+    g = g.cpu().numpy()
+    fin = np.isfinite(o)
+    assert np.array_equal(np.isfinite(g), fin), name
+    scale = max(np.abs(o[fin]).max(), 1e-300)
+    d = np.abs(g[fin] - o[fin]).max() / scale
+    print(f"  {name}: bit-exact={np.array_equal(g, o)} max|diff|/max|ref| {d:.2e}", flush=True)
+    assert d <= TOL, (name, d)
+
+
+def _fields(wl, om, seed=17):
+    """Smooth fields plus noise on Aa, Aa x nZ, Ac and AaAc points."""
+    rng = np.random.default_rng(seed)
+    V = wl.mesh.V
+    VAc = om["VAc"]
+    VAaAc = om["VAaAc"]
+
+    def f(X, k=0):
+        return (1e3 * np.sin(X[:, 0] / 2e5 + k) * np.cos(X[:, 1] / 3e5) + 0.5 * X[:, 0] / 1e3
+                + rng.standard_normal(len(X)))
+    aa3 = np.concatenate([f(V, k) for k in range(NZ)])      # column-major (nV, nZ)
+    ac3 = np.concatenate([f(VAc, k) for k in range(NZ)])
+    return f(V), aa3, f(VAc), ac3, f(VAaAc)
+
+
+def test_operator_exports_match_oracle(ctx, mesh_pair):
+    wl, om = mesh_pair
+    po = _oracle()
+    nV, nAc, nA = wl.mesh.nV, om.nAc, om.nVAaAc
+    aa, aa3, ac, ac3, aaac = _fields(wl, om)
+
+    def run(ops_name, po_name, ins, outs, nz=None):
+        """ins: host arrays; outs: sizes.  Returns nothing; compares every output."""
+        dins = [_dev(a) for a in ins]
+        douts = [_empty(n) for n in outs]
+        _sync()
+        ctx.op(ops_name, *(t.data_ptr() for t in dins + douts), nz=nz)
+        hins = [np.ascontiguousarray(a, np.float64).copy() for a in ins]
+        houts = [np.full(n, np.nan) for n in outs]
+        po.op(po_name, om, *(hins + houts), nz=nz)
+        for k, (g, o) in enumerate(zip(douts, houts)):
+            _cmp(f"{ops_name}[{k}]", g, o)
+
+    def run_inplace(ops_name, po_name, a, nz=None):
+        d = _dev(a)
+        _sync()
+        ctx.op(ops_name, d.data_ptr(), nz=nz)
+        h = np.ascontiguousarray(a, np.float64).copy()
+        po.op(po_name, om, h, nz=nz)
+        _cmp(ops_name, d, h)
+
+    print("[ops] Aa", flush=True)
+    run("derivatives_Aa", "get_mesh_derivatives", [aa], [nV, nV])
+    run("curvatures_Aa", "get_mesh_curvatures", [aa], [nV, nV, nV])
+    run("derivatives_Aa_3D", "get_mesh_derivatives_3D", [aa3], [nV * NZ, nV * NZ], nz=NZ)
+    run_inplace("neumann_Aa", "apply_Neumann_boundary", aa)
+    run_inplace("neumann_Aa_3D", "apply_Neumann_boundary_3D", aa3, nz=NZ)
+    print("[ops] Ac and AaAc", flush=True)
+    run("derivatives_Ac", "get_mesh_derivatives_Ac", [aa], [nAc, nAc, nAc, nAc])
+    run("derivatives_AaAc", "get_mesh_derivatives_AaAc", [aaac], [nA, nA])
+    run("curvatures_AaAc", "get_mesh_curvatures_AaAc", [aaac], [nA, nA, nA])
+    run_inplace("neumann_AaAc", "apply_Neumann_boundary_AaAc", aaac)
+    run("map_Aa_to_Ac", "map_Aa_to_Ac", [aa], [nAc])
+    run("map_Aa_to_Ac_3D", "map_Aa_to_Ac_3D", [aa3], [nAc * NZ], nz=NZ)
+    run("map_Ac_to_Aa", "map_Ac_to_Aa", [ac], [nV])
+    run("map_Ac_to_Aa_3D", "map_Ac_to_Aa_3D", [ac3], [nV * NZ], nz=NZ)
+    run("rotate_xy_to_po", "rotate_xy_to_po", [ac, ac[::-1].copy()], [nAc, nAc])
+
+
+def test_curvatures_Aa_closed_form(ctx, mesh_pair):
+    """get_mesh_curvatures (src/mesh_derivatives_module.f90:393-432) on the
+    device: second-derivative weights annihilate linear fields, and on
+    f = x^2 + 3xy - y^2 the interior curvatures approach (2, 3, -2) (first-order
+    consistent weights: median error a few percent at this resolution)."""
+    wl, om = mesh_pair
+    nV = wl.mesh.nV
+    x, y = wl.mesh.V[:, 0] / 1e5, wl.mesh.V[:, 1] / 1e5
+    inner = wl.mesh.edge_index == 0
+    outs = [_empty(nV) for _ in range(3)]
+    d = _dev(3.0 + 2.0 * x - 5.0 * y)
+    _sync()
+    ctx.op("curvatures_Aa", d.data_ptr(), *(t.data_ptr() for t in outs))
+    for t in outs:
+        assert np.abs(t.cpu().numpy()[inner]).max() < 1e-9
+    d = _dev(x * x + 3 * x * y - y * y)
+    _sync()
+    ctx.op("curvatures_Aa", d.data_ptr(), *(t.data_ptr() for t in outs))
+    for t, exact in zip(outs, (2.0, 3.0, -2.0)):
+        e = np.median(np.abs(t.cpu().numpy()[inner] * 1e10 - exact))
+        print(f"  curvature exact {exact}: median |error| {e:.3e}", flush=True)
+        assert e < 0.05 * 3.0
+
+
+def test_step_with_till_yield_stress_matches_oracle(ctx):
+    """solve_SSA without the friction override: basal_yield_stress
+    (src/ice_dynamics_module.f90:780-844) computes phi_fric and tau_c from the
+    bed and the thickness on Aa and Ac; the device values, the sweep counts and
+    the step's velocities and thickness equal the oracle's."""
+    print("[tau_c] marine20k step with the till yield stress", flush=True)
+    wl = geometry.marine(20000, seed=5, picard=8)
+    wl.params.use_tau_c_override = False
+    step = Step(ctx, wl)
+    out = step.run()
+    phi = ctx.get("phi_fric_AaAc")
+    ref = _oracle().run_step(wl)
+    print(f"  sweeps gpu={out['sweeps']} oracle={ref['sweeps']}", flush=True)
+    assert out["sweeps"] == ref["sweeps"]
+    ophi = ref["ice"]["phi_fric_AaAc"]
+    for name, g, o in (("tau_c_AaAc", out["tau_c_AaAc"], ref["tau_c_AaAc"]), ("phi_fric_AaAc", phi, ophi)):
+        d = np.abs(g - o).max() / np.abs(o).max()
+        print(f"  {name}: bit-exact={np.array_equal(g, o)} max|diff|/max|ref| {d:.2e}", flush=True)
+        assert d <= 1e-14, (name, d)
+    # the till law gives friction from 5 to 20 degrees and varies over the bed
+    assert ophi.min() >= 5.0 and ophi.max() <= 20.0 and np.ptp(out["tau_c_AaAc"]) > 0
+    for k in ("U_SSA", "V_SSA", "Ux_SSA_Ac", "Uy_SSA_Ac", "Hi"):
+        a, b = out[k], ref[k]
+        e = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"  {k}: rel L2 {e:.2e}", flush=True)
+        assert e <= 1e-9, (k, e)

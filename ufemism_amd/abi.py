@@ -308,12 +308,23 @@ class MeshArrays:
         return d
 
 
+_contexts_opened = 0
+
+
+def contexts_opened() -> int:
+    """Contexts created in this process so far: after the first, the process
+    holds a HIP runtime and must not fork workers."""
+    return _contexts_opened
+
+
 class Context:
     """One device context: uploaded mesh plus device-resident ice fields."""
 
     def __init__(self, device=0):
+        global _contexts_opened
         self.lib = load()
         h = ct.c_void_p()
+        _contexts_opened += 1
         r = self.lib.ufe_ctx_create(int(device), ct.byref(h))
         if r != UFE_OK:
             raise UFEError(r, "ufe_ctx_create failed (no HIP device?)")

@@ -65,6 +65,31 @@ class Step:
         return out
 
 
+def frozen_ssa_system(ctx: abi.Context, wl, picard=3, warm_tol=1e-6):
+    """Set up a workload and freeze one linearised SSA system at a flowing
+    state: general update, SIA, then `picard` Picard steps of solve_SSA whose
+    linear systems are solved to the relative residual `warm_tol` (FP64
+    Krylov, multigrid-preconditioned), and finally ufe_ssa_prepare, which
+    rebuilds viscosity, sliding and coefficients from those velocities (the
+    next Picard step's system, src/ice_dynamics_module.f90:501-531).  The
+    reference's own SOR rule (max |dU| < 2.5 m/yr) stops after one sweep on
+    fine meshes from rest, so it would leave the velocities near zero.  The
+    parameters are left at the workload's FP64 reference setting."""
+    import copy
+    step = Step(ctx, wl)
+    ctx.update_general_ice_model_data()
+    ctx.solve_SIA()
+    p = copy.deepcopy(wl.params)
+    p.SSA_max_outer_loops = picard
+    p.SSA_RN_tol = 0.0
+    ctx.set_params(p, abi.UFE_FP64_KRYLOV, linear_rel_tol=warm_tol, gmres_restart=20,
+                   precond_sweeps=2, precond_type=abi.UFE_PRECOND_MULTIGRID)
+    st = ctx.solve_SSA()
+    ctx.set_params(wl.params, abi.UFE_FP64)
+    ctx.ssa_prepare()
+    return step, st
+
+
 @dataclasses.dataclass
 class TimeConfig:
     """Time-step configuration, src/configuration_module.f90:34-44 (defaults)."""
