@@ -219,7 +219,7 @@ def test_c4_adjoint_200k_central_differences(ctx):
     """Gradient of J = 1/2 |u - u_obs|^2 with respect to log tau_c on the 200k
     marine mesh (u_obs from friction perturbed log-normally, sigma 0.3), after
     5 Picard steps, against central differences (step 1e-6, FP64) along 32
-    seeded directions.  The gradient's forward and adjoint solves go to the
+    seeded directions: |fd - g.d| <= 1e-5 |g| in every direction.  The gradient's forward and adjoint solves go to the
     relative residual 1e-11 around the multigrid cycle (FP64 Krylov around the
     FP32 cycle stagnates near 1e-12 at this size).  The difference
     quotients need J(+) - J(-) (about 2e-6 dJ) free of solver noise: their
@@ -251,6 +251,7 @@ def test_c4_adjoint_200k_central_differences(ctx):
     assert fs.max_forward_residual <= 1e-11 and bs.max_adjoint_residual <= 1e-11
     assert bs.backward_seconds <= 3.0 * fs.forward_seconds
     g_tau = lt.grad
+    gnorm = float(torch.linalg.norm(g_tau))
     import dataclasses
     prob.opts = dataclasses.replace(prob.opts, lin_rel_tol=1e-12, precond_type=abi.UFE_PRECOND_SOR,
                                     gmres_restart=40, precond_sweeps=5)
@@ -264,7 +265,11 @@ def test_c4_adjoint_200k_central_differences(ctx):
         Um, Vm = prob.forward(log_tau0 - h * d, A0)
         fd = 0.5 * float(((Up - Um) * (Up + Um - 2 * U_obs)).sum() +
                          ((Vp - Vm) * (Vp + Vm - 2 * V_obs)).sum()) / (2 * h)
-        errs.append(abs(fd - ad) / abs(ad))
-        print(f"  dir {k:2d}: adjoint {ad:+.9e} fd {fd:+.9e} rel {errs[-1]:.1e}", flush=True)
-    print(f"  worst {max(errs):.2e} median {np.median(errs):.2e}", flush=True)
+        errs.append(abs(fd - ad) / gnorm)
+        print(f"  dir {k:2d}: adjoint {ad:+.9e} fd {fd:+.9e} rel {abs(fd - ad) / abs(ad):.1e} "
+              f"vs |g| {errs[-1]:.1e}", flush=True)
+    print(f"  |g| {gnorm:.3e}; worst |fd - adjoint| / |g| {max(errs):.2e}, median {np.median(errs):.2e}", flush=True)
+    # d has independent unit normal entries, so g.d is typically |g|: the error is
+    # measured against that scale, which a direction whose derivative happens to
+    # cancel (g.d << |g|) does not shrink
     assert max(errs) <= 1e-5
