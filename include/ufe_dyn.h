@@ -355,6 +355,22 @@ int ufe_multigrid_info_get(ufe_ctx *ctx, ufe_multigrid_info *out);
 /* FP64 relative residual of the current velocities. */
 int ufe_ssa_residual(ufe_ctx *ctx, double *rel_residual);
 
+/* No counterpart in the reference: an allocation-free variant for callers
+ * that cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc).  It is the plain FP64 SOR path of ufe_ssa_linear_solve
+ * (precision UFE_FP64, gmres_restart 0, method 0: the reference's sweeps of
+ * src/ice_dynamics_module.f90:601-691, stopped on the FP64 relative residual
+ * every check_interval sweeps) with the same sweeps, velocities and
+ * statistics.  Its residual partial sums live in a file-scope device array of
+ * the library (512 KiB on each GPU it runs on, kept until the process ends)
+ * instead of context buffers; it calls no allocating routine.  Meshes with
+ * more than 8,388,608 SOR rows (nVAaAc) are refused with UFE_BAD_INPUT.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (from
+ * several contexts or threads) race.  omega <= 0: params.SSA_SOR_omega;
+ * check_interval <= 0: 16. */
+int ufe_ssa_sor_solve_noalloc(ufe_ctx *ctx, double rel_tol, int32_t max_sweeps, double omega,
+                              int32_t check_interval, ufe_linsolve_stats *st);
+
 /* ---- adjoint of the SSA solve ------------------------------------------- */
 /* No counterpart in the reference.  The forward runs exactly
  * picard_iterations Picard steps of solve_SSA (src/ice_dynamics_module.f90:

@@ -38,7 +38,7 @@ EXPORTS = [
     "ufe_update_general_ice_model_data", "ufe_ice_physical_properties", "ufe_solve_SIA", "ufe_solve_SSA",
     "ufe_calculate_ice_thickness_change", "ufe_solve_SIA_3D", "ufe_critical_timesteps",
     "ufe_op_upwind_derivatives_3D", "ufe_ssa_prepare", "ufe_ssa_linear_solve",
-    "ufe_ssa_residual",
+    "ufe_ssa_residual", "ufe_ssa_sor_solve_noalloc",
     "ufe_op_derivatives_Aa", "ufe_op_curvatures_Aa", "ufe_op_derivatives_Aa_3D",
     "ufe_op_neumann_Aa", "ufe_op_neumann_Aa_3D", "ufe_op_derivatives_Ac",
     "ufe_op_derivatives_AaAc", "ufe_op_curvatures_AaAc", "ufe_op_neumann_AaAc",
@@ -225,6 +225,7 @@ def load():
         lib.ufe_field_id.argtypes = [ct.c_char_p]
         lib.ufe_calculate_ice_thickness_change.argtypes = [_P, ct.c_double]
         lib.ufe_ssa_linear_solve.argtypes = [_P, _P, _P]
+        lib.ufe_ssa_sor_solve_noalloc.argtypes = [_P, ct.c_double, ct.c_int32, ct.c_double, ct.c_int32, _P]
         lib.ufe_field_fill.argtypes = [_P, ct.c_int, ct.c_double]
         for name in EXPORTS:
             getattr(lib, name)  # raises AttributeError if a symbol is missing
@@ -515,6 +516,15 @@ class Context:
         st = LinSolveStats()
         self._chk(self.lib.ufe_ssa_linear_solve(self.h, ct.byref(o), ct.byref(st)),
                   "ssa_linear_solve")
+        return st
+
+    def ssa_sor_solve_noalloc(self, rel_tol, max_sweeps, omega=0.0, check_interval=16) -> LinSolveStats:
+        """The plain FP64 SOR solve without any device allocation (scratch in a
+        static device array of the library); the same results as
+        ssa_linear_solve(rel_tol, max_sweeps, UFE_FP64, omega, check_interval)."""
+        st = LinSolveStats()
+        self._chk(self.lib.ufe_ssa_sor_solve_noalloc(self.h, float(rel_tol), int(max_sweeps), float(omega),
+                                                     int(check_interval), ct.byref(st)), "ssa_sor_solve_noalloc")
         return st
 
     def multigrid_info(self) -> dict:

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -195,6 +196,7 @@ struct ufe_ctx {
   double *d_part = nullptr;         // block partial sums
   int n_part = 0;
   double *d_scalar = nullptr;       // small device scalars
+  double *noalloc_scratch = nullptr;   // this device's copy of `scratch` (ufe_ssa_sor_solve_noalloc)
   double *d_tmp[4] = {nullptr, nullptr, nullptr, nullptr};   // nA-sized temporaries
   double *d_ViSMB = nullptr;
   unsigned long long *d_count = nullptr;
@@ -512,6 +514,37 @@ static ufe_params default_params() {
 // ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
+// ---- scratch of the allocation-free SOR solve -------------------------------
+// ufe_ssa_sor_solve_noalloc keeps the FP64 residual's block partial sums and
+// its result here instead of in context buffers.  Each GPU holds its own copy
+// for the life of the process; host code reaches it only through the address
+// hipGetSymbolAddress gives for that device (the array's name is a host-side
+// stand-in, not a device address).
+constexpr int NOALLOC_MAX_ROWS = 1 << 23;                      // SOR rows (nVAaAc) at most
+constexpr int NOALLOC_PARTS = NOALLOC_MAX_ROWS / BLOCK + 8;    // residual blocks: one extra per colour
+constexpr int NOALLOC_DOUBLES = 2 * NOALLOC_PARTS + 8;         // partial pairs, then the result
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+constexpr int NOALLOC_DEVICES = 64;
+static double *g_scratch_addr[NOALLOC_DEVICES];
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on `device`, which must be current; looked up
+// once per device.
+static int noalloc_scratch_setup(int device, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= NOALLOC_DEVICES) return UFE_BAD_INPUT;
+  if (!g_scratch_addr[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return UFE_HIP_ERROR;
+    }
+    g_scratch_addr[device] = static_cast<double *>(p);
+  }
+  *out = g_scratch_addr[device];
+  return UFE_OK;
+}
+
 // An exception from host code (memory or threads exhausted) ends the entry
 // point it reached with UFE_HOST_ERROR; none leaves through the C interface.
 static int host_failure(ufe_ctx *ctx, const char *what) {
@@ -539,6 +572,9 @@ int ufe_ctx_create(int device, ufe_ctx **out) try {
     delete ctx;
     return UFE_HIP_ERROR;
   }
+  // the allocation-free solve's scratch on this device (left null on failure:
+  // that entry point then refuses with UFE_HIP_ERROR, the others are unaffected)
+  if (noalloc_scratch_setup(device, &ctx->noalloc_scratch) != UFE_OK) ctx->noalloc_scratch = nullptr;
   *out = ctx;
   return UFE_OK;
 } catch (const std::exception &e) {
@@ -2611,7 +2647,9 @@ int ufe_ssa_prepare(ufe_ctx *ctx) {
 }
 
 // FP64 relative residual; with store=true also hands r to the FP32 sweeps.
-static int residual_fp64(ufe_ctx *ctx, double *rel, bool store = false) {
+// The FP64 relative residual with its block partial sums in `part` (at most
+// sum over colours of nblk(rows) pairs) and the result in *scalar.
+static int residual_fp64_into(ufe_ctx *ctx, double *part, double *scalar, double *rel, bool store) {
   const AaAcMesh m = aaac_mesh(ctx);
   const uint8_t *skip = ctx->gl_active ? ctx->d_skip : nullptr;
   int nb_tot = 0;
@@ -2621,20 +2659,25 @@ static int residual_fp64(ufe_ctx *ctx, double *rel, bool store = false) {
     const int nb = nblk(n);
     if (store)
       k_residual_store<<<nb, BLOCK, 0, ctx->stream>>>(m, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
-                                                       ctx->colour_start[c], n, ctx->d_R32, ctx->d_part + 2 * nb_tot,
+                                                       ctx->colour_start[c], n, ctx->d_R32, part + 2 * nb_tot,
                                                        skip);
     else
       k_ssa_residual<<<nb, BLOCK, 0, ctx->stream>>>(m, ctx->d_pair[0], ctx->d_pair[1], ctx->d_pair[2],
-                                                     ctx->colour_start[c], n, ctx->d_part + 2 * nb_tot, skip);
+                                                     ctx->colour_start[c], n, part + 2 * nb_tot, skip);
     nb_tot += nb;
   }
-  k_ssa_rn<<<1, 1024, 0, ctx->stream>>>(nb_tot, ctx->d_part, ctx->d_scalar);
+  k_ssa_rn<<<1, 1024, 0, ctx->stream>>>(nb_tot, part, scalar);
   LAUNCHCHK(ctx);
-  HIPCHK(ctx, hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->h_scalar, scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  prof_collect(ctx);
   *rel = ctx->h_scalar[0];
   return UFE_OK;
+}
+
+static int residual_fp64(ufe_ctx *ctx, double *rel, bool store = false) {
+  const int r = residual_fp64_into(ctx, ctx->d_part, ctx->d_scalar, rel, store);
+  if (r == UFE_OK) prof_collect(ctx);
+  return r;
 }
 
 int ufe_ssa_residual(ufe_ctx *ctx, double *rel_residual) {
@@ -3053,6 +3096,57 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
   return host_failure(ctx, "unknown exception");
 }
 
+
+// The plain FP64 SOR path of ufe_ssa_linear_solve with the residual's scratch
+// in the file-scope array: the same kernels, launch shapes and summation
+// order, so the same sweeps and bit-identical velocities.  Nothing is
+// allocated: no hipMalloc, no events (kernel profiling is suspended for the
+// call), the result is read through the context's pinned scalar.
+int ufe_ssa_sor_solve_noalloc(ufe_ctx *ctx, double rel_tol, int32_t max_sweeps, double omega,
+                              int32_t check_interval, ufe_linsolve_stats *st) try {
+  if (!ctx || !ctx->have_mesh || !st) return UFE_BAD_INPUT;
+  std::memset(st, 0, sizeof(*st));
+  if (ctx->nA > NOALLOC_MAX_ROWS) {
+    ctx->err = "ufe_ssa_sor_solve_noalloc: the mesh has more SOR rows (nVAaAc) than its scratch holds "
+               "(at most 8388608)";
+    return UFE_BAD_INPUT;
+  }
+  if (!ctx->noalloc_scratch) {
+    ctx->err = "ufe_ssa_sor_solve_noalloc: no device address for the scratch array on this device";
+    return UFE_HIP_ERROR;
+  }
+  double *part = ctx->noalloc_scratch, *scalar = ctx->noalloc_scratch + 2 * NOALLOC_PARTS;
+  struct ProfPause {      // event pairs would be created on demand: none during this call
+    ufe_ctx *c;
+    bool was;
+    ~ProfPause() { c->prof = was; }
+  } pause{ctx, ctx->prof};
+  ctx->prof = false;
+  const AaAcMesh m = aaac_mesh(ctx);
+  const double om = omega > 0.0 ? omega : ctx->params.SSA_SOR_omega;
+  const int check = check_interval > 0 ? check_interval : 16;
+  const auto t0 = std::chrono::steady_clock::now();
+  double rel = 0.0;
+  int r;
+  if ((r = residual_fp64_into(ctx, part, scalar, &rel, false))) return r;
+  st->refinements = 1;
+  while (rel > rel_tol && st->sweeps < max_sweeps) {
+    const int nb = std::min(check, max_sweeps - st->sweeps);
+    for (int b = 0; b < nb; b++) enqueue_sweep_fp64(ctx, m, om, 0.0, 1 << 30, false);
+    LAUNCHCHK(ctx);
+    st->sweeps += nb;
+    if ((r = residual_fp64_into(ctx, part, scalar, &rel, false))) return r;
+    st->refinements++;
+  }
+  st->rel_residual = rel;
+  st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  st->status = rel <= rel_tol ? UFE_OK : UFE_SOR_NOT_CONVERGED;
+  return UFE_OK;
+} catch (const std::exception &e) {
+  return host_failure(ctx, e.what());
+} catch (...) {
+  return host_failure(ctx, "unknown exception");
+}
 
 // ---- adjoint of the SSA solve ------------------------------------------------
 // (no reference counterpart; the forward mirrors the Picard loop of
