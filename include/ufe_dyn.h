@@ -350,6 +350,9 @@ typedef struct ufe_multigrid_info {
   int32_t direct;                   /* the coarsest level is solved by a dense inverse */
   int32_t kstep[16];                /* level solved by two cycles (K-cycle) */
   double setup_seconds;
+  int32_t tail_level;               /* first of the small levels solved in one workgroup per visit
+                                       (0: none; UFE_AMG_TAIL=0 in the environment at set-up turns
+                                       it off) */
 } ufe_multigrid_info;
 int ufe_multigrid_info_get(ufe_ctx *ctx, ufe_multigrid_info *out);
 /* FP64 relative residual of the current velocities. */
@@ -419,11 +422,12 @@ int ufe_ssa_adjoint_backward(ufe_ctx *ctx, const double *dJ_dU, const double *dJ
                              double *grad_log_tau_c, double *grad_A, ufe_adjoint_stats *st);
 
 /* ---- autotuned launch shape of the FP32 sweep ----------------------------- */
-/* No counterpart in the reference.  The FP32 colour pass has 24 launch
+/* No counterpart in the reference.  The FP32 colour pass has 32 launch
  * shapes: threads per block 64/128/256/512, block order (schedule 0: XCD-major
  * bands of rows, 1: round-robin) and load variant (0: plain, 1: non-temporal
  * loads for the once-streamed coefficients, 2: as 1 with eight slots' loads in
- * flight).  All give identical results.  The autotuner times `sweeps` sweeps
+ * flight, 3: plain loads, all six slots' loads of a six-slot row in one batch).
+ * All give identical results.  The autotuner times `sweeps` sweeps
  * of the uploaded mesh (single or batch) per shape and keeps the fastest for
  * every later FP32 sweep of the context (default: 256, 0, 0). */
 typedef struct ufe_sweep_tuning {

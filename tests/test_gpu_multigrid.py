@@ -247,3 +247,45 @@ def test_multigrid_on_tiny_meshes(n):
             assert _rel(b[k], a[k]) <= 1e-7, (k, _rel(b[k], a[k]))
     finally:
         ctx.close()
+
+
+def test_small_level_kernel_is_bit_identical():
+    """The levels below the last K-step level solved by one workgroup per visit
+    (k_amg_tail) give the same bits as one kernel per step (UFE_AMG_TAIL=0 at
+    set-up): the forward mixed solve and both directions of the multigrid
+    adjoint, whose backward cycles the transposed hierarchy."""
+    import os
+    import torch
+    from ufemism_amd.adjoint import AdjointOptions, SSAProblem, ssa_velocities
+    out = {}
+    try:
+        for flag in ("1", "0"):
+            os.environ["UFE_AMG_TAIL"] = flag
+            ctx = abi.Context(0)
+            try:
+                wl, step = _linear_system(ctx)
+                st, uv = _solve(ctx, 1e-9, precision=abi.UFE_MIXED, gmres=20, precond_sweeps=2,
+                                precond_type=abi.UFE_PRECOND_MULTIGRID)
+                info = ctx.multigrid_info()
+                print(f"  UFE_AMG_TAIL={flag}: hierarchy {info['points']} tail level {info['tail_level']} "
+                      f"matvecs {st.matvecs}", flush=True)
+                assert st.status == 0
+                assert (info["tail_level"] > 0) == (flag == "1")
+                step.reset()
+                ctx.update_general_ice_model_data()
+                prob = SSAProblem(ctx, AdjointOptions(picard_iterations=3, lin_rel_tol=1e-10, gmres_restart=20,
+                                                      precond_sweeps=2, precond_type=abi.UFE_PRECOND_MULTIGRID))
+                lt = torch.log(torch.tensor(wl.tau_c_override(step.ma["VAaAc"]), dtype=torch.float64,
+                                            device=prob.dev)).requires_grad_(True)
+                A0 = torch.full((prob.nA,), wl.A_flow, dtype=torch.float64, device=prob.dev)
+                U, V = ssa_velocities(lt, A0, prob)
+                (0.5 * (U ** 2 + V ** 2).sum()).backward()
+                out[flag] = (st.matvecs, uv, U.detach().cpu().numpy(), lt.grad.cpu().numpy())
+            finally:
+                ctx.close()
+    finally:
+        os.environ.pop("UFE_AMG_TAIL", None)
+    a, b = out["1"], out["0"]
+    assert a[0] == b[0]
+    for k in (1, 2, 3):
+        assert np.array_equal(a[k], b[k]), k

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -271,6 +272,7 @@ struct ufe_ctx {
   double2 *amg_y64 = nullptr;       // FP64 fine-level A z (adjoint)
   float2 *amg_v32 = nullptr, *amg_z32 = nullptr;   // FP32 copies around an FP64 Krylov method
   double amg_setup_seconds = 0.0;
+  int amg_tail_l0 = 0;              // first level solved by k_amg_tail (0: none)
 };
 
 #define HIPCHK(ctx, x)                                                            \
@@ -1952,6 +1954,13 @@ static NeumannList cor_list(ufe_ctx *c) {
   return L;
 }
 
+// The edge rows of w from the Neumann condition, in FP64 (a correction
+// computed in FP32 satisfies it only to FP32 rounding of the correction).
+static void neumann_w(ufe_ctx *ctx) {
+  if (ctx->n_neu) k_neumann_edges<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), ctx->d_pair[0], nullptr);
+  k_neumann_scalar_pair_corners<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), ctx->d_pair[0]);
+}
+
 static hipEvent_t prof_event(ufe_ctx *ctx) {
   if (ctx->ev_pool.empty()) {
     hipEvent_t e;
@@ -2038,27 +2047,35 @@ static void launch_sor_f32(ufe_ctx *ctx, int bs, int sched, int loads, const AaA
     case 512: launch_sor_f32_shape<64, 0, 0>(ctx, m, q, D, R, c, omega); break;
     case 513: launch_sor_f32_shape<64, 0, 1>(ctx, m, q, D, R, c, omega); break;
     case 514: launch_sor_f32_shape<64, 0, 2>(ctx, m, q, D, R, c, omega); break;
+    case 515: launch_sor_f32_shape<64, 0, 3>(ctx, m, q, D, R, c, omega); break;
     case 516: launch_sor_f32_shape<64, 1, 0>(ctx, m, q, D, R, c, omega); break;
     case 517: launch_sor_f32_shape<64, 1, 1>(ctx, m, q, D, R, c, omega); break;
     case 518: launch_sor_f32_shape<64, 1, 2>(ctx, m, q, D, R, c, omega); break;
+    case 519: launch_sor_f32_shape<64, 1, 3>(ctx, m, q, D, R, c, omega); break;
     case 1024: launch_sor_f32_shape<128, 0, 0>(ctx, m, q, D, R, c, omega); break;
     case 1025: launch_sor_f32_shape<128, 0, 1>(ctx, m, q, D, R, c, omega); break;
     case 1026: launch_sor_f32_shape<128, 0, 2>(ctx, m, q, D, R, c, omega); break;
+    case 1027: launch_sor_f32_shape<128, 0, 3>(ctx, m, q, D, R, c, omega); break;
     case 1028: launch_sor_f32_shape<128, 1, 0>(ctx, m, q, D, R, c, omega); break;
     case 1029: launch_sor_f32_shape<128, 1, 1>(ctx, m, q, D, R, c, omega); break;
     case 1030: launch_sor_f32_shape<128, 1, 2>(ctx, m, q, D, R, c, omega); break;
+    case 1031: launch_sor_f32_shape<128, 1, 3>(ctx, m, q, D, R, c, omega); break;
     case 2048: launch_sor_f32_shape<256, 0, 0>(ctx, m, q, D, R, c, omega); break;
     case 2049: launch_sor_f32_shape<256, 0, 1>(ctx, m, q, D, R, c, omega); break;
     case 2050: launch_sor_f32_shape<256, 0, 2>(ctx, m, q, D, R, c, omega); break;
+    case 2051: launch_sor_f32_shape<256, 0, 3>(ctx, m, q, D, R, c, omega); break;
     case 2052: launch_sor_f32_shape<256, 1, 0>(ctx, m, q, D, R, c, omega); break;
     case 2053: launch_sor_f32_shape<256, 1, 1>(ctx, m, q, D, R, c, omega); break;
     case 2054: launch_sor_f32_shape<256, 1, 2>(ctx, m, q, D, R, c, omega); break;
+    case 2055: launch_sor_f32_shape<256, 1, 3>(ctx, m, q, D, R, c, omega); break;
     case 4096: launch_sor_f32_shape<512, 0, 0>(ctx, m, q, D, R, c, omega); break;
     case 4097: launch_sor_f32_shape<512, 0, 1>(ctx, m, q, D, R, c, omega); break;
     case 4098: launch_sor_f32_shape<512, 0, 2>(ctx, m, q, D, R, c, omega); break;
+    case 4099: launch_sor_f32_shape<512, 0, 3>(ctx, m, q, D, R, c, omega); break;
     case 4100: launch_sor_f32_shape<512, 1, 0>(ctx, m, q, D, R, c, omega); break;
     case 4101: launch_sor_f32_shape<512, 1, 1>(ctx, m, q, D, R, c, omega); break;
     case 4102: launch_sor_f32_shape<512, 1, 2>(ctx, m, q, D, R, c, omega); break;
+    case 4103: launch_sor_f32_shape<512, 1, 3>(ctx, m, q, D, R, c, omega); break;
     default: launch_sor_f32_shape<256, 0, 0>(ctx, m, q, D, R, c, omega); break;
   }
 }
@@ -2114,6 +2131,7 @@ static ActiveRanges active_ranges(ufe_ctx *ctx);
 constexpr int AMG_KSTEP_MIN = 1000;      // levels with at least this many points take two cycles
 constexpr float AMG_BJ_OMEGA = 0.8f;     // damping of the coarse-level block Jacobi
 constexpr int AMG_COARSE_SWEEPS = 20;    // coarsest level too large for the dense inverse
+constexpr size_t AMG_TAIL_LDS_MAX = 61 * 1024;   // dynamic LDS of k_amg_tail (vectors, staged couplings)
 constexpr int AMG_MAX_LEVELS = 16;
 
 static int amg_setup(ufe_ctx *ctx) {
@@ -2262,6 +2280,20 @@ static int amg_setup(ufe_ctx *ctx) {
     if ((r = dalloc(ctx, &ctx->amg_inv, N * N))) return r;
   }
   HIPCHK(ctx, hipDeviceSynchronize());
+  // The levels below the last K-step level, each visited by one V-cycle, are
+  // solved by one workgroup per visit (k_amg_tail) when their vectors fit in
+  // LDS; UFE_AMG_TAIL=0 in the environment keeps one kernel per step.
+  ctx->amg_tail_l0 = 0;
+  {
+    const char *env = std::getenv("UFE_AMG_TAIL");
+    int l0 = L - 1;
+    while (l0 > 1 && !ctx->amg[l0 - 1].kstep) l0--;
+    size_t lds = 0;
+    for (int l = l0; l < L; l++) lds += 4 * (size_t)ctx->amg[l].n * sizeof(float2);
+    const int nlev = L - l0;
+    if (!(env && env[0] == '0') && nlev >= 2 && nlev <= AMG_TAIL_LEVELS && lds <= AMG_TAIL_LDS_MAX)
+      ctx->amg_tail_l0 = l0;
+  }
   ctx->amg_ready = true;
   ctx->amg_setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return UFE_OK;
@@ -2331,9 +2363,52 @@ static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
   }
 }
 
+// Levels l0..L-1 (no K-steps) as one V-cycle in one workgroup: A_l0 e1 = b
+static void amg_tail_launch(ufe_ctx *ctx, int l0) {
+  const int L = (int)ctx->amg.size();
+  AmgTail T{};
+  T.nlev = L - l0;
+  int off = 0;
+  for (int k = 0; k < T.nlev; k++) {
+    AmgLevel &V = ctx->amg[l0 + k];
+    T.L[k].A = amg_op(ctx, V);
+    T.L[k].nnz = (int)V.nnz;
+    T.L[k].mptr = V.mptr;
+    T.L[k].mem = V.mem;
+    T.L[k].agg = V.agg;
+    T.L[k].lds = off;
+    off += 4 * V.n;
+  }
+  // the couplings of as many levels as fit follow the vectors (coarsest first:
+  // it is visited most often inside one call)
+  size_t bytes = (size_t)off * sizeof(float2);
+  for (int k = T.nlev - 1; k >= 0; k--) {
+    const size_t need = (size_t)T.L[k].nnz * (sizeof(float2) + sizeof(int32_t)) +
+                        ((size_t)T.L[k].A.n + 1) * sizeof(int32_t);
+    const size_t at = (bytes + 7) & ~(size_t)7;
+    T.L[k].stage = -1;
+    if (at + need <= AMG_TAIL_LDS_MAX) {
+      T.L[k].stage = (int)at;
+      bytes = at + need;
+    }
+  }
+  T.inv = ctx->amg_direct ? ctx->amg_inv : nullptr;
+  T.trans = ctx->amg_T ? 1 : 0;
+  T.coarse_sweeps = AMG_COARSE_SWEEPS;
+  T.nu = std::max(1, ctx->amg_nu);
+  T.omega = AMG_BJ_OMEGA;
+  T.b_in = ctx->amg[l0].b;
+  T.x_out = ctx->amg[l0].e1;
+  k_amg_tail<<<1, AMG_TAIL_THREADS, bytes, ctx->stream>>>(T);
+}
+
 // A_l e = b_l: one cycle into e1, or (kstep) two cycles combined by GCR(2),
 // the coefficients left in sc[6], sc[7] for the prolongation
 static void amg_solve_level(ufe_ctx *ctx, int l) {
+  if (ctx->amg_tail_l0 > 0 && l == ctx->amg_tail_l0) {
+    amg_tail_launch(ctx, l);
+    return;
+  }
   AmgLevel &V = ctx->amg[l];
   amg_cycle(ctx, l, V.b, V.e1);
   if (!V.kstep) return;
@@ -2862,6 +2937,7 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
   HIPCHK(ctx, hipMemcpyAsync(ctx->kry_y, y.data(), sizeof(double) * kk, hipMemcpyHostToDevice, ctx->stream));
   k_lincomb<T><<<nb, BLOCK, 0, ctx->stream>>>(n, kk, Z, ctx->kry_y, x);
   k_add_to_w<T><<<nb, BLOCK, 0, ctx->stream>>>(n, x, ctx->d_pair[0]);
+  neumann_w(ctx);
   LAUNCHCHK(ctx);
   return UFE_OK;
 }
@@ -2949,6 +3025,7 @@ static int bicg_inner(ufe_ctx *ctx, int ks, double omega, double reduce, int max
     }
   }
   k_add_to_w<T><<<nb, BLOCK, 0, ctx->stream>>>(n, x, ctx->d_pair[0]);
+  neumann_w(ctx);
   LAUNCHCHK(ctx);
   return UFE_OK;
 }
@@ -2966,6 +3043,7 @@ int ufe_multigrid_info_get(ufe_ctx *ctx, ufe_multigrid_info *out) {
   }
   out->direct = ctx->amg_direct ? 1 : 0;
   out->setup_seconds = ctx->amg_setup_seconds;
+  out->tail_level = ctx->amg_tail_l0;
   return UFE_OK;
 }
 
@@ -3078,6 +3156,7 @@ int ufe_ssa_linear_solve(ufe_ctx *ctx, const ufe_linsolve_opts *o, ufe_linsolve_
       k_fill<float2><<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->d_D32, ctx->nA, make_float2(0.0f, 0.0f));
       for (int b = 0; b < nb; b++) enqueue_sweep_fp32(ctx, m, (float)omega);
       k_apply_correction<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_D32, ctx->d_pair[0]);
+      neumann_w(ctx);
     } else {
       for (int b = 0; b < nb; b++) enqueue_sweep_fp64(ctx, m, omega, 0.0, 1 << 30, false);
     }
@@ -3604,6 +3683,7 @@ static void batch_enqueue_cycle(ufe_ctx *ctx, int m, int ks, double omega, doubl
   k_blsq<<<NB, 64, 0, ctx->stream>>>(m, ctx->bk_H, ctx->bk_Hs, beta, ctx->bk_y);
   k_blincomb<T><<<nb, BLOCK, 0, ctx->stream>>>(n, m, Z, ctx->bk_y, ctx->d_mid, x);
   k_add_to_w<T><<<nb, BLOCK, 0, ctx->stream>>>(n, x, ctx->d_pair[0]);
+  neumann_w(ctx);
   batch_residual<T>(ctx, tol);
 }
 
@@ -3777,7 +3857,7 @@ int ufe_sweep_autotune(ufe_ctx *ctx, int sweeps, ufe_sweep_tuning *out) try {
   HIPCHK(ctx, hipEventCreate(&b));
   double best = HUGE_VAL;
   int nbest = 256, sbest = 0, lbest = 0, k = 0;
-  for (int loads = 0; loads < 3; loads++)
+  for (int loads = 0; loads < 4; loads++)
     for (int bs : bss)
       for (int sched = 0; sched < 2; sched++) {
         ctx->tune_bs = bs;
@@ -3827,7 +3907,7 @@ int ufe_sweep_autotune(ufe_ctx *ctx, int sweeps, ufe_sweep_tuning *out) try {
 
 int ufe_sweep_tuning_set(ufe_ctx *ctx, int threads_per_block, int schedule, int loads) {
   if (!ctx || (threads_per_block != 64 && threads_per_block != 128 && threads_per_block != 256 &&
-               threads_per_block != 512) || schedule < 0 || schedule > 1 || loads < 0 || loads > 2)
+               threads_per_block != 512) || schedule < 0 || schedule > 1 || loads < 0 || loads > 3)
     return UFE_BAD_INPUT;
   ctx->tune_bs = threads_per_block;
   ctx->tune_sched = schedule;
