@@ -387,6 +387,11 @@ int ufe_ssa_sor_solve_noalloc(ufe_ctx *ctx, double rel_tol, int32_t max_sweeps, 
  * transposed solve (FP64 GMRES, transposed-SOR preconditioner) and the chain
  * through the sliding term, the right-hand side and the effective viscosity.
  * Iterates between checkpoints are recomputed in the reverse sweep.
+ * With params.use_analytical_GL_flux the forward prescribes the grounding-line
+ * Ac velocities once (calculate_GL_flux, :848-949) and holds them in every
+ * step, as solve_SSA does; the backward keeps those rows out of the transposed
+ * solves and adds their contribution to dJ/dA (the prescribed velocity is
+ * proportional to A at the grounded end of each grounding-line Ac point).
  * All arrays of the backward are device pointers of nVAaAc doubles in the
  * reference AaAc order (Aa points 1..nV, then Ac points). */
 typedef struct ufe_adjoint_opts {

@@ -127,6 +127,57 @@ def test_multigrid_adjoint_matches_sor(problem):
     print(f"  U {rel(U1, U0):.1e} V {rel(V1, V0):.1e} grad tau {rel(g1_tau, g0_tau):.1e} grad A {rel(g1_A, g0_A):.1e}",
           flush=True)
     assert bs.max_adjoint_residual <= 1e-12 and fs.max_forward_residual <= 1e-12
-    # two solutions to a 1e-12 residual, carried through 5 Picard steps: within the conditioning
-    assert rel(U1, U0) <= 1e-6 and rel(V1, V0) <= 1e-6
-    assert rel(g1_tau, g0_tau) <= 1e-6 and rel(g1_A, g0_A) <= 1e-6
+    # two solutions to a 1e-12 residual, carried through 5 Picard steps: the
+    # difference is the solves' error, about 4e-8 at this conditioning
+    assert rel(U1, U0) <= 5e-8 and rel(V1, V0) <= 5e-8
+    assert rel(g1_tau, g0_tau) <= 5e-8 and rel(g1_A, g0_A) <= 5e-8
+
+
+def test_adjoint_with_gl_flux_matches_central_differences():
+    """With the grounding-line flux (src/ice_dynamics_module.f90:848-949) the
+    forward holds the grounding-line Ac rows at the prescribed velocities in
+    every Picard step, as solve_SSA does.  The gradient with respect to
+    log tau_c (which those velocities do not depend on) and with respect to A
+    (which they are proportional to, at the grounded end of each point) match
+    central differences: |fd - g.d| <= 1e-5 |g| along 8 + 8 seeded directions."""
+    from ufemism_amd.adjoint import AdjointOptions, SSAProblem
+    print("[adjoint] marine 20k with the grounding-line flux", flush=True)
+    ctx = abi.Context(0)
+    try:
+        wl = geometry.marine(20000, seed=5, picard=5)
+        wl.A_flow = 1e-18
+        wl.params.use_analytical_GL_flux = True
+        step = Step(ctx, wl)
+        ctx.update_general_ice_model_data()
+        prob = SSAProblem(ctx, AdjointOptions(picard_iterations=5, lin_rel_tol=1e-12))
+        dev = prob.dev
+        VA = step.ma["VAaAc"]
+        log_tau0 = torch.log(torch.tensor(wl.tau_c_override(VA), dtype=torch.float64, device=dev))
+        A0 = torch.full((prob.nA,), wl.A_flow, dtype=torch.float64, device=dev)
+        g = torch.tensor(geometry.gaussian_field(VA, 50e3, 4244), dtype=torch.float64, device=dev)
+        U_obs, V_obs = prob.forward(log_tau0 + 0.3 * g, A0)
+        Qabs = ctx.get("Qabs_GL_Ac")
+        print(f"  grounding-line points {(Qabs > 0).sum()}", flush=True)
+        assert (Qabs > 0).sum() > 20
+        g_tau, g_A, U, V = _grad(prob, log_tau0, A0, U_obs, V_obs)
+        fs, bs = prob.forward_stats, prob.backward_stats
+        assert fs.max_forward_residual <= 1e-12 and bs.max_adjoint_residual <= 1e-12
+        h = 1e-6
+        gen = torch.Generator(device="cpu").manual_seed(17)
+        for name, grad in (("tau", g_tau), ("A", g_A * A0)):
+            gn = float(torch.linalg.norm(grad))
+            worst = 0.0
+            for k in range(8):
+                d = torch.randn(prob.nA, generator=gen, dtype=torch.float64).to(dev)
+                ad = float((grad * d).sum())
+                if name == "tau":
+                    plus, minus = (log_tau0 + h * d, A0), (log_tau0 - h * d, A0)
+                else:
+                    plus, minus = (log_tau0, A0 * (1 + h * d)), (log_tau0, A0 * (1 - h * d))
+                fd = _dJ(prob, log_tau0, A0, U_obs, V_obs, plus, minus) / (2 * h)
+                worst = max(worst, abs(fd - ad) / gn)
+                print(f"  {name} dir {k}: adjoint {ad:+.10e} fd {fd:+.10e} |fd - ad| / |g| {abs(fd - ad) / gn:.1e}",
+                      flush=True)
+            assert worst <= 1e-5, (name, worst)
+    finally:
+        ctx.close()

@@ -227,7 +227,8 @@ struct ufe_ctx {
   int adj_m = 0, adj_K = 0, adj_every = 1, adj_nck = 0;
   bool adj_ready = false;
   double2 *adj_V = nullptr, *adj_Z = nullptr, *adj_w = nullptr, *adj_r = nullptr, *adj_g = nullptr,
-          *adj_lam = nullptr, *adj_ubar = nullptr, *adj_uK = nullptr;
+          *adj_lam = nullptr, *adj_ubar = nullptr, *adj_uK = nullptr, *adj_pbar = nullptr;
+  bool adj_gl = false;               // the forward held the grounding-line rows at the GL-flux velocities
   double *adj_gs = nullptr, *adj_gtau = nullptr, *adj_gA = nullptr, *adj_H1 = nullptr, *adj_Hs = nullptr,
          *adj_y = nullptr, *adj_nrm = nullptr;
   std::vector<double2 *> adj_ck;    // stored iterates u_0, u_c, u_2c, ...
@@ -352,7 +353,8 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->adj_m = ctx->adj_K = ctx->adj_nck = 0;
   ctx->adj_ready = false;
   ctx->adj_V = ctx->adj_Z = ctx->adj_w = ctx->adj_r = ctx->adj_g = ctx->adj_lam = ctx->adj_ubar = ctx->adj_uK =
-      nullptr;
+      ctx->adj_pbar = nullptr;
+  ctx->adj_gl = false;
   ctx->adj_gs = ctx->adj_gtau = ctx->adj_gA = ctx->adj_H1 = ctx->adj_Hs = ctx->adj_y = ctx->adj_nrm = nullptr;
   ctx->adj_ck.clear();
   ctx->adj_seg.clear();
@@ -2490,7 +2492,8 @@ static void amg_apply_f64(ufe_ctx *ctx, const AaAcMesh &am, float omega, const d
   k_to_f64_pair<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, ctx->amg_z32, z);
 }
 
-static void apply_AT(ufe_ctx *ctx, const AaAcMesh &mT, const ActiveRanges &ar, const double2 *x, double2 *y);
+static void apply_AT(ufe_ctx *ctx, const AaAcMesh &mT, const ActiveRanges &ar, const double2 *x, double2 *y,
+                     bool keep_fixed = false);
 
 // The transposed operator P^T A^T R^T of the adjoint solves: nu forward FP64
 // transposed SOR sweeps from zero (edge rows stay zero), the coarse correction
@@ -2504,7 +2507,7 @@ static void amg_apply_T(ufe_ctx *ctx, const AaAcMesh &mT, double omega, const do
       const int m = ctx->colour_active[c];
       if (m == 0) continue;
       k_sor_colour<<<xcd_grid(m), BLOCK, 0, ctx->stream>>>(mT, z, ctx->d_pair[1], v, ctx->colour_start[c], m,
-                                                         omega, nullptr, nullptr);
+                                                         omega, nullptr, ctx->gl_active ? ctx->d_skip : nullptr);
     }
   };
   k_fill<double2><<<nblk(n), BLOCK, 0, ctx->stream>>>(z, n, make_double2(0.0, 0.0));
@@ -2518,7 +2521,8 @@ static void amg_apply_T(ufe_ctx *ctx, const AaAcMesh &mT, double omega, const do
     if (!launched) (void)hipGetLastError();
   }
   if (!launched) amg_solve_level(ctx, 1);   // the same kernels, eager
-  k_amg_prolong64<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, z);
+  k_amg_prolong64<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, z,
+                                                     ctx->gl_active ? ctx->d_skip : nullptr);
   for (int k = 0; k < nu; k++) sweep(true);
 }
 
@@ -2591,6 +2595,23 @@ static int ssa_finish(ufe_ctx *ctx) {
   return UFE_OK;
 }
 
+// calculate_GL_flux (:848-949): the grounding-line Ac velocities into W, the
+// GL fluxes and the skip mask of the rows they prescribe
+static int gl_prescribe(ufe_ctx *ctx) {
+  GLFields g;
+  g.Hi = F(ctx, "Hi"); g.Hb = F(ctx, "Hb"); g.SL = F(ctx, "SL"); g.Am = F(ctx, "A_flow_mean");
+  g.sheet = FI(ctx, "mask_sheet"); g.phi = F(ctx, "phi_fric_AaAc"); g.iperm = ctx->d_iperm;
+  g.glA = FI(ctx, "mask_gl_Ac");
+  g.dHi_dx = F(ctx, "dHi_dx_Ac"); g.dHi_dy = F(ctx, "dHi_dy_Ac"); g.dSL_dx = F(ctx, "dSL_dx_Ac");
+  g.dSL_dy = F(ctx, "dSL_dy_Ac"); g.dHb_dx = F(ctx, "dHb_dx_Ac"); g.dHb_dy = F(ctx, "dHb_dy_Ac");
+  g.Qabs = F(ctx, "Qabs_GL_Ac"); g.Qp = F(ctx, "Qp_GL_Ac"); g.Ux = F(ctx, "Ux_SSA_Ac"); g.Uy = F(ctx, "Uy_SSA_Ac");
+  g.W = ctx->d_pair[0];
+  g.skip = ctx->d_skip;
+  k_gl_flux<<<nblk(ctx->nAc), BLOCK, 0, ctx->stream>>>(ac_mesh(ctx), g);
+  LAUNCHCHK(ctx);
+  return UFE_OK;
+}
+
 static int zero_ssa_outputs(ufe_ctx *ctx) {
   const char *aa[] = {"U_SSA", "V_SSA"};
   const char *ac[] = {"Ux_SSA_Ac", "Uy_SSA_Ac", "Up_SSA_Ac", "Uo_SSA_Ac"};
@@ -2632,19 +2653,7 @@ int ufe_solve_SSA(ufe_ctx *ctx, ufe_ssa_stats *st) try {
     ~GLGuard() { c->gl_active = false; }
   } gl_guard{ctx};
   ctx->gl_active = P.use_analytical_GL_flux != 0;
-  if (ctx->gl_active) {
-    GLFields g;
-    g.Hi = F(ctx, "Hi"); g.Hb = F(ctx, "Hb"); g.SL = F(ctx, "SL"); g.Am = F(ctx, "A_flow_mean");
-    g.sheet = FI(ctx, "mask_sheet"); g.phi = F(ctx, "phi_fric_AaAc"); g.iperm = ctx->d_iperm;
-    g.glA = FI(ctx, "mask_gl_Ac");
-    g.dHi_dx = F(ctx, "dHi_dx_Ac"); g.dHi_dy = F(ctx, "dHi_dy_Ac"); g.dSL_dx = F(ctx, "dSL_dx_Ac");
-    g.dSL_dy = F(ctx, "dSL_dy_Ac"); g.dHb_dx = F(ctx, "dHb_dx_Ac"); g.dHb_dy = F(ctx, "dHb_dy_Ac");
-    g.Qabs = F(ctx, "Qabs_GL_Ac"); g.Qp = F(ctx, "Qp_GL_Ac"); g.Ux = F(ctx, "Ux_SSA_Ac"); g.Uy = F(ctx, "Uy_SSA_Ac");
-    g.W = ctx->d_pair[0];
-    g.skip = ctx->d_skip;
-    k_gl_flux<<<nblk(ctx->nAc), BLOCK, 0, ctx->stream>>>(ac_mesh(ctx), g);
-    LAUNCHCHK(ctx);
-  }
+  if (ctx->gl_active && (r = gl_prescribe(ctx))) return r;
   int it = 0, did_reset_before = 0, did_reset_now = 0;
   while (it < P.SSA_max_outer_loops) {
     it++;
@@ -3266,6 +3275,7 @@ static int adj_alloc(ufe_ctx *ctx, int m, int K, int every) {
     if ((r = dalloc(ctx, &ctx->adj_gs, 4 * n))) return r;
     if ((r = dalloc(ctx, &ctx->adj_gtau, n))) return r;
     if ((r = dalloc(ctx, &ctx->adj_gA, n))) return r;
+    if ((r = dalloc(ctx, &ctx->adj_pbar, n))) return r;
     if ((r = dalloc(ctx, &ctx->adj_H1, 65 * 64))) return r;
     if ((r = dalloc(ctx, &ctx->adj_Hs, 65))) return r;
     if ((r = dalloc(ctx, &ctx->adj_y, 65))) return r;
@@ -3311,10 +3321,16 @@ static void fold_T(ufe_ctx *ctx, double2 *y) {
   if (ctx->n_rev) k_fold_edges<<<nblk(ctx->n_rev), BLOCK, 0, ctx->stream>>>(rev_list(ctx), y);
   if (ctx->n_neu) k_zero_rows<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), y);
 }
-// y = P^T A^T R^T x
-static void apply_AT(ufe_ctx *ctx, const AaAcMesh &mT, const ActiveRanges &ar, const double2 *x, double2 *y) {
+// y = P^T A^T R^T x.  With the grounding-line rows held fixed they are no
+// unknowns of the system: `keep_fixed` false zeroes them in y, true keeps
+// their values, which are the couplings of the prescribed velocities into the
+// free rows' equations.
+static void apply_AT(ufe_ctx *ctx, const AaAcMesh &mT, const ActiveRanges &ar, const double2 *x, double2 *y,
+                     bool keep_fixed) {
   k_matvec_T<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(mT, ctx->d_pair[1], ar, x, y);
   fold_T(ctx, y);
+  if (ctx->gl_active && !keep_fixed)
+    k_zero_skip_rows<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(ctx->nA, ctx->d_skip, y);
 }
 // z = ks transposed SOR sweeps from zero on A^T z = v (edge rows stay zero)
 static void precond_T(ufe_ctx *ctx, const AaAcMesh &mT, double omega, int ks, const double2 *v, double2 *z) {
@@ -3325,7 +3341,7 @@ static void precond_T(ufe_ctx *ctx, const AaAcMesh &mT, double omega, int ks, co
       const int n = ctx->colour_active[c];
       if (n == 0) continue;
       k_sor_colour<<<xcd_grid(n), BLOCK, 0, ctx->stream>>>(mT, z, ctx->d_pair[1], v, ctx->colour_start[c], n,
-                                                         omega, nullptr, nullptr);
+                                                         omega, nullptr, ctx->gl_active ? ctx->d_skip : nullptr);
     }
 }
 
@@ -3421,10 +3437,6 @@ int ufe_ssa_adjoint_forward(ufe_ctx *ctx, const ufe_adjoint_opts *o, ufe_adjoint
                "lin_rel_tol > 0 and precision UFE_FP64 or UFE_MIXED are required";
     return UFE_BAD_INPUT;
   }
-  if (ctx->params.use_analytical_GL_flux) {
-    ctx->err = "use_analytical_GL_flux is not supported by the adjoint forward";
-    return st->status = UFE_NOT_SUPPORTED;
-  }
   const int K = o->picard_iterations, every = o->checkpoint_every;
   int r;
   if ((r = adj_alloc(ctx, o->gmres_restart, K, every))) return r;
@@ -3441,6 +3453,15 @@ int ufe_ssa_adjoint_forward(ufe_ctx *ctx, const ufe_adjoint_opts *o, ufe_adjoint
   ctx->adj_ready = false;
   auto t0 = std::chrono::steady_clock::now();
   if ((r = ssa_gather(ctx))) return r;
+  // with the GL flux (:471) the grounding-line rows hold its velocities in
+  // every Picard step, as in solve_SSA
+  struct GLGuard {
+    ufe_ctx *c;
+    ~GLGuard() { c->gl_active = false; }
+  } gl_guard{ctx};
+  ctx->adj_gl = ctx->params.use_analytical_GL_flux != 0;
+  ctx->gl_active = ctx->adj_gl;
+  if (ctx->adj_gl && (r = gl_prescribe(ctx))) return r;
   for (int k = 1; k <= K; k++) {
     if ((k - 1) % every == 0) {
       if ((r = copy_pair(ctx, ctx->adj_ck[(k - 1) / every], ctx->d_pair[0]))) return r;
@@ -3474,15 +3495,32 @@ int ufe_ssa_adjoint_backward(ufe_ctx *ctx, const double *dJ_dU, const double *dJ
   const ufe_linsolve_opts &L = ctx->adj_lin;
   const double omega = L.omega > 0.0 ? L.omega : ctx->params.SSA_SOR_omega;
   const int ks = L.precond_sweeps > 0 ? L.precond_sweeps : 2;
+  struct GLGuard {
+    ufe_ctx *c;
+    ~GLGuard() { c->gl_active = false; }
+  } gl_guard{ctx};
+  ctx->gl_active = ctx->adj_gl;      // the rows the forward held fixed stay out of the adjoint solves
   const AaAcMesh am = aaac_mesh(ctx), mT = aaac_mesh_T(ctx);
   const ActiveRanges ar = active_ranges(ctx);
   int r;
   auto t0 = std::chrono::steady_clock::now();
+  // The GL velocities P are the same in every iterate, so their adjoint pbar
+  // collects: the adjoint of each iterate on those rows, and the coupling of
+  // P into each step's free equations.  P is proportional to the rate factor of
+  // the grounded vertex (calculate_GL_flux, :908-917), which turns pbar into a
+  // contribution to dJ/dA at the end.
+  if (ctx->adj_gl)
+    k_fill<double2><<<nb, BLOCK, 0, ctx->stream>>>(ctx->adj_pbar, n, make_double2(0.0, 0.0));
+  auto take_fixed = [&]() {
+    if (ctx->adj_gl)
+      k_take_skip_rows<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_skip, ctx->adj_g, ctx->adj_pbar);
+  };
   // g_K = P^T dJ/du_K
   k_gather_perm_pair<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, dJ_dU, ctx->adj_ubar, 0);
   k_gather_perm_pair<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, dJ_dV, ctx->adj_ubar, 1);
   fold_T(ctx, ctx->adj_ubar);
   if ((r = copy_pair(ctx, ctx->adj_g, ctx->adj_ubar))) return r;
+  take_fixed();
   k_fill<double><<<nb, BLOCK, 0, ctx->stream>>>(ctx->adj_gtau, n, 0.0);
   k_fill<double><<<nb, BLOCK, 0, ctx->stream>>>(ctx->adj_gA, n, 0.0);
   k_fill<double2><<<nb, BLOCK, 0, ctx->stream>>>(ctx->adj_lam, n, make_double2(0.0, 0.0));
@@ -3526,15 +3564,25 @@ int ufe_ssa_adjoint_backward(ufe_ctx *ctx, const double *dJ_dU, const double *dJ
     st->max_adjoint_residual = std::max(st->max_adjoint_residual, rel);
     if (r == UFE_SOR_NOT_CONVERGED) st->status = UFE_SOR_NOT_CONVERGED;
     else if (r) return r;
+    if (ctx->adj_gl) {                // pbar -= (A_free,fixed)^T lam
+      apply_AT(ctx, mT, ar, ctx->adj_lam, ctx->adj_w, true);
+      k_sub_skip_rows<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_skip, ctx->adj_w, ctx->adj_pbar);
+    }
     ap.uk = u_k;
     ap.ukm1 = u_km1;
     k_adj_point<<<nb, BLOCK, 0, ctx->stream>>>(am, ssa_fields(ctx), ar, ap);
     k_adj_strain_T<<<nb, BLOCK, 0, ctx->stream>>>(mT, ar, ap.gux, ap.guy, ap.gvx, ap.gvy, ctx->adj_ubar);
     fold_T(ctx, ctx->adj_ubar);
     if ((r = copy_pair(ctx, ctx->adj_g, ctx->adj_ubar))) return r;
+    take_fixed();
     LAUNCHCHK(ctx);
     u_k = u_km1;
   }
+  if (ctx->adj_gl)
+    k_gl_grad_A<<<nblk(ctx->nV), BLOCK, 0, ctx->stream>>>(ac_mesh(ctx), FI(ctx, "mask_gl_Ac"),
+                                                            FI(ctx, "mask_sheet"), F(ctx, "A_flow_mean"),
+                                                            F(ctx, "Ux_SSA_Ac"), F(ctx, "Uy_SSA_Ac"), ctx->d_iperm,
+                                                            ctx->d_iAci, ctx->nCm, ctx->adj_pbar, ctx->adj_gA);
   // leave the forward's final velocities in place
   if ((r = copy_pair(ctx, ctx->d_pair[0], ctx->adj_uK))) return r;
   k_scatter_perm<<<nb, BLOCK, 0, ctx->stream>>>(n, ctx->d_perm, ctx->adj_gtau, g_logtau);
