@@ -274,6 +274,7 @@ struct ufe_ctx {
   float2 *amg_v32 = nullptr, *amg_z32 = nullptr;   // FP32 copies around an FP64 Krylov method
   double amg_setup_seconds = 0.0;
   int amg_tail_l0 = 0;              // first level solved by k_amg_tail (0: none)
+  size_t amg_tail_lds = 0;          // dynamic LDS k_amg_tail may take, bytes
 };
 
 #define HIPCHK(ctx, x)                                                            \
@@ -2133,7 +2134,36 @@ static ActiveRanges active_ranges(ufe_ctx *ctx);
 constexpr int AMG_KSTEP_MIN = 1000;      // levels with at least this many points take two cycles
 constexpr float AMG_BJ_OMEGA = 0.8f;     // damping of the coarse-level block Jacobi
 constexpr int AMG_COARSE_SWEEPS = 20;    // coarsest level too large for the dense inverse
-constexpr size_t AMG_TAIL_LDS_MAX = 61 * 1024;   // dynamic LDS of k_amg_tail (vectors, staged couplings)
+constexpr size_t AMG_TAIL_LDS_MARGIN = 256;     // bytes of LDS left unused by k_amg_tail
+
+// Dynamic LDS that k_amg_tail may take on this device: the device's limit per
+// workgroup (160 KiB on gfx950) less the kernel's static LDS.
+static int amg_tail_lds_budget(ufe_ctx *ctx, size_t *out) {
+  int per_block = 0;
+  HIPCHK(ctx, hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+  hipFuncAttributes fa{};
+  HIPCHK(ctx, hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_amg_tail)));
+  const size_t stat = fa.sharedSizeBytes + AMG_TAIL_LDS_MARGIN;
+  *out = (size_t)per_block > stat ? (size_t)per_block - stat : 0;
+  if (*out + stat > 65536 &&
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_amg_tail), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)*out) != hipSuccess) {
+    (void)hipGetLastError();     // the runtime keeps the 64 KiB default
+    *out = 65536 > stat ? 65536 - stat : 0;
+  }
+  return UFE_OK;
+}
+
+// Levels with at least this many points take K-steps (UFE_AMG_KSTEP_MIN in the
+// environment at set-up overrides AMG_KSTEP_MIN).
+static int amg_kstep_min() {
+  const char *env = std::getenv("UFE_AMG_KSTEP_MIN");
+  if (env && env[0]) {
+    const long v = std::strtol(env, nullptr, 10);
+    if (v > 0 && v < (1L << 30)) return (int)v;
+  }
+  return AMG_KSTEP_MIN;
+}
 constexpr int AMG_MAX_LEVELS = 16;
 
 static int amg_setup(ufe_ctx *ctx) {
@@ -2207,6 +2237,8 @@ static int amg_setup(ufe_ctx *ctx) {
     ops.push_back(std::move(C));
   }
   const int L = (int)ops.size();
+  const int kstep_min = amg_kstep_min();
+  if ((r = amg_tail_lds_budget(ctx, &ctx->amg_tail_lds))) return r;
   ctx->amg.assign(L, AmgLevel{});
   ctx->amg[0].n = n0;
   for (int l = 1; l < L; l++) {
@@ -2214,7 +2246,7 @@ static int amg_setup(ufe_ctx *ctx) {
     const AmgHostOp &C = ops[l];
     V.n = C.n;
     V.nnz = (int64_t)C.col.size();
-    V.kstep = l < L - 1 && C.n >= AMG_KSTEP_MIN;
+    V.kstep = l < L - 1 && C.n >= kstep_min;
     std::vector<float2> cf32(std::max<size_t>(C.cf.size(), 1));
     for (size_t k = 0; k < C.cf.size(); k++) cf32[k] = make_float2((float)C.cf[k].x, (float)C.cf[k].y);
     std::vector<float> S32(C.n);
@@ -2279,7 +2311,7 @@ static int amg_setup(ufe_ctx *ctx) {
     if ((r = dupload(ctx, &Lc.S64, C.S.data(), C.S.size()))) return r;
     const size_t N = 2 * (size_t)Lc.n;
     if ((r = dalloc(ctx, &ctx->amg_W, N * 2 * N))) return r;
-    if ((r = dalloc(ctx, &ctx->amg_inv, N * N))) return r;
+    if ((r = dalloc(ctx, &ctx->amg_inv, 2 * N * N))) return r;   // the inverse and its transpose
   }
   HIPCHK(ctx, hipDeviceSynchronize());
   // The levels below the last K-step level, each visited by one V-cycle, are
@@ -2291,9 +2323,9 @@ static int amg_setup(ufe_ctx *ctx) {
     int l0 = L - 1;
     while (l0 > 1 && !ctx->amg[l0 - 1].kstep) l0--;
     size_t lds = 0;
-    for (int l = l0; l < L; l++) lds += 4 * (size_t)ctx->amg[l].n * sizeof(float2);
+    for (int l = l0; l < L; l++) lds += 3 * (size_t)ctx->amg[l].n * sizeof(float2);
     const int nlev = L - l0;
-    if (!(env && env[0] == '0') && nlev >= 2 && nlev <= AMG_TAIL_LEVELS && lds <= AMG_TAIL_LDS_MAX)
+    if (!(env && env[0] == '0') && nlev >= 2 && nlev <= AMG_TAIL_LEVELS && lds <= ctx->amg_tail_lds)
       ctx->amg_tail_l0 = l0;
   }
   ctx->amg_ready = true;
@@ -2336,7 +2368,7 @@ static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
   const int nb = nblk(V.n), last = (int)ctx->amg.size() - 1;
   if (l == last) {
     if (ctx->amg_direct) {
-      k_amg_dense_apply<<<1, 2 * AMG_DIRECT, 0, ctx->stream>>>(V.n, ctx->amg_inv, b, x, ctx->amg_T ? 1 : 0);
+      k_amg_dense_apply<<<1, AMG_DENSE_THREADS, 0, ctx->stream>>>(V.n, ctx->amg_inv, b, x, ctx->amg_T ? 1 : 0);
       return;
     }
     k_amg_bj<<<nb, BLOCK, 0, ctx->stream>>>(A, b, nullptr, V.t, AMG_BJ_OMEGA);
@@ -2379,7 +2411,7 @@ static void amg_tail_launch(ufe_ctx *ctx, int l0) {
     T.L[k].mem = V.mem;
     T.L[k].agg = V.agg;
     T.L[k].lds = off;
-    off += 4 * V.n;
+    off += 3 * V.n;
   }
   // the couplings of as many levels as fit follow the vectors (coarsest first:
   // it is visited most often inside one call)
@@ -2389,7 +2421,7 @@ static void amg_tail_launch(ufe_ctx *ctx, int l0) {
                         ((size_t)T.L[k].A.n + 1) * sizeof(int32_t);
     const size_t at = (bytes + 7) & ~(size_t)7;
     T.L[k].stage = -1;
-    if (at + need <= AMG_TAIL_LDS_MAX) {
+    if (at + need <= ctx->amg_tail_lds) {
       T.L[k].stage = (int)at;
       bytes = at + need;
     }
