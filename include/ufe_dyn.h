@@ -35,7 +35,7 @@ enum {
   UFE_COLOURING_FAILED = 5,    /* src/mesh_five_colour_module.f90:168-171, 270-273, 329-341 */
   UFE_BAD_INPUT = 6,
   UFE_HIP_ERROR = 7,
-  UFE_NOT_SUPPORTED = 8,
+  UFE_NOT_SUPPORTED = 8,      /* reserved for options an entry point does not take; none is returned now */
   UFE_HOST_ERROR = 9           /* host memory or threads could not be had (no exception leaves the library) */
 };
 

@@ -128,9 +128,11 @@ def test_multigrid_adjoint_matches_sor(problem):
           flush=True)
     assert bs.max_adjoint_residual <= 1e-12 and fs.max_forward_residual <= 1e-12
     # two solutions to a 1e-12 residual, carried through 5 Picard steps: the
-    # difference is the solves' error, about 4e-8 at this conditioning
-    assert rel(U1, U0) <= 5e-8 and rel(V1, V0) <= 5e-8
-    assert rel(g1_tau, g0_tau) <= 5e-8 and rel(g1_A, g0_A) <= 5e-8
+    # difference is the two solves' errors, 3e-12 to 9e-12 on this mesh since the
+    # FP64 Neumann update of the solution; a wrong transposed operator on any
+    # level would stall the adjoint solve or move the gradient by far more
+    assert rel(U1, U0) <= 1e-10 and rel(V1, V0) <= 1e-10
+    assert rel(g1_tau, g0_tau) <= 1e-10 and rel(g1_A, g0_A) <= 1e-10
 
 
 def test_adjoint_with_gl_flux_matches_central_differences():
