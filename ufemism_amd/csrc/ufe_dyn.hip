@@ -133,6 +133,8 @@ struct AmgLevel {
   float2 *ev32 = nullptr;
   int32_t *mptr = nullptr, *mem = nullptr;  // members: rows of the fine level (level 1) or points above
   int32_t *agg = nullptr;           // row / point above -> this level's point (-1: not aggregated)
+  int nmem = 0, nagg = 0;           // lengths of mem and agg
+  float2 *ev32_img = nullptr;       // ev32's place in the small-level kernel's image, or null
   float2 *b = nullptr, *t = nullptr, *e1 = nullptr, *e2 = nullptr, *w1 = nullptr, *w2 = nullptr,
          *rc2 = nullptr, *r = nullptr;
   double *part = nullptr, *sc = nullptr;
@@ -275,6 +277,9 @@ struct ufe_ctx {
   double amg_setup_seconds = 0.0;
   int amg_tail_l0 = 0;              // first level solved by k_amg_tail (0: none)
   size_t amg_tail_lds = 0;          // dynamic LDS k_amg_tail may take, bytes
+  uint4 *amg_tail_imgA = nullptr, *amg_tail_imgAT = nullptr, *amg_tail_imgB = nullptr;   // staged images
+  int amg_tail_wA = 0, amg_tail_wB = 0;                   // their lengths, 16-byte words
+  int amg_tail_offA[AMG_TAIL_LEVELS] = {}, amg_tail_offB[AMG_TAIL_LEVELS] = {};
 };
 
 #define HIPCHK(ctx, x)                                                            \
@@ -2141,15 +2146,22 @@ constexpr size_t AMG_TAIL_LDS_MARGIN = 256;     // bytes of LDS left unused by k
 static int amg_tail_lds_budget(ufe_ctx *ctx, size_t *out) {
   int per_block = 0;
   HIPCHK(ctx, hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
-  hipFuncAttributes fa{};
-  HIPCHK(ctx, hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_amg_tail)));
-  const size_t stat = fa.sharedSizeBytes + AMG_TAIL_LDS_MARGIN;
+  size_t stat = 0;
+  const void *kernels[2] = {reinterpret_cast<const void *>(&k_amg_tail<true>),
+                            reinterpret_cast<const void *>(&k_amg_tail<false>)};
+  for (const void *k : kernels) {
+    hipFuncAttributes fa{};
+    HIPCHK(ctx, hipFuncGetAttributes(&fa, k));
+    stat = std::max(stat, (size_t)fa.sharedSizeBytes + AMG_TAIL_LDS_MARGIN);
+  }
   *out = (size_t)per_block > stat ? (size_t)per_block - stat : 0;
-  if (*out + stat > 65536 &&
-      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_amg_tail), hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)*out) != hipSuccess) {
-    (void)hipGetLastError();     // the runtime keeps the 64 KiB default
-    *out = 65536 > stat ? 65536 - stat : 0;
+  if (*out + stat > 65536) {
+    for (const void *k : kernels)
+      if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)*out) != hipSuccess) {
+        (void)hipGetLastError();     // the runtime keeps the 64 KiB default
+        *out = 65536 > stat ? 65536 - stat : 0;
+        break;
+      }
   }
   return UFE_OK;
 }
@@ -2165,6 +2177,55 @@ static int amg_kstep_min() {
   return AMG_KSTEP_MIN;
 }
 constexpr int AMG_MAX_LEVELS = 16;
+
+// The small-level kernel's images (see AmgTail): the constant arrays of the
+// tail levels packed in its LDS layout, when they fit beside the vectors.  The
+// diagonal pairs are written into image B by k_amg_ev with every new system.
+static int amg_tail_images(ufe_ctx *ctx) {
+  ctx->amg_tail_imgA = ctx->amg_tail_imgAT = ctx->amg_tail_imgB = nullptr;
+  ctx->amg_tail_wA = ctx->amg_tail_wB = 0;
+  const int l0 = ctx->amg_tail_l0, L = (int)ctx->amg.size();
+  if (l0 <= 0) return UFE_OK;
+  size_t vec = 0, bA = 0, bB = 0;
+  for (int k = 0; k < L - l0; k++) {
+    const AmgLevel &V = ctx->amg[l0 + k];
+    vec += 3 * (size_t)V.n * sizeof(float2);
+    ctx->amg_tail_offA[k] = (int)bA;
+    ctx->amg_tail_offB[k] = (int)bB;
+    bA += amg_tail_bytes_A(V.n, (int)V.nnz);
+    bB += amg_tail_bytes_B(V.n, V.nmem, V.nagg, k > 0);
+  }
+  vec = (vec + 15) & ~(size_t)15;
+  if (vec + bA + bB > ctx->amg_tail_lds) return UFE_OK;
+  int r;
+  if ((r = dalloc(ctx, &ctx->amg_tail_imgA, bA / 16))) return r;
+  if ((r = dalloc(ctx, &ctx->amg_tail_imgAT, bA / 16))) return r;
+  if ((r = dalloc(ctx, &ctx->amg_tail_imgB, bB / 16))) return r;
+  auto put = [&](uint4 *img, size_t off, const void *src, size_t bytes) -> int {
+    if (bytes)
+      HIPCHK(ctx, hipMemcpyAsync(reinterpret_cast<char *>(img) + off, src, bytes, hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+    return UFE_OK;
+  };
+  for (int k = 0; k < L - l0; k++) {
+    AmgLevel &V = ctx->amg[l0 + k];
+    const size_t n = (size_t)V.n, nnz = (size_t)V.nnz, a = ctx->amg_tail_offA[k], b = ctx->amg_tail_offB[k];
+    if ((r = put(ctx->amg_tail_imgA, a, V.cf, nnz * 8)) || (r = put(ctx->amg_tail_imgA, a + nnz * 8, V.col, nnz * 4)) ||
+        (r = put(ctx->amg_tail_imgA, a + nnz * 12, V.ptr, (n + 1) * 4)) ||
+        (r = put(ctx->amg_tail_imgAT, a, V.cfT, nnz * 8)) || (r = put(ctx->amg_tail_imgAT, a + nnz * 8, V.colT, nnz * 4)) ||
+        (r = put(ctx->amg_tail_imgAT, a + nnz * 12, V.ptrT, (n + 1) * 4)) ||
+        (r = put(ctx->amg_tail_imgB, b + n * 8, V.S, n * 4)))
+      return r;
+    if (k > 0 && ((r = put(ctx->amg_tail_imgB, b + n * 12, V.mptr, (n + 1) * 4)) ||
+                  (r = put(ctx->amg_tail_imgB, b + n * 12 + (n + 1) * 4, V.mem, (size_t)V.nmem * 4)) ||
+                  (r = put(ctx->amg_tail_imgB, b + n * 12 + (n + 1 + V.nmem) * 4, V.agg, (size_t)V.nagg * 4))))
+      return r;
+    V.ev32_img = reinterpret_cast<float2 *>(reinterpret_cast<char *>(ctx->amg_tail_imgB) + b);
+  }
+  ctx->amg_tail_wA = (int)(bA / 16);
+  ctx->amg_tail_wB = (int)(bB / 16);
+  return UFE_OK;
+}
 
 static int amg_setup(ufe_ctx *ctx) {
   if (ctx->amg_ready) return UFE_OK;
@@ -2290,6 +2351,8 @@ static int amg_setup(ufe_ctx *ctx) {
     if ((r = dupload(ctx, &V.mptr, mptrs[l - 1].data(), mptrs[l - 1].size()))) return r;
     if ((r = dupload(ctx, &V.mem, mem.data(), mem.size()))) return r;
     if ((r = dupload(ctx, &V.agg, agg.data(), agg.size()))) return r;
+    V.nmem = (int)mem.size();
+    V.nagg = (int)agg.size();
     float2 **vecs[8] = {&V.b, &V.t, &V.e1, &V.e2, &V.w1, &V.w2, &V.rc2, &V.r};
     for (float2 **v : vecs)
       if ((r = dalloc(ctx, v, (size_t)C.n))) return r;
@@ -2328,6 +2391,7 @@ static int amg_setup(ufe_ctx *ctx) {
     if (!(env && env[0] == '0') && nlev >= 2 && nlev <= AMG_TAIL_LEVELS && lds <= ctx->amg_tail_lds)
       ctx->amg_tail_l0 = l0;
   }
+  if ((r = amg_tail_images(ctx))) return r;
   ctx->amg_ready = true;
   ctx->amg_setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return UFE_OK;
@@ -2350,7 +2414,7 @@ static void amg_refresh(ufe_ctx *ctx) {
   for (int l = 1; l < L; l++) {
     AmgLevel &V = ctx->amg[l];
     const double2 *evf = l == 1 ? ctx->d_pair[1] : ctx->amg[l - 1].ev64;
-    k_amg_ev<<<nblk(V.n), BLOCK, 0, ctx->stream>>>(V.n, V.mptr, V.mem, evf, V.fd, V.ev64, V.ev32);
+    k_amg_ev<<<nblk(V.n), BLOCK, 0, ctx->stream>>>(V.n, V.mptr, V.mem, evf, V.fd, V.ev64, V.ev32, V.ev32_img);
   }
   if (ctx->amg_direct) {
     AmgLevel &V = ctx->amg[L - 1];
@@ -2410,20 +2474,24 @@ static void amg_tail_launch(ufe_ctx *ctx, int l0) {
     T.L[k].mptr = V.mptr;
     T.L[k].mem = V.mem;
     T.L[k].agg = V.agg;
+    T.L[k].nmem = V.nmem;
     T.L[k].lds = off;
     off += 3 * V.n;
   }
-  // the couplings of as many levels as fit follow the vectors (coarsest first:
-  // it is visited most often inside one call)
-  size_t bytes = (size_t)off * sizeof(float2);
-  for (int k = T.nlev - 1; k >= 0; k--) {
-    const size_t need = (size_t)T.L[k].nnz * (sizeof(float2) + sizeof(int32_t)) +
-                        ((size_t)T.L[k].A.n + 1) * sizeof(int32_t);
-    const size_t at = (bytes + 7) & ~(size_t)7;
-    T.L[k].stage = -1;
-    if (at + need <= ctx->amg_tail_lds) {
-      T.L[k].stage = (int)at;
-      bytes = at + need;
+  // the two images follow the vectors when the set-up made room for them
+  size_t bytes = ((size_t)off * sizeof(float2) + 15) & ~(size_t)15;
+  T.imgA = T.imgB = nullptr;
+  if (ctx->amg_tail_imgA) {
+    T.imgA = ctx->amg_T ? ctx->amg_tail_imgAT : ctx->amg_tail_imgA;
+    T.imgB = ctx->amg_tail_imgB;
+    T.wordsA = ctx->amg_tail_wA;
+    T.wordsB = ctx->amg_tail_wB;
+    T.ldsA = (int)bytes;
+    T.ldsB = (int)(bytes + 16 * (size_t)ctx->amg_tail_wA);
+    bytes += 16 * ((size_t)ctx->amg_tail_wA + ctx->amg_tail_wB);
+    for (int k = 0; k < T.nlev; k++) {
+      T.L[k].offA = ctx->amg_tail_offA[k];
+      T.L[k].offB = ctx->amg_tail_offB[k];
     }
   }
   T.inv = ctx->amg_direct ? ctx->amg_inv : nullptr;
@@ -2433,7 +2501,10 @@ static void amg_tail_launch(ufe_ctx *ctx, int l0) {
   T.omega = AMG_BJ_OMEGA;
   T.b_in = ctx->amg[l0].b;
   T.x_out = ctx->amg[l0].e1;
-  k_amg_tail<<<1, AMG_TAIL_THREADS, bytes, ctx->stream>>>(T);
+  if (T.imgA)
+    k_amg_tail<true><<<1, AMG_TAIL_THREADS, bytes, ctx->stream>>>(T);
+  else
+    k_amg_tail<false><<<1, AMG_TAIL_THREADS, bytes, ctx->stream>>>(T);
 }
 
 // A_l e = b_l: one cycle into e1, or (kstep) two cycles combined by GCR(2),
