@@ -353,6 +353,11 @@ typedef struct ufe_multigrid_info {
   int32_t tail_level;               /* first of the small levels solved in one workgroup per visit
                                        (0: none; UFE_AMG_TAIL=0 in the environment at set-up turns
                                        it off) */
+  int32_t fused[16];                /* halo depth of a level whose cycle steps run as two kernels
+                                       with block halos, for up to that many smoothing sweeps
+                                       (0: none; forward operator; UFE_AMG_FUSE=0 at set-up turns
+                                       them off) */
+  int32_t fuse_list_max[16];        /* its longest block list (own points and halo) */
 } ufe_multigrid_info;
 int ufe_multigrid_info_get(ufe_ctx *ctx, ufe_multigrid_info *out);
 /* FP64 relative residual of the current velocities. */

@@ -289,3 +289,36 @@ def test_small_level_kernel_is_bit_identical():
     assert a[0] == b[0]
     for k in (1, 2, 3):
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_fused_cycle_kernels_are_bit_identical():
+    """The cycle steps of the small coarse levels as two kernels with block
+    halos (k_amg_fuse_pre, k_amg_fuse_post) give the same bits as one kernel
+    per step (UFE_AMG_FUSE=0 at set-up), for 1, 2 and 3 smoothing sweeps and
+    with FP32 and FP64 Krylov vectors."""
+    import os
+    out = {}
+    try:
+        for flag in ("1", "0"):
+            os.environ["UFE_AMG_FUSE"] = flag
+            ctx = abi.Context(0)
+            try:
+                _linear_system(ctx)
+                res = []
+                for prec, nu in ((abi.UFE_MIXED, 1), (abi.UFE_MIXED, 2), (abi.UFE_MIXED, 3), (abi.UFE_FP64, 2)):
+                    st, uv = _solve(ctx, 1e-9, precision=prec, gmres=20, precond_sweeps=nu,
+                                    precond_type=abi.UFE_PRECOND_MULTIGRID)
+                    assert st.status == 0
+                    res.append((st.matvecs, uv))
+                info = ctx.multigrid_info()
+                print(f"  UFE_AMG_FUSE={flag}: hierarchy {info['points']} fused {info['fused']} "
+                      f"lists {info['fuse_list_max']} matvecs {[m for m, _ in res]}", flush=True)
+                assert any(info["fused"]) == (flag == "1")
+                out[flag] = res
+            finally:
+                ctx.close()
+    finally:
+        os.environ.pop("UFE_AMG_FUSE", None)
+    for (ma, ua), (mb, ub) in zip(out["1"], out["0"]):
+        assert ma == mb
+        assert np.array_equal(ua, ub)

@@ -15,7 +15,11 @@ unequal couplings in the two directions:
   threads give the same coarse operator bit for bit;
 * the fine operator with the Neumann rows eliminated (a grid with an edge
   ring and four corners in sliced-ELLPACK storage) equals the full rows
-  applied to the expanded vector, with 1 and 5 host threads.
+  applied to the expanded vector, with 1 and 5 host threads;
+* the blocks of the fused small-level kernels partition the points into
+  connected sets of at most the block size; their halos list every point
+  within the halo depth at its exact graph distance (breadth-first search),
+  and their local rows equal the original rows with renumbered columns.
 """
 import os
 import shutil
@@ -252,8 +256,85 @@ static bool same(const AmgHostOp &a, const AmgHostOp &b) {
   return a.n == b.n && a.ptr == b.ptr && a.col == b.col && a.S == b.S && eq(a.cf, b.cf) && eq(a.fd, b.fd);
 }
 
+// Block halos: the blocks' own points partition the points, each block is at
+// most fb points and connected in the operator's graph, each listed point is
+// at exactly the graph distance of its class, every point within `depth` is
+// listed (breadth-first search from the block), and the local rows equal the
+// original rows.
+static void check_halos(const AmgHostOp &A, int fb, int depth) {
+  AmgHalos H;
+  amg_block_halos(A.n, A.ptr, A.col, fb, depth, H);
+  CHECK(H.nblocks >= (A.n + fb - 1) / fb, "halo blocks %d", H.nblocks);
+  std::vector<int> owner(A.n, -1);
+  int maxe = 0;
+  for (int b = 0; b < H.nblocks; b++) {
+    const int e0 = H.ext_ptr[b], m = H.ext_ptr[b + 1] - e0;
+    const int32_t *cb = &H.cnt[(size_t)b * (depth + 1)];
+    maxe = std::max(maxe, m);
+    CHECK(cb[depth] == m, "block %d: count %d of %d listed", b, cb[depth], m);
+    const int own = cb[0];
+    CHECK(own >= 1 && own <= fb, "block %d: %d own points", b, own);
+    for (int q = 0; q < own; q++) {
+      const int i = H.ext[e0 + q];
+      CHECK(owner[i] < 0, "point %d in blocks %d and %d", i, owner[i], b);
+      owner[i] = b;
+    }
+    {   // connected: a search inside the block from its first point reaches all of it
+      std::vector<int> seen(1, H.ext[e0]), in(A.n, 0);
+      for (int q = 0; q < own; q++) in[H.ext[e0 + q]] = 1;
+      in[H.ext[e0]] = 2;
+      for (size_t h = 0; h < seen.size(); h++)
+        for (int k = A.ptr[seen[h]]; k < A.ptr[seen[h] + 1]; k++)
+          if (in[A.col[k]] == 1) {
+            in[A.col[k]] = 2;
+            seen.push_back(A.col[k]);
+          }
+      CHECK((int)seen.size() == own, "block %d: %d of %d own points connected", b, (int)seen.size(), own);
+    }
+    std::vector<int> dist(A.n, -1), frontier;
+    for (int q = 0; q < own; q++) {
+      dist[H.ext[e0 + q]] = 0;
+      frontier.push_back(H.ext[e0 + q]);
+    }
+    for (int d = 1; d <= depth; d++) {
+      std::vector<int> next;
+      for (int p : frontier)
+        for (int k = A.ptr[p]; k < A.ptr[p + 1]; k++)
+          if (dist[A.col[k]] < 0) {
+            dist[A.col[k]] = d;
+            next.push_back(A.col[k]);
+          }
+      frontier = next;
+    }
+    int within = 0;
+    for (int i = 0; i < A.n; i++) within += dist[i] >= 0;
+    CHECK(within == m, "block %d: %d points within distance %d, %d listed", b, within, depth, m);
+    for (int d = 0, q = 0; d <= depth; d++)
+      for (; q < cb[d]; q++) CHECK(dist[H.ext[e0 + q]] == d, "block %d: point %d listed at distance %d", b, q, d);
+    const int rows = cb[depth - 1];
+    for (int q = 0; q < rows; q++) {
+      const int p = H.ext[e0 + q], r = H.rstart[b] + q;
+      CHECK(H.rptr[r + 1] - H.rptr[r] == A.ptr[p + 1] - A.ptr[p], "block %d: row %d length", b, q);
+      for (int k = H.rptr[r], j = A.ptr[p]; k < H.rptr[r + 1] && j < A.ptr[p + 1]; k++, j++) {
+        CHECK(H.lsrc[k] == j, "block %d: row %d coupling order", b, q);
+        CHECK(H.lcol[k] >= 0 && H.lcol[k] < m && H.ext[e0 + H.lcol[k]] == A.col[j], "block %d: row %d column", b, q);
+      }
+    }
+    CHECK(H.rstart[b + 1] - H.rstart[b] == rows, "block %d: %d local rows", b, H.rstart[b + 1] - H.rstart[b]);
+  }
+  CHECK(H.max_ext == maxe, "max_ext %d of %d", H.max_ext, maxe);
+  for (int i = 0; i < A.n; i++) CHECK(owner[i] >= 0, "point %d in no block", i);
+}
+
 int main() {
   std::mt19937 g(12345);
+  {
+    AmgHostOp A = grid(41, 23, g), C1;
+    check_halos(A, 64, 3);
+    check_halos(A, 256, 2);
+    check_level(A, g, 1, C1);
+    check_halos(C1, 16, 3);
+  }
   for (int rep = 0; rep < 3; rep++) {
     AmgHostOp A = grid(37 + 11 * rep, 29 + 5 * rep, g), C1, C2;
     check_level(A, g, 1, C1);

@@ -144,13 +144,14 @@ class LinSolveOpts(ct.Structure):
 class MultigridInfo(ct.Structure):
     _fields_ = [("levels", ct.c_int32), ("points", ct.c_int32 * 16), ("couplings", ct.c_int64 * 16),
                 ("direct", ct.c_int32), ("kstep", ct.c_int32 * 16), ("setup_seconds", ct.c_double),
-                ("tail_level", ct.c_int32)]
+                ("tail_level", ct.c_int32), ("fused", ct.c_int32 * 16), ("fuse_list_max", ct.c_int32 * 16)]
 
     def as_dict(self):
         L = self.levels
         return {"levels": L, "points": list(self.points)[:L], "couplings": list(self.couplings)[:L],
                 "direct": bool(self.direct), "kstep": [bool(k) for k in list(self.kstep)[:L]],
-                "setup_seconds": self.setup_seconds, "tail_level": self.tail_level}
+                "setup_seconds": self.setup_seconds, "tail_level": self.tail_level,
+                "fused": list(self.fused)[:L], "fuse_list_max": list(self.fuse_list_max)[:L]}
 
 
 class KernelProfile(ct.Structure):

@@ -282,4 +282,87 @@ inline void amg_transpose(int n, const std::vector<int32_t> &ptr, const std::vec
     }
 }
 
+// Blocks of at most fb points with their neighbourhoods up to `depth`
+// couplings away, for kernels in which each block recomputes its halo rather
+// than wait for the other blocks.  A block is grown breadth-first from the
+// lowest-numbered point not yet in a block, through points not yet in a block,
+// so that it is compact in the operator's graph whatever the numbering.  Block
+// b's list ext[ext_ptr[b], ext_ptr[b+1]) holds its own points (in the order
+// they were taken), then the points at distance 1, 2, ..., depth (each
+// distance in order of discovery); cnt[b (depth+1) + d] counts the points
+// within distance d.  The rows of the points within distance depth-1 are kept
+// with their columns renumbered into the block's list: local row q of block b
+// is rptr[rstart[b] + q] .. rptr[rstart[b] + q + 1] in lcol, its couplings in
+// the original order, and lsrc gives each local coupling's index in col.
+struct AmgHalos {
+  int depth = 0, fb = 0, nblocks = 0, max_ext = 0;
+  std::vector<int32_t> ext_ptr, ext, cnt, rstart, rptr, lcol;
+  std::vector<int64_t> lsrc;
+};
+inline void amg_block_halos(int n, const std::vector<int32_t> &ptr, const std::vector<int32_t> &col, int fb,
+                            int depth, AmgHalos &H) {
+  H = AmgHalos{};
+  H.depth = depth;
+  H.fb = fb;
+  H.ext_ptr.assign(1, 0);
+  H.rstart.assign(1, 0);
+  H.rptr.assign(1, 0);
+  std::vector<int32_t> local(n, -1);
+  std::vector<char> taken(n, 0);
+  int seed = 0;
+  for (int b = 0;; b++) {
+    while (seed < n && taken[seed]) seed++;
+    if (seed >= n) break;
+    const size_t e0 = H.ext.size();
+    int m = 0;
+    local[seed] = m++;
+    taken[seed] = 1;
+    H.ext.push_back(seed);
+    for (int q = 0; q < m && m < fb; q++) {          // the block, breadth-first from the seed
+      const int p = H.ext[e0 + q];
+      for (int k = ptr[p]; k < ptr[p + 1] && m < fb; k++) {
+        const int j = col[k];
+        if (!taken[j]) {
+          taken[j] = 1;
+          local[j] = m++;
+          H.ext.push_back(j);
+        }
+      }
+    }
+    H.nblocks = b + 1;
+    H.cnt.resize((size_t)H.nblocks * (depth + 1), 0);
+    int32_t *cb = &H.cnt[(size_t)b * (depth + 1)];
+    cb[0] = m;
+    int lo = 0;
+    for (int d = 1; d <= depth; d++) {
+      const int hi = m;
+      for (int q = lo; q < hi; q++) {
+        const int p = H.ext[e0 + q];
+        for (int k = ptr[p]; k < ptr[p + 1]; k++) {
+          const int j = col[k];
+          if (local[j] < 0) {
+            local[j] = m++;
+            H.ext.push_back(j);
+          }
+        }
+      }
+      cb[d] = m;
+      lo = hi;
+    }
+    const int rows = depth > 0 ? cb[depth - 1] : 0;
+    for (int q = 0; q < rows; q++) {
+      const int p = H.ext[e0 + q];
+      for (int k = ptr[p]; k < ptr[p + 1]; k++) {
+        H.lcol.push_back(local[col[k]]);
+        H.lsrc.push_back(k);
+      }
+      H.rptr.push_back((int32_t)H.lcol.size());
+    }
+    H.rstart.push_back((int32_t)H.rptr.size() - 1);
+    for (size_t q = e0; q < H.ext.size(); q++) local[H.ext[q]] = -1;
+    H.ext_ptr.push_back((int32_t)H.ext.size());
+    H.max_ext = std::max(H.max_ext, m);
+  }
+}
+
 }  // namespace ufe

@@ -135,6 +135,9 @@ struct AmgLevel {
   int32_t *agg = nullptr;           // row / point above -> this level's point (-1: not aggregated)
   int nmem = 0, nagg = 0;           // lengths of mem and agg
   float2 *ev32_img = nullptr;       // ev32's place in the small-level kernel's image, or null
+  bool fuse = false;                // cycle steps by k_amg_fuse_pre / k_amg_fuse_post (forward operator)
+  AmgFuse F{};
+  int fuse_blocks = 0;
   float2 *b = nullptr, *t = nullptr, *e1 = nullptr, *e2 = nullptr, *w1 = nullptr, *w2 = nullptr,
          *rc2 = nullptr, *r = nullptr;
   double *part = nullptr, *sc = nullptr;
@@ -2140,6 +2143,22 @@ constexpr int AMG_KSTEP_MIN = 1000;      // levels with at least this many point
 constexpr float AMG_BJ_OMEGA = 0.8f;     // damping of the coarse-level block Jacobi
 constexpr int AMG_COARSE_SWEEPS = 20;    // coarsest level too large for the dense inverse
 constexpr size_t AMG_TAIL_LDS_MARGIN = 256;     // bytes of LDS left unused by k_amg_tail
+constexpr int AMG_FUSE_MAX_POINTS = 65536;     // levels up to this size take the fused cycle kernels
+constexpr size_t AMG_FUSE_LDS_MAX = 60 * 1024;  // their LDS arrays (36 bytes per listed point)
+
+// Coarse levels of at most this many points run their cycle steps in the fused
+// kernels (UFE_AMG_FUSE=0 in the environment at set-up turns them off,
+// UFE_AMG_FUSE_MAX=n sets the size limit).
+static int amg_fuse_max() {
+  const char *off = std::getenv("UFE_AMG_FUSE");
+  if (off && off[0] == '0') return 0;
+  const char *env = std::getenv("UFE_AMG_FUSE_MAX");
+  if (env && env[0]) {
+    const long v = std::strtol(env, nullptr, 10);
+    if (v >= 0 && v < (1L << 30)) return (int)v;
+  }
+  return AMG_FUSE_MAX_POINTS;
+}
 
 // Dynamic LDS that k_amg_tail may take on this device: the device's limit per
 // workgroup (160 KiB on gfx950) less the kernel's static LDS.
@@ -2299,6 +2318,7 @@ static int amg_setup(ufe_ctx *ctx) {
   }
   const int L = (int)ops.size();
   const int kstep_min = amg_kstep_min();
+  const int fuse_max = amg_fuse_max();
   if ((r = amg_tail_lds_budget(ctx, &ctx->amg_tail_lds))) return r;
   ctx->amg.assign(L, AmgLevel{});
   ctx->amg[0].n = n0;
@@ -2330,6 +2350,31 @@ static int amg_setup(ufe_ctx *ctx) {
       if ((r = dupload(ctx, &V.cfT, fT.data(), fT.size()))) return r;
     }
     if ((r = dupload(ctx, &V.S, S32.data(), S32.size()))) return r;
+    if (l < L - 1 && C.n <= fuse_max) {   // the fused cycle kernels' block halos, as deep as fit
+      AmgHalos H;
+      for (int depth = AMG_FUSE_NU; depth >= 1; depth--) {
+        amg_block_halos(C.n, C.ptr, C.col, AMG_FUSE_BLOCK, depth, H);
+        if ((size_t)H.max_ext * 36 <= AMG_FUSE_LDS_MAX) break;
+      }
+      if ((size_t)H.max_ext * 36 <= AMG_FUSE_LDS_MAX) {
+        std::vector<float2> lcf(std::max<size_t>(H.lsrc.size(), 1));
+        for (size_t k = 0; k < H.lsrc.size(); k++) lcf[k] = cf32[H.lsrc[k]];
+        if (H.lcol.empty()) H.lcol.push_back(0);
+        int32_t *d_ext_ptr, *d_ext, *d_cnt, *d_rstart, *d_rptr, *d_lcol;
+        float2 *d_lcf;
+        if ((r = dupload(ctx, &d_ext_ptr, H.ext_ptr.data(), H.ext_ptr.size())) ||
+            (r = dupload(ctx, &d_ext, H.ext.data(), H.ext.size())) ||
+            (r = dupload(ctx, &d_cnt, H.cnt.data(), H.cnt.size())) ||
+            (r = dupload(ctx, &d_rstart, H.rstart.data(), H.rstart.size())) ||
+            (r = dupload(ctx, &d_rptr, H.rptr.data(), H.rptr.size())) ||
+            (r = dupload(ctx, &d_lcol, H.lcol.data(), H.lcol.size())) ||
+            (r = dupload(ctx, &d_lcf, lcf.data(), lcf.size())))
+          return r;
+        V.F = AmgFuse{H.depth, d_ext_ptr, d_ext, d_cnt, d_rstart, d_rptr, d_lcol, d_lcf, V.S, nullptr, H.max_ext};
+        V.fuse_blocks = H.nblocks;
+        V.fuse = true;
+      }
+    }
     // the part of the diagonal that does not come from the level above: at level
     // 1 the fine fixed parts (Neumann self-couplings) as well, since the fine
     // diagonal pairs EV do not hold them; below, the couplings inside each
@@ -2338,6 +2383,7 @@ static int amg_setup(ufe_ctx *ctx) {
     if ((r = dupload(ctx, &V.fd, reinterpret_cast<const double2 *>(own.data()), own.size()))) return r;
     if ((r = dalloc(ctx, &V.ev64, (size_t)C.n))) return r;
     if ((r = dalloc(ctx, &V.ev32, (size_t)C.n))) return r;
+    V.F.ev = V.ev32;
     // members and aggregate map: level 1 in fine rows, deeper levels in points
     std::vector<int32_t> mem = mems[l - 1];
     std::vector<int32_t> agg;
@@ -2444,6 +2490,16 @@ static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
     return;
   }
   const int nu = std::max(1, ctx->amg_nu);
+  if (V.fuse && !ctx->amg_T && nu <= V.F.depth) {   // two kernels with block halos, same results
+    AmgLevel &C = ctx->amg[l + 1];
+    const size_t lds = (size_t)V.F.cap * 36;
+    k_amg_fuse_pre<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(V.F, nu, b, V.t, V.r, AMG_BJ_OMEGA);
+    k_amg_gather<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(C.n, C.mptr, C.mem, V.r, nullptr, C.b);
+    amg_solve_level(ctx, l + 1);
+    k_amg_fuse_post<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(V.F, nu, b, V.t, C.agg, C.e1, C.e2,
+                                                                         C.kstep ? C.sc : nullptr, x, AMG_BJ_OMEGA);
+    return;
+  }
   float2 *cur = V.t, *oth = x;
   k_amg_bj<<<nb, BLOCK, 0, ctx->stream>>>(A, b, nullptr, cur, AMG_BJ_OMEGA);
   for (int s = 1; s < nu; s++) {
@@ -3152,6 +3208,8 @@ int ufe_multigrid_info_get(ufe_ctx *ctx, ufe_multigrid_info *out) {
     out->points[l] = ctx->amg[l].n;
     out->couplings[l] = ctx->amg[l].nnz;
     out->kstep[l] = ctx->amg[l].kstep ? 1 : 0;
+    out->fused[l] = ctx->amg[l].fuse ? ctx->amg[l].F.depth : 0;
+    out->fuse_list_max[l] = ctx->amg[l].fuse ? ctx->amg[l].F.cap : 0;
   }
   out->direct = ctx->amg_direct ? 1 : 0;
   out->setup_seconds = ctx->amg_setup_seconds;
