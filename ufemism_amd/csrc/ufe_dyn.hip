@@ -268,8 +268,8 @@ struct ufe_ctx {
   std::vector<AmgLevel> amg;        // [0] unused (the fine level), coarse levels from 1
   bool amg_ready = false, amg_direct = false, pc_amg = false;
   int amg_nu = 2;                   // smoothing sweeps before and after the coarse correction
-  double *amg_W = nullptr, *amg_inv = nullptr;
-  float2 *amg_y = nullptr;          // fine-level A z of the restriction
+  double *amg_inv = nullptr;
+  float2 *amg_y = nullptr;          // fine-level residual v - A z of the restriction
   hipGraphExec_t amg_graph = nullptr;   // the coarse levels of one application, captured
   int amg_graph_nu = 0;
   bool amg_T = false;               // cycle the transposed hierarchy (adjoint solves)
@@ -381,7 +381,7 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->bk_graph = nullptr;
   ctx->amg.clear();
   ctx->amg_ready = ctx->amg_direct = false;
-  ctx->amg_W = ctx->amg_inv = nullptr;
+  ctx->amg_inv = nullptr;
   ctx->amg_y = nullptr;
   if (ctx->amg_graph) (void)hipGraphExecDestroy(ctx->amg_graph);
   ctx->amg_graph = nullptr;
@@ -2143,12 +2143,12 @@ constexpr int AMG_KSTEP_MIN = 1000;      // levels with at least this many point
 constexpr float AMG_BJ_OMEGA = 0.8f;     // damping of the coarse-level block Jacobi
 constexpr int AMG_COARSE_SWEEPS = 20;    // coarsest level too large for the dense inverse
 constexpr size_t AMG_TAIL_LDS_MARGIN = 256;     // bytes of LDS left unused by k_amg_tail
-constexpr int AMG_FUSE_MAX_POINTS = 65536;     // levels up to this size take the fused cycle kernels
+constexpr int AMG_FUSE_MAX_POINTS = 1 << 24;   // levels up to this size take the fused cycle kernels
 constexpr size_t AMG_FUSE_LDS_MAX = 60 * 1024;  // their LDS arrays (36 bytes per listed point)
 
 // Coarse levels of at most this many points run their cycle steps in the fused
-// kernels (UFE_AMG_FUSE=0 in the environment at set-up turns them off,
-// UFE_AMG_FUSE_MAX=n sets the size limit).
+// kernels when their block lists fit in LDS (UFE_AMG_FUSE=0 in the environment
+// at set-up turns them off, UFE_AMG_FUSE_MAX=n sets the size limit).
 static int amg_fuse_max() {
   const char *off = std::getenv("UFE_AMG_FUSE");
   if (off && off[0] == '0') return 0;
@@ -2419,8 +2419,10 @@ static int amg_setup(ufe_ctx *ctx) {
     if ((r = dupload(ctx, &Lc.cf64, reinterpret_cast<const double2 *>(cf.data()), cf.size()))) return r;
     if ((r = dupload(ctx, &Lc.S64, C.S.data(), C.S.size()))) return r;
     const size_t N = 2 * (size_t)Lc.n;
-    if ((r = dalloc(ctx, &ctx->amg_W, N * 2 * N))) return r;
     if ((r = dalloc(ctx, &ctx->amg_inv, 2 * N * N))) return r;   // the inverse and its transpose
+    if (N * N * sizeof(double) > 65536)                           // the matrix in LDS
+      HIPCHK(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_amg_dense_inv),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)(N * N * sizeof(double))));
   }
   HIPCHK(ctx, hipDeviceSynchronize());
   // The levels below the last K-step level, each visited by one V-cycle, are
@@ -2464,7 +2466,8 @@ static void amg_refresh(ufe_ctx *ctx) {
   }
   if (ctx->amg_direct) {
     AmgLevel &V = ctx->amg[L - 1];
-    k_amg_dense_inv<<<1, AMG_INV_THREADS, 0, ctx->stream>>>(V.n, V.ptr, V.col, V.cf64, V.S64, V.ev64, ctx->amg_W, ctx->amg_inv);
+    const size_t lds = 4 * (size_t)V.n * V.n * sizeof(double);
+    k_amg_dense_inv<<<1, AMG_INV_THREADS, lds, ctx->stream>>>(V.n, V.ptr, V.col, V.cf64, V.S64, V.ev64, ctx->amg_inv);
   }
 }
 
@@ -2622,9 +2625,9 @@ static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float
   k_fill<float2><<<nblk(n), BLOCK, 0, ctx->stream>>>(z, n, make_float2(0.0f, 0.0f));
   for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, false, false, true);
   AmgLevel &C = ctx->amg[1];
-  k_matvec<float><<<nblk(n), BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32,
-                                                      active_ranges(ctx), z, ctx->amg_y);
-  k_amg_gather<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(C.n, C.mptr, C.mem, v, ctx->amg_y, C.b);
+  k_matvec_residual<<<nblk(n), BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32,
+                                                       active_ranges(ctx), z, v, ctx->amg_y);
+  k_amg_gather<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(C.n, C.mptr, C.mem, ctx->amg_y, nullptr, C.b);
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   (void)hipStreamIsCapturing(ctx->stream, &cap);
   // eager launches when the stream is being captured (recorded into the
