@@ -2144,7 +2144,7 @@ constexpr float AMG_BJ_OMEGA = 0.8f;     // damping of the coarse-level block Ja
 constexpr int AMG_COARSE_SWEEPS = 20;    // coarsest level too large for the dense inverse
 constexpr size_t AMG_TAIL_LDS_MARGIN = 256;     // bytes of LDS left unused by k_amg_tail
 constexpr int AMG_FUSE_MAX_POINTS = 1 << 24;   // levels up to this size take the fused cycle kernels
-constexpr size_t AMG_FUSE_LDS_MAX = 60 * 1024;  // their LDS arrays (36 bytes per listed point)
+constexpr size_t AMG_FUSE_LDS_MAX = 56 * 1024;  // their LDS arrays (36 bytes per listed point; 6 KiB static beside)
 
 // Coarse levels of at most this many points run their cycle steps in the fused
 // kernels when their block lists fit in LDS (UFE_AMG_FUSE=0 in the environment
@@ -2473,6 +2473,13 @@ static void amg_refresh(ufe_ctx *ctx) {
 
 static void amg_solve_level(ufe_ctx *ctx, int l);
 
+// level l's cycle runs as k_amg_fuse_pre / k_amg_fuse_post (forward operator,
+// halos deep enough for the smoothing)
+static bool amg_fused_cycle(const ufe_ctx *ctx, int l) {
+  const AmgLevel &V = ctx->amg[l];
+  return l >= 1 && V.fuse && !ctx->amg_T && std::max(1, ctx->amg_nu) <= V.F.depth;
+}
+
 // x ~ A_l^-1 b: nu block-Jacobi sweeps, the coarse correction, nu sweeps.  The
 // 2 nu sweeps alternate between t and x, starting in t, so they end in x.
 static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
@@ -2493,14 +2500,17 @@ static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
     return;
   }
   const int nu = std::max(1, ctx->amg_nu);
-  if (V.fuse && !ctx->amg_T && nu <= V.F.depth) {   // two kernels with block halos, same results
+  if (amg_fused_cycle(ctx, l)) {   // two kernels with block halos, same results
     AmgLevel &C = ctx->amg[l + 1];
     const size_t lds = (size_t)V.F.cap * 36;
     k_amg_fuse_pre<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(V.F, nu, b, V.t, V.r, AMG_BJ_OMEGA);
     k_amg_gather<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(C.n, C.mptr, C.mem, V.r, nullptr, C.b);
     amg_solve_level(ctx, l + 1);
-    k_amg_fuse_post<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(V.F, nu, b, V.t, C.agg, C.e1, C.e2,
-                                                                         C.kstep ? C.sc : nullptr, x, AMG_BJ_OMEGA);
+    // a K-step level below leaves its step-2 partials to this kernel (amg_solve_level)
+    const int cnd = std::min(AMG_DOT_BLOCKS, nblk(C.n, 256));
+    k_amg_fuse_post<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(
+        V.F, nu, b, V.t, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, C.kstep ? C.part : nullptr, cnd, x,
+        AMG_BJ_OMEGA);
     return;
   }
   float2 *cur = V.t, *oth = x;
@@ -2579,11 +2589,17 @@ static void amg_solve_level(ufe_ctx *ctx, int l) {
   const AmgOp A = amg_op(ctx, V);
   const int nd = std::min(AMG_DOT_BLOCKS, nblk(V.n, 256));
   k_amg_matvec_dot<<<nd, 256, 0, ctx->stream>>>(A, V.e1, V.w1, V.b, nullptr, V.part);
-  k_amg_kfin<<<1, 256, 0, ctx->stream>>>(nd, V.part, V.sc, 1);
-  k_amg_axpy<<<nblk(V.n), BLOCK, 0, ctx->stream>>>(V.n, V.b, V.w1, V.sc, V.rc2);
+  if (l >= 2) {   // the step-1 sums in every block of the update (no launch of their own)
+    k_amg_kfin1_axpy<<<nblk(V.n, 256), 256, 0, ctx->stream>>>(nd, V.part, V.sc, V.n, V.b, V.w1, V.rc2);
+  } else {
+    k_amg_kfin<<<1, 256, 0, ctx->stream>>>(nd, V.part, V.sc, 1);
+    k_amg_axpy<<<nblk(V.n), BLOCK, 0, ctx->stream>>>(V.n, V.b, V.w1, V.sc, V.rc2);
+  }
   amg_cycle(ctx, l, V.rc2, V.e2);
   k_amg_matvec_dot<<<nd, 256, 0, ctx->stream>>>(A, V.e2, V.w2, V.w1, V.rc2, V.part);
-  k_amg_kfin<<<1, 256, 0, ctx->stream>>>(nd, V.part, V.sc, 2);
+  // the step-2 coefficients: computed by the fused post-smoothing of the level
+  // above from these partials, or here for the other prolongations
+  if (!amg_fused_cycle(ctx, l - 1)) k_amg_kfin<<<1, 256, 0, ctx->stream>>>(nd, V.part, V.sc, 2);
 }
 
 // The coarse levels of one application (level 1's b in, e1, e2 and sc out)
