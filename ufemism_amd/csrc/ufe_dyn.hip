@@ -2091,12 +2091,24 @@ static void launch_sor_f32(ufe_ctx *ctx, int bs, int sched, int loads, const AaA
   }
 }
 
+// The Neumann update of an FP32 pair vector: one workgroup when the edge list
+// is short enough (k_neumann_all_f32), else the edge and corner kernels
+static void neumann_f32(ufe_ctx *ctx, float2 *D) {
+  if (ctx->n_neu <= NEU_THREADS * NEU_ROWS) {
+    k_neumann_all_f32<<<1, NEU_THREADS, 0, ctx->stream>>>(neu_list(ctx), cor_list(ctx), D);
+    return;
+  }
+  k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D);
+  k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), D);
+}
+
 // skip_first: leave out the first colour of the pass (a preconditioner that
 // alternates direction would otherwise repeat the previous pass's last colour
-// with nothing changed in between)
+// with nothing changed in between).  from_zero: D holds zeros, so the first
+// colour's rows need neither neighbours nor coefficients (k_sor_zero_colour_f32).
 static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, float2 *D = nullptr,
                                const float2 *R = nullptr, bool backward = false, bool skip_first = false,
-                               bool neumann = true) {
+                               bool neumann = true, bool from_zero = false) {
   Mesh32 q;
   q.cU = ctx->d_cU32;
   q.Sxy = ctx->d_Sxy32;
@@ -2108,11 +2120,14 @@ static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, flo
     const int c = backward ? 4 - k : k;
     const int n = ctx->colour_active[c];
     if (n == 0) continue;
+    if (from_zero && k == 0) {
+      k_sor_zero_colour_f32<<<nblk(n), BLOCK, 0, ctx->stream>>>(q.EV, R, q.skip, D, ctx->colour_start[c], n, omega);
+      continue;
+    }
     launch_sor_f32(ctx, ctx->tune_bs, ctx->tune_sched, ctx->tune_loads, m, q, D, R, c, omega);
   }
   if (!neumann) return;
-  if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D);
-  k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), D);
+  neumann_f32(ctx, D);
 }
 
 // FP64 sweep on an arbitrary vector pair (Krylov preconditioner)
@@ -2639,7 +2654,7 @@ static int amg_capture(ufe_ctx *ctx) {
 static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float2 *v, float2 *z) {
   const int n = ctx->nA, nu = std::max(1, ctx->amg_nu);
   k_fill<float2><<<nblk(n), BLOCK, 0, ctx->stream>>>(z, n, make_float2(0.0f, 0.0f));
-  for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, false, false, true);
+  for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, false, false, true, k == 0);
   AmgLevel &C = ctx->amg[1];
   k_matvec_residual<<<nblk(n), BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32,
                                                        active_ranges(ctx), z, v, ctx->amg_y);
@@ -2657,8 +2672,7 @@ static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float
   if (!launched) amg_solve_level(ctx, 1);
   k_amg_prolong<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, z,
                                                    ctx->gl_active ? ctx->d_skip : nullptr);
-  if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), z);
-  k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), z);
+  neumann_f32(ctx, z);
   for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, true, false, true);
 }
 
