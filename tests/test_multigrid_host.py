@@ -16,8 +16,9 @@ unequal couplings in the two directions:
 * the fine operator with the Neumann rows eliminated (a grid with an edge
   ring and four corners in sliced-ELLPACK storage) equals the full rows
   applied to the expanded vector, with 1 and 5 host threads;
-* the blocks of the fused small-level kernels partition the points into
-  connected sets of at most the block size; their halos list every point
+* the blocks of the fused coarse-level kernels are whole aggregates of the
+  next level, each aggregate in one block with its members in list order,
+  connected and of at most the block size; their halos list every point
   within the halo depth at its exact graph distance (breadth-first search),
   and their local rows equal the original rows with renumbered columns.
 """
@@ -256,16 +257,29 @@ static bool same(const AmgHostOp &a, const AmgHostOp &b) {
   return a.n == b.n && a.ptr == b.ptr && a.col == b.col && a.S == b.S && eq(a.cf, b.cf) && eq(a.fd, b.fd);
 }
 
-// Block halos: the blocks' own points partition the points, each block is at
-// most fb points and connected in the operator's graph, each listed point is
-// at exactly the graph distance of its class, every point within `depth` is
-// listed (breadth-first search from the block), and the local rows equal the
-// original rows.
+// Block halos over whole aggregates of the next level: every aggregate in
+// exactly one block, a block's own points are its aggregates' members in the
+// order of the member lists (the local ranges glo), at most fb of them unless
+// the block is one aggregate, each block connected in the operator's graph,
+// each listed point at exactly the graph distance of its class, every point
+// within `depth` listed (breadth-first search from the block), and the local
+// rows equal the original rows.
 static void check_halos(const AmgHostOp &A, int fb, int depth) {
+  int na = 0;
+  const std::vector<int32_t> agg = amg_aggregate(A, &na);
+  AmgHostOp C;
+  std::vector<int32_t> mptr, mem;
+  std::vector<AmgPair> intra;
+  amg_galerkin(A, agg, na, C, mptr, mem, intra, 1);
+  std::vector<int32_t> gptr, gagg;
+  amg_blocks_by_aggregates(na, C.ptr, C.col, mptr, fb, gptr, gagg);
   AmgHalos H;
-  amg_block_halos(A.n, A.ptr, A.col, fb, depth, H);
-  CHECK(H.nblocks >= (A.n + fb - 1) / fb, "halo blocks %d", H.nblocks);
-  std::vector<int> owner(A.n, -1);
+  amg_block_halos(A.n, A.ptr, A.col, gptr, gagg, mptr, mem, depth, H);
+  CHECK(H.nblocks == (int)gptr.size() - 1 && H.nblocks > 0, "halo blocks %d", H.nblocks);
+  std::vector<int> seen_agg(na, 0), owner(A.n, -1);
+  for (int g : gagg) seen_agg[g]++;
+  for (int I = 0; I < na; I++) CHECK(seen_agg[I] == 1, "aggregate %d in %d blocks", I, seen_agg[I]);
+  CHECK((int)H.glo.size() == (int)gagg.size() + H.nblocks, "glo entries %d", (int)H.glo.size());
   int maxe = 0;
   for (int b = 0; b < H.nblocks; b++) {
     const int e0 = H.ext_ptr[b], m = H.ext_ptr[b + 1] - e0;
@@ -273,23 +287,31 @@ static void check_halos(const AmgHostOp &A, int fb, int depth) {
     maxe = std::max(maxe, m);
     CHECK(cb[depth] == m, "block %d: count %d of %d listed", b, cb[depth], m);
     const int own = cb[0];
-    CHECK(own >= 1 && own <= fb, "block %d: %d own points", b, own);
+    const int nag = gptr[b + 1] - gptr[b];
+    CHECK(own >= 1 && (own <= fb || nag == 1), "block %d: %d own points in %d aggregates", b, own, nag);
+    for (int g = gptr[b]; g < gptr[b + 1]; g++) {
+      const int I = gagg[g], lo = H.glo[g + b], hi = H.glo[g + b + 1];
+      CHECK(hi - lo == mptr[I + 1] - mptr[I], "block %d: aggregate %d range", b, I);
+      for (int k = 0; k < hi - lo && k < mptr[I + 1] - mptr[I]; k++)
+        CHECK(H.ext[e0 + lo + k] == mem[mptr[I] + k], "block %d: aggregate %d member %d", b, I, k);
+    }
+    CHECK(H.glo[gptr[b] + b] == 0 && H.glo[gptr[b + 1] + b] == own, "block %d: member ranges", b);
     for (int q = 0; q < own; q++) {
       const int i = H.ext[e0 + q];
       CHECK(owner[i] < 0, "point %d in blocks %d and %d", i, owner[i], b);
       owner[i] = b;
     }
     {   // connected: a search inside the block from its first point reaches all of it
-      std::vector<int> seen(1, H.ext[e0]), in(A.n, 0);
+      std::vector<int> found(1, H.ext[e0]), in(A.n, 0);
       for (int q = 0; q < own; q++) in[H.ext[e0 + q]] = 1;
       in[H.ext[e0]] = 2;
-      for (size_t h = 0; h < seen.size(); h++)
-        for (int k = A.ptr[seen[h]]; k < A.ptr[seen[h] + 1]; k++)
+      for (size_t h = 0; h < found.size(); h++)
+        for (int k = A.ptr[found[h]]; k < A.ptr[found[h] + 1]; k++)
           if (in[A.col[k]] == 1) {
             in[A.col[k]] = 2;
-            seen.push_back(A.col[k]);
+            found.push_back(A.col[k]);
           }
-      CHECK((int)seen.size() == own, "block %d: %d of %d own points connected", b, (int)seen.size(), own);
+      CHECK((int)found.size() == own, "block %d: %d of %d own points connected", b, (int)found.size(), own);
     }
     std::vector<int> dist(A.n, -1), frontier;
     for (int q = 0; q < own; q++) {

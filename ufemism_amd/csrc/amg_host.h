@@ -282,55 +282,88 @@ inline void amg_transpose(int n, const std::vector<int32_t> &ptr, const std::vec
     }
 }
 
-// Blocks of at most fb points with their neighbourhoods up to `depth`
-// couplings away, for kernels in which each block recomputes its halo rather
-// than wait for the other blocks.  A block is grown breadth-first from the
-// lowest-numbered point not yet in a block, through points not yet in a block,
-// so that it is compact in the operator's graph whatever the numbering.  Block
-// b's list ext[ext_ptr[b], ext_ptr[b+1]) holds its own points (in the order
-// they were taken), then the points at distance 1, 2, ..., depth (each
-// distance in order of discovery); cnt[b (depth+1) + d] counts the points
-// within distance d.  The rows of the points within distance depth-1 are kept
-// with their columns renumbered into the block's list: local row q of block b
-// is rptr[rstart[b] + q] .. rptr[rstart[b] + q + 1] in lcol, its couplings in
-// the original order, and lsrc gives each local coupling's index in col.
+// Blocks of points with their neighbourhoods up to `depth` couplings away,
+// for kernels in which each block recomputes its halo rather than wait for the
+// other blocks.  Block b's list ext[ext_ptr[b], ext_ptr[b+1]) holds its own
+// points, then the points at distance 1, 2, ..., depth (each distance in order
+// of discovery); cnt[b (depth+1) + d] counts the points within distance d.
+// The rows of the points within distance depth-1 are kept with their columns
+// renumbered into the block's list: local row q of block b is
+// rptr[rstart[b] + q] .. rptr[rstart[b] + q + 1] in lcol, its couplings in the
+// original order, and lsrc gives each local coupling's index in col.
+//
+// The blocks are whole aggregates of the next level (amg_blocks_by_aggregates):
+// block b owns the aggregates gagg[gptr[b], gptr[b+1]), whose members, in the
+// order of the member lists, are its own points; aggregate entry g has its
+// members at local positions [glo[g], glo[g+1]) (glo has one more entry per
+// block, so glo[gptr[b+1] + b] closes block b's last aggregate: the entries of
+// block b are glo[gptr[b] + b .. gptr[b+1] + b]).
 struct AmgHalos {
-  int depth = 0, fb = 0, nblocks = 0, max_ext = 0;
+  int depth = 0, nblocks = 0, max_ext = 0;
   std::vector<int32_t> ext_ptr, ext, cnt, rstart, rptr, lcol;
   std::vector<int64_t> lsrc;
+  std::vector<int32_t> gptr, gagg, glo;
 };
-inline void amg_block_halos(int n, const std::vector<int32_t> &ptr, const std::vector<int32_t> &col, int fb,
-                            int depth, AmgHalos &H) {
+
+// Aggregates of the next level (nc of them, members mptr/mem; graph cptr/ccol)
+// gathered into blocks of at most fb members: breadth-first from the
+// lowest-numbered aggregate not yet taken, through aggregates not yet taken,
+// adding each one whose members still fit (a first aggregate always fits).
+inline void amg_blocks_by_aggregates(int nc, const std::vector<int32_t> &cptr, const std::vector<int32_t> &ccol,
+                                     const std::vector<int32_t> &mptr, int fb, std::vector<int32_t> &gptr,
+                                     std::vector<int32_t> &gagg) {
+  gptr.assign(1, 0);
+  gagg.clear();
+  std::vector<char> taken(nc, 0);
+  std::vector<int32_t> queue;
+  int seed = 0;
+  while (true) {
+    while (seed < nc && taken[seed]) seed++;
+    if (seed >= nc) break;
+    int members = mptr[seed + 1] - mptr[seed];
+    taken[seed] = 1;
+    gagg.push_back(seed);
+    queue.assign(1, seed);
+    for (size_t h = 0; h < queue.size(); h++)
+      for (int k = cptr[queue[h]]; k < cptr[queue[h] + 1]; k++) {
+        const int J = ccol[k];
+        const int m = mptr[J + 1] - mptr[J];
+        if (taken[J] || members + m > fb) continue;
+        taken[J] = 1;
+        members += m;
+        gagg.push_back(J);
+        queue.push_back(J);
+      }
+    gptr.push_back((int32_t)gagg.size());
+  }
+}
+
+inline void amg_block_halos(int n, const std::vector<int32_t> &ptr, const std::vector<int32_t> &col,
+                            const std::vector<int32_t> &gptr, const std::vector<int32_t> &gagg,
+                            const std::vector<int32_t> &mptr, const std::vector<int32_t> &mem, int depth,
+                            AmgHalos &H) {
   H = AmgHalos{};
   H.depth = depth;
-  H.fb = fb;
+  H.gptr = gptr;
+  H.gagg = gagg;
+  H.nblocks = (int)gptr.size() - 1;
   H.ext_ptr.assign(1, 0);
   H.rstart.assign(1, 0);
   H.rptr.assign(1, 0);
+  H.cnt.assign((size_t)H.nblocks * (depth + 1), 0);
   std::vector<int32_t> local(n, -1);
-  std::vector<char> taken(n, 0);
-  int seed = 0;
-  for (int b = 0;; b++) {
-    while (seed < n && taken[seed]) seed++;
-    if (seed >= n) break;
+  for (int b = 0; b < H.nblocks; b++) {
     const size_t e0 = H.ext.size();
     int m = 0;
-    local[seed] = m++;
-    taken[seed] = 1;
-    H.ext.push_back(seed);
-    for (int q = 0; q < m && m < fb; q++) {          // the block, breadth-first from the seed
-      const int p = H.ext[e0 + q];
-      for (int k = ptr[p]; k < ptr[p + 1] && m < fb; k++) {
-        const int j = col[k];
-        if (!taken[j]) {
-          taken[j] = 1;
-          local[j] = m++;
-          H.ext.push_back(j);
-        }
+    for (int g = gptr[b]; g < gptr[b + 1]; g++) {
+      H.glo.push_back(m);
+      const int I = gagg[g];
+      for (int k = mptr[I]; k < mptr[I + 1]; k++) {
+        local[mem[k]] = m++;
+        H.ext.push_back(mem[k]);
       }
     }
-    H.nblocks = b + 1;
-    H.cnt.resize((size_t)H.nblocks * (depth + 1), 0);
+    H.glo.push_back(m);
     int32_t *cb = &H.cnt[(size_t)b * (depth + 1)];
     cb[0] = m;
     int lo = 0;

@@ -2091,14 +2091,11 @@ static void launch_sor_f32(ufe_ctx *ctx, int bs, int sched, int loads, const AaA
   }
 }
 
-// The Neumann update of an FP32 pair vector: one workgroup when the edge list
-// is short enough (k_neumann_all_f32), else the edge and corner kernels
+// The Neumann update of an FP32 pair vector: the edge rows, then the corners.
+// (One workgroup doing both, with a barrier between, took 11.2 us at 1M against
+// 4.7 + 4.7 us for the two kernels.)
 static void neumann_f32(ufe_ctx *ctx, float2 *D) {
-  if (ctx->n_neu <= NEU_THREADS * NEU_ROWS) {
-    k_neumann_all_f32<<<1, NEU_THREADS, 0, ctx->stream>>>(neu_list(ctx), cor_list(ctx), D);
-    return;
-  }
-  k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D);
+  if (ctx->n_neu) k_neumann_edges_f32<<<nblk(ctx->n_neu), BLOCK, 0, ctx->stream>>>(neu_list(ctx), D);
   k_neumann_corners_f32<<<nblk(ctx->n_cor, 64), 64, 0, ctx->stream>>>(cor_list(ctx), D);
 }
 
@@ -2365,17 +2362,19 @@ static int amg_setup(ufe_ctx *ctx) {
       if ((r = dupload(ctx, &V.cfT, fT.data(), fT.size()))) return r;
     }
     if ((r = dupload(ctx, &V.S, S32.data(), S32.size()))) return r;
-    if (l < L - 1 && C.n <= fuse_max) {   // the fused cycle kernels' block halos, as deep as fit
+    if (l < L - 1 && C.n <= fuse_max) {   // the fused cycle kernels' blocks and halos, as deep as fit
+      std::vector<int32_t> gptr, gagg;
+      amg_blocks_by_aggregates(ops[l + 1].n, ops[l + 1].ptr, ops[l + 1].col, mptrs[l], AMG_FUSE_BLOCK, gptr, gagg);
       AmgHalos H;
       for (int depth = AMG_FUSE_NU; depth >= 1; depth--) {
-        amg_block_halos(C.n, C.ptr, C.col, AMG_FUSE_BLOCK, depth, H);
+        amg_block_halos(C.n, C.ptr, C.col, gptr, gagg, mptrs[l], mems[l], depth, H);
         if ((size_t)H.max_ext * 36 <= AMG_FUSE_LDS_MAX) break;
       }
       if ((size_t)H.max_ext * 36 <= AMG_FUSE_LDS_MAX) {
         std::vector<float2> lcf(std::max<size_t>(H.lsrc.size(), 1));
         for (size_t k = 0; k < H.lsrc.size(); k++) lcf[k] = cf32[H.lsrc[k]];
         if (H.lcol.empty()) H.lcol.push_back(0);
-        int32_t *d_ext_ptr, *d_ext, *d_cnt, *d_rstart, *d_rptr, *d_lcol;
+        int32_t *d_ext_ptr, *d_ext, *d_cnt, *d_rstart, *d_rptr, *d_lcol, *d_gptr, *d_gagg, *d_glo;
         float2 *d_lcf;
         if ((r = dupload(ctx, &d_ext_ptr, H.ext_ptr.data(), H.ext_ptr.size())) ||
             (r = dupload(ctx, &d_ext, H.ext.data(), H.ext.size())) ||
@@ -2383,9 +2382,13 @@ static int amg_setup(ufe_ctx *ctx) {
             (r = dupload(ctx, &d_rstart, H.rstart.data(), H.rstart.size())) ||
             (r = dupload(ctx, &d_rptr, H.rptr.data(), H.rptr.size())) ||
             (r = dupload(ctx, &d_lcol, H.lcol.data(), H.lcol.size())) ||
-            (r = dupload(ctx, &d_lcf, lcf.data(), lcf.size())))
+            (r = dupload(ctx, &d_lcf, lcf.data(), lcf.size())) ||
+            (r = dupload(ctx, &d_gptr, H.gptr.data(), H.gptr.size())) ||
+            (r = dupload(ctx, &d_gagg, H.gagg.data(), H.gagg.size())) ||
+            (r = dupload(ctx, &d_glo, H.glo.data(), H.glo.size())))
           return r;
-        V.F = AmgFuse{H.depth, d_ext_ptr, d_ext, d_cnt, d_rstart, d_rptr, d_lcol, d_lcf, V.S, nullptr, H.max_ext};
+        V.F = AmgFuse{H.depth, d_ext_ptr, d_ext, d_cnt, d_rstart, d_rptr, d_lcol, d_lcf, V.S, nullptr,
+                      d_gptr, d_gagg, d_glo, H.max_ext};
         V.fuse_blocks = H.nblocks;
         V.fuse = true;
       }
@@ -2518,8 +2521,9 @@ static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
   if (amg_fused_cycle(ctx, l)) {   // two kernels with block halos, same results
     AmgLevel &C = ctx->amg[l + 1];
     const size_t lds = (size_t)V.F.cap * 36;
-    k_amg_fuse_pre<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(V.F, nu, b, V.t, V.r, AMG_BJ_OMEGA);
-    k_amg_gather<<<nblk(C.n), BLOCK, 0, ctx->stream>>>(C.n, C.mptr, C.mem, V.r, nullptr, C.b);
+    // the blocks are whole aggregates of level l + 1: the pre-smoothing kernel
+    // also sums their members' residuals into C.b (as k_amg_gather)
+    k_amg_fuse_pre<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(V.F, nu, b, V.t, C.b, AMG_BJ_OMEGA);
     amg_solve_level(ctx, l + 1);
     // a K-step level below leaves its step-2 partials to this kernel (amg_solve_level)
     const int cnd = std::min(AMG_DOT_BLOCKS, nblk(C.n, 256));
