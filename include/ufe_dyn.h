@@ -355,8 +355,9 @@ typedef struct ufe_multigrid_info {
                                        it off) */
   int32_t fused[16];                /* halo depth of a level whose cycle steps run as two kernels
                                        with block halos, for up to that many smoothing sweeps
-                                       (0: none; forward operator; UFE_AMG_FUSE=0 at set-up turns
-                                       them off) */
+                                       (0: none; the transposed hierarchy too when the level's
+                                       pattern is symmetric; UFE_AMG_FUSE=0 at set-up turns them
+                                       off) */
   int32_t fuse_list_max[16];        /* its longest block list (own points and halo) */
 } ufe_multigrid_info;
 int ufe_multigrid_info_get(ufe_ctx *ctx, ufe_multigrid_info *out);

@@ -292,18 +292,21 @@ def test_small_level_kernel_is_bit_identical():
 
 
 def test_fused_cycle_kernels_are_bit_identical():
-    """The cycle steps of the small coarse levels as two kernels with block
-    halos (k_amg_fuse_pre, k_amg_fuse_post) give the same bits as one kernel
-    per step (UFE_AMG_FUSE=0 at set-up), for 1, 2 and 3 smoothing sweeps and
-    with FP32 and FP64 Krylov vectors."""
+    """The cycle steps of the coarse levels as two kernels with block halos
+    (k_amg_fuse_pre, k_amg_fuse_post) give the same bits as one kernel per
+    step (UFE_AMG_FUSE=0 at set-up): for 1, 2 and 3 smoothing sweeps, with
+    FP32 and FP64 Krylov vectors, and in both directions of the multigrid
+    adjoint (the backward cycles the transposed hierarchy)."""
     import os
+    import torch
+    from ufemism_amd.adjoint import AdjointOptions, SSAProblem, ssa_velocities
     out = {}
     try:
         for flag in ("1", "0"):
             os.environ["UFE_AMG_FUSE"] = flag
             ctx = abi.Context(0)
             try:
-                _linear_system(ctx)
+                wl, step = _linear_system(ctx)
                 res = []
                 for prec, nu in ((abi.UFE_MIXED, 1), (abi.UFE_MIXED, 2), (abi.UFE_MIXED, 3), (abi.UFE_FP64, 2)):
                     st, uv = _solve(ctx, 1e-9, precision=prec, gmres=20, precond_sweeps=nu,
@@ -314,6 +317,17 @@ def test_fused_cycle_kernels_are_bit_identical():
                 print(f"  UFE_AMG_FUSE={flag}: hierarchy {info['points']} fused {info['fused']} "
                       f"lists {info['fuse_list_max']} matvecs {[m for m, _ in res]}", flush=True)
                 assert any(info["fused"]) == (flag == "1")
+                step.reset()
+                ctx.update_general_ice_model_data()
+                prob = SSAProblem(ctx, AdjointOptions(picard_iterations=3, lin_rel_tol=1e-10, gmres_restart=20,
+                                                      precond_sweeps=2, precond_type=abi.UFE_PRECOND_MULTIGRID))
+                lt = torch.log(torch.tensor(wl.tau_c_override(step.ma["VAaAc"]), dtype=torch.float64,
+                                            device=prob.dev)).requires_grad_(True)
+                A0 = torch.full((prob.nA,), wl.A_flow, dtype=torch.float64, device=prob.dev)
+                U, V = ssa_velocities(lt, A0, prob)
+                (0.5 * (U ** 2 + V ** 2).sum()).backward()
+                res.append((prob.backward_stats.adjoint_matvecs, np.concatenate([U.detach().cpu().numpy(),
+                                                                                 lt.grad.cpu().numpy()])))
                 out[flag] = res
             finally:
                 ctx.close()

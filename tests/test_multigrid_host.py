@@ -20,7 +20,8 @@ unequal couplings in the two directions:
   next level, each aggregate in one block with its members in list order,
   connected and of at most the block size; their halos list every point
   within the halo depth at its exact graph distance (breadth-first search),
-  and their local rows equal the original rows with renumbered columns.
+  and their local rows equal the original rows with renumbered columns, for
+  the operator and, when its pattern is symmetric, for its transpose.
 """
 import os
 import shutil
@@ -273,8 +274,19 @@ static void check_halos(const AmgHostOp &A, int fb, int depth) {
   amg_galerkin(A, agg, na, C, mptr, mem, intra, 1);
   std::vector<int32_t> gptr, gagg;
   amg_blocks_by_aggregates(na, C.ptr, C.col, mptr, fb, gptr, gagg);
+  std::vector<int32_t> pT, cT;
+  std::vector<AmgPair> fT;
+  amg_transpose(A.n, A.ptr, A.col, A.cf, pT, cT, fT);
   AmgHalos H;
-  amg_block_halos(A.n, A.ptr, A.col, gptr, gagg, mptr, mem, depth, H);
+  amg_block_halos(A.n, A.ptr, A.col, gptr, gagg, mptr, mem, depth, H, &pT, &cT);
+  bool symmetric = true;   // every coupling i -> j has its j -> i
+  for (int i = 0; i < A.n && symmetric; i++)
+    for (int k = A.ptr[i]; k < A.ptr[i + 1] && symmetric; k++) {
+      bool found = false;
+      for (int q = A.ptr[A.col[k]]; q < A.ptr[A.col[k] + 1]; q++) found = found || A.col[q] == i;
+      symmetric = found;
+    }
+  CHECK(!symmetric || H.transposed_ok, "symmetric pattern, transposed rows refused");
   CHECK(H.nblocks == (int)gptr.size() - 1 && H.nblocks > 0, "halo blocks %d", H.nblocks);
   std::vector<int> seen_agg(na, 0), owner(A.n, -1);
   for (int g : gagg) seen_agg[g]++;
@@ -343,9 +355,44 @@ static void check_halos(const AmgHostOp &A, int fb, int depth) {
       }
     }
     CHECK(H.rstart[b + 1] - H.rstart[b] == rows, "block %d: %d local rows", b, H.rstart[b + 1] - H.rstart[b]);
+    for (int q = 0; H.transposed_ok && q < rows; q++) {
+      const int p = H.ext[e0 + q], r = H.rstart[b] + q;
+      CHECK(H.rptrT[r + 1] - H.rptrT[r] == pT[p + 1] - pT[p], "block %d: transposed row %d length", b, q);
+      for (int k = H.rptrT[r], j = pT[p]; k < H.rptrT[r + 1] && j < pT[p + 1]; k++, j++) {
+        CHECK(H.lsrcT[k] == j, "block %d: transposed row %d order", b, q);
+        CHECK(H.lcolT[k] >= 0 && H.lcolT[k] < m && H.ext[e0 + H.lcolT[k]] == cT[j], "block %d: transposed column", b);
+      }
+    }
   }
   CHECK(H.max_ext == maxe, "max_ext %d of %d", H.max_ext, maxe);
   for (int i = 0; i < A.n; i++) CHECK(owner[i] >= 0, "point %d in no block", i);
+}
+
+// the operator with every coupling's reverse added (a structurally symmetric pattern)
+static AmgHostOp symmetrized(const AmgHostOp &A, std::mt19937 &g) {
+  std::uniform_real_distribution<double> u(0.1, 1.0);
+  std::vector<std::vector<std::pair<int, AmgPair>>> rows(A.n);
+  for (int i = 0; i < A.n; i++)
+    for (int k = A.ptr[i]; k < A.ptr[i + 1]; k++) rows[i].push_back({A.col[k], A.cf[k]});
+  for (int i = 0; i < A.n; i++)
+    for (int k = A.ptr[i]; k < A.ptr[i + 1]; k++) {
+      const int j = A.col[k];
+      bool found = false;
+      for (auto &e : rows[j]) found = found || e.first == i;
+      if (!found) rows[j].push_back({i, AmgPair{u(g), u(g)}});
+    }
+  AmgHostOp S = A;
+  S.ptr.assign(1, 0);
+  S.col.clear();
+  S.cf.clear();
+  for (int i = 0; i < A.n; i++) {
+    for (auto &e : rows[i]) {
+      S.col.push_back(e.first);
+      S.cf.push_back(e.second);
+    }
+    S.ptr.push_back((int32_t)S.col.size());
+  }
+  return S;
 }
 
 int main() {
@@ -354,8 +401,10 @@ int main() {
     AmgHostOp A = grid(41, 23, g), C1;
     check_halos(A, 64, 3);
     check_halos(A, 256, 2);
+    check_halos(symmetrized(A, g), 64, 3);
     check_level(A, g, 1, C1);
     check_halos(C1, 16, 3);
+    check_halos(symmetrized(C1, g), 16, 2);
   }
   for (int rep = 0; rep < 3; rep++) {
     AmgHostOp A = grid(37 + 11 * rep, 29 + 5 * rep, g), C1, C2;

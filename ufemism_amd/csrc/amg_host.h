@@ -298,11 +298,19 @@ inline void amg_transpose(int n, const std::vector<int32_t> &ptr, const std::vec
 // members at local positions [glo[g], glo[g+1]) (glo has one more entry per
 // block, so glo[gptr[b+1] + b] closes block b's last aggregate: the entries of
 // block b are glo[gptr[b] + b .. gptr[b+1] + b]).
+//
+// Given the transposed couplings too (ptrT/colT), the same rows of the
+// transposed operator are kept alike in rptrT/lcolT/lsrcT (rstart serves
+// both); transposed_ok is false when a transposed row reaches outside its
+// block's list (an operator that is not structurally symmetric).
 struct AmgHalos {
   int depth = 0, nblocks = 0, max_ext = 0;
   std::vector<int32_t> ext_ptr, ext, cnt, rstart, rptr, lcol;
   std::vector<int64_t> lsrc;
   std::vector<int32_t> gptr, gagg, glo;
+  bool transposed_ok = false;
+  std::vector<int32_t> rptrT, lcolT;
+  std::vector<int64_t> lsrcT;
 };
 
 // Aggregates of the next level (nc of them, members mptr/mem; graph cptr/ccol)
@@ -341,8 +349,11 @@ inline void amg_blocks_by_aggregates(int nc, const std::vector<int32_t> &cptr, c
 inline void amg_block_halos(int n, const std::vector<int32_t> &ptr, const std::vector<int32_t> &col,
                             const std::vector<int32_t> &gptr, const std::vector<int32_t> &gagg,
                             const std::vector<int32_t> &mptr, const std::vector<int32_t> &mem, int depth,
-                            AmgHalos &H) {
+                            AmgHalos &H, const std::vector<int32_t> *ptrT = nullptr,
+                            const std::vector<int32_t> *colT = nullptr) {
   H = AmgHalos{};
+  H.transposed_ok = ptrT && colT;
+  if (H.transposed_ok) H.rptrT.assign(1, 0);
   H.depth = depth;
   H.gptr = gptr;
   H.gagg = gagg;
@@ -390,6 +401,15 @@ inline void amg_block_halos(int n, const std::vector<int32_t> &ptr, const std::v
         H.lsrc.push_back(k);
       }
       H.rptr.push_back((int32_t)H.lcol.size());
+      if (H.transposed_ok) {
+        for (int k = (*ptrT)[p]; k < (*ptrT)[p + 1]; k++) {
+          const int j = local[(*colT)[k]];
+          if (j < 0) H.transposed_ok = false;
+          H.lcolT.push_back(j);
+          H.lsrcT.push_back(k);
+        }
+        H.rptrT.push_back((int32_t)H.lcolT.size());
+      }
     }
     H.rstart.push_back((int32_t)H.rptr.size() - 1);
     for (size_t q = e0; q < H.ext.size(); q++) local[H.ext[q]] = -1;

@@ -135,9 +135,12 @@ struct AmgLevel {
   int32_t *agg = nullptr;           // row / point above -> this level's point (-1: not aggregated)
   int nmem = 0, nagg = 0;           // lengths of mem and agg
   float2 *ev32_img = nullptr;       // ev32's place in the small-level kernel's image, or null
-  bool fuse = false;                // cycle steps by k_amg_fuse_pre / k_amg_fuse_post (forward operator)
+  bool fuse = false;                // cycle steps by k_amg_fuse_pre / k_amg_fuse_post
   AmgFuse F{};
   int fuse_blocks = 0;
+  bool fuseT = false;               // and for the transposed operator: its local rows
+  int32_t *fuse_rptrT = nullptr, *fuse_lcolT = nullptr;
+  float2 *fuse_lcfT = nullptr;
   float2 *b = nullptr, *t = nullptr, *e1 = nullptr, *e2 = nullptr, *w1 = nullptr, *w2 = nullptr,
          *rc2 = nullptr, *r = nullptr;
   double *part = nullptr, *sc = nullptr;
@@ -2349,9 +2352,9 @@ static int amg_setup(ufe_ctx *ctx) {
     if ((r = dupload(ctx, &V.ptr, C.ptr.data(), C.ptr.size()))) return r;
     if ((r = dupload(ctx, &V.col, colv.data(), colv.size()))) return r;
     if ((r = dupload(ctx, &V.cf, cf32.data(), cf32.size()))) return r;
-    {   // the transposed couplings (adjoint solves)
-      std::vector<int32_t> pT, cT;
-      std::vector<float2> fT;
+    std::vector<int32_t> pT, cT;   // the transposed couplings (adjoint solves)
+    std::vector<float2> fT;
+    {
       amg_transpose(C.n, C.ptr, C.col, cf32, pT, cT, fT);
       if (cT.empty()) {
         cT.push_back(0);
@@ -2367,7 +2370,7 @@ static int amg_setup(ufe_ctx *ctx) {
       amg_blocks_by_aggregates(ops[l + 1].n, ops[l + 1].ptr, ops[l + 1].col, mptrs[l], AMG_FUSE_BLOCK, gptr, gagg);
       AmgHalos H;
       for (int depth = AMG_FUSE_NU; depth >= 1; depth--) {
-        amg_block_halos(C.n, C.ptr, C.col, gptr, gagg, mptrs[l], mems[l], depth, H);
+        amg_block_halos(C.n, C.ptr, C.col, gptr, gagg, mptrs[l], mems[l], depth, H, &pT, &cT);
         if ((size_t)H.max_ext * 36 <= AMG_FUSE_LDS_MAX) break;
       }
       if ((size_t)H.max_ext * 36 <= AMG_FUSE_LDS_MAX) {
@@ -2391,6 +2394,16 @@ static int amg_setup(ufe_ctx *ctx) {
                       d_gptr, d_gagg, d_glo, H.max_ext};
         V.fuse_blocks = H.nblocks;
         V.fuse = true;
+        if (H.transposed_ok) {   // the same blocks for the transposed hierarchy
+          std::vector<float2> lcfT(std::max<size_t>(H.lsrcT.size(), 1));
+          for (size_t k = 0; k < H.lsrcT.size(); k++) lcfT[k] = fT[H.lsrcT[k]];
+          if (H.lcolT.empty()) H.lcolT.push_back(0);
+          if ((r = dupload(ctx, &V.fuse_rptrT, H.rptrT.data(), H.rptrT.size())) ||
+              (r = dupload(ctx, &V.fuse_lcolT, H.lcolT.data(), H.lcolT.size())) ||
+              (r = dupload(ctx, &V.fuse_lcfT, lcfT.data(), lcfT.size())))
+            return r;
+          V.fuseT = true;
+        }
       }
     }
     // the part of the diagonal that does not come from the level above: at level
@@ -2491,11 +2504,11 @@ static void amg_refresh(ufe_ctx *ctx) {
 
 static void amg_solve_level(ufe_ctx *ctx, int l);
 
-// level l's cycle runs as k_amg_fuse_pre / k_amg_fuse_post (forward operator,
-// halos deep enough for the smoothing)
+// level l's cycle runs as k_amg_fuse_pre / k_amg_fuse_post (halos deep enough
+// for the smoothing, local rows of the operator in use)
 static bool amg_fused_cycle(const ufe_ctx *ctx, int l) {
   const AmgLevel &V = ctx->amg[l];
-  return l >= 1 && V.fuse && !ctx->amg_T && std::max(1, ctx->amg_nu) <= V.F.depth;
+  return l >= 1 && V.fuse && (!ctx->amg_T || V.fuseT) && std::max(1, ctx->amg_nu) <= V.F.depth;
 }
 
 // x ~ A_l^-1 b: nu block-Jacobi sweeps, the coarse correction, nu sweeps.  The
@@ -2521,14 +2534,20 @@ static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
   if (amg_fused_cycle(ctx, l)) {   // two kernels with block halos, same results
     AmgLevel &C = ctx->amg[l + 1];
     const size_t lds = (size_t)V.F.cap * 36;
+    AmgFuse F = V.F;
+    if (ctx->amg_T) {
+      F.rptr = V.fuse_rptrT;
+      F.lcol = V.fuse_lcolT;
+      F.lcf = V.fuse_lcfT;
+    }
     // the blocks are whole aggregates of level l + 1: the pre-smoothing kernel
     // also sums their members' residuals into C.b (as k_amg_gather)
-    k_amg_fuse_pre<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(V.F, nu, b, V.t, C.b, AMG_BJ_OMEGA);
+    k_amg_fuse_pre<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(F, nu, b, V.t, C.b, AMG_BJ_OMEGA);
     amg_solve_level(ctx, l + 1);
     // a K-step level below leaves its step-2 partials to this kernel (amg_solve_level)
     const int cnd = std::min(AMG_DOT_BLOCKS, nblk(C.n, 256));
     k_amg_fuse_post<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(
-        V.F, nu, b, V.t, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, C.kstep ? C.part : nullptr, cnd, x,
+        F, nu, b, V.t, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, C.kstep ? C.part : nullptr, cnd, x,
         AMG_BJ_OMEGA);
     return;
   }
