@@ -273,6 +273,13 @@ struct ufe_ctx {
   int amg_nu = 2;                   // smoothing sweeps before and after the coarse correction
   double *amg_inv = nullptr;
   float2 *amg_y = nullptr;          // fine-level residual v - A z of the restriction
+  struct SorGraph {                    // a batch of reference FP64 sweeps, captured
+    hipGraphExec_t exec = nullptr;
+    int nb = 0, max_sweeps = 0;
+    double omega = 0.0, tol = 0.0;
+    const uint8_t *skip = nullptr;
+  };
+  SorGraph sor_graph[8];
   hipGraphExec_t amg_graph = nullptr;   // the coarse levels of one application, captured
   int amg_graph_nu = 0;
   bool amg_T = false;               // cycle the transposed hierarchy (adjoint solves)
@@ -386,6 +393,10 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->amg_ready = ctx->amg_direct = false;
   ctx->amg_inv = nullptr;
   ctx->amg_y = nullptr;
+  for (auto &G : ctx->sor_graph) {
+    if (G.exec) (void)hipGraphExecDestroy(G.exec);
+    G = ufe_ctx::SorGraph{};
+  }
   if (ctx->amg_graph) (void)hipGraphExecDestroy(ctx->amg_graph);
   ctx->amg_graph = nullptr;
   ctx->amg_graph_nu = 0;
@@ -2745,6 +2756,52 @@ static void amg_apply_T(ufe_ctx *ctx, const AaAcMesh &mT, double omega, const do
 static int64_t amg_passes(const ufe_ctx *ctx) { return 10 * (int64_t)std::max(1, ctx->amg_nu); }
 
 // solve_SSA_linearised sweep loop (:601-691): returns the flags at exit.
+// nb reference FP64 sweeps.  When the library is not sampling kernel times, a
+// batch of 2^k sweeps (k < 8) is captured once as a HIP graph (the sweeps'
+// kernels and their arguments: omega, tolerance, sweep limit, skip mask) and
+// replayed with one launch; on small meshes the per-kernel launch costs more
+// than the sweep's work.  The kernels are the same, so are the results.
+// UFE_SOR_GRAPH=0 in the environment keeps eager launches.
+static int sor_batch(ufe_ctx *ctx, const AaAcMesh &m, double omega, double tol, int max_sweeps, int nb) {
+  const char *env = std::getenv("UFE_SOR_GRAPH");
+  const bool graphs = !(env && env[0] == '0');
+  int k = 0;
+  while (k < 8 && (1 << k) < nb) k++;
+  const uint8_t *skip = ctx->gl_active ? ctx->d_skip : nullptr;
+  if (!graphs || ctx->prof || k >= 8 || (1 << k) != nb) {
+    for (int b = 0; b < nb; b++) enqueue_sweep_fp64(ctx, m, omega, tol, max_sweeps);
+    LAUNCHCHK(ctx);
+    return UFE_OK;
+  }
+  ufe_ctx::SorGraph &G = ctx->sor_graph[k];
+  if (!G.exec || G.omega != omega || G.tol != tol || G.max_sweeps != max_sweeps || G.skip != skip) {
+    if (G.exec) (void)hipGraphExecDestroy(G.exec);
+    G = ufe_ctx::SorGraph{};
+    hipGraph_t g = nullptr;
+    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int b = 0; b < nb; b++) enqueue_sweep_fp64(ctx, m, omega, tol, max_sweeps);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e3 = (e1 == hipSuccess && e2 == hipSuccess) ? hipGraphInstantiate(&ex, g, nullptr, nullptr, 0)
+                                                                 : hipErrorUnknown;
+    if (g) (void)hipGraphDestroy(g);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+      ctx->err = std::string("reference sweep graph: ") +
+                 hipGetErrorString(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3));
+      return UFE_HIP_ERROR;
+    }
+    G.exec = ex;
+    G.nb = nb;
+    G.omega = omega;
+    G.tol = tol;
+    G.max_sweeps = max_sweeps;
+    G.skip = skip;
+  }
+  HIPCHK(ctx, hipGraphLaunch(G.exec, ctx->stream));
+  return UFE_OK;
+}
+
 static int run_sor_reference(ufe_ctx *ctx, double tol, int max_sweeps, SorFlags *out, double *seconds) {
   const AaAcMesh m = aaac_mesh(ctx);
   const double omega = ctx->params.SSA_SOR_omega;
@@ -2755,9 +2812,9 @@ static int run_sor_reference(ufe_ctx *ctx, double tol, int max_sweeps, SorFlags 
   auto t0 = std::chrono::steady_clock::now();
   while (true) {
     const int nb = std::min(batch, max_sweeps - queued);
-    for (int b = 0; b < nb; b++) enqueue_sweep_fp64(ctx, m, omega, tol, max_sweeps);
+    int r;
+    if ((r = sor_batch(ctx, m, omega, tol, max_sweeps, nb))) return r;
     queued += nb;
-    LAUNCHCHK(ctx);
     HIPCHK(ctx, hipMemcpyAsync(ctx->h_flags, ctx->d_flags, sizeof(SorFlags), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     prof_collect(ctx);
