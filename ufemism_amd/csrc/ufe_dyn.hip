@@ -135,6 +135,7 @@ struct AmgLevel {
   int32_t *agg = nullptr;           // row / point above -> this level's point (-1: not aggregated)
   int nmem = 0, nagg = 0;           // lengths of mem and agg
   float2 *ev32_img = nullptr;       // ev32's place in the small-level kernel's image, or null
+  bool blocks = false;              // F holds the fused kernels' blocks and halos (K-step dots by block)
   bool fuse = false;                // cycle steps by k_amg_fuse_pre / k_amg_fuse_post
   AmgFuse F{};
   int fuse_blocks = 0;
@@ -2172,12 +2173,15 @@ constexpr size_t AMG_TAIL_LDS_MARGIN = 256;     // bytes of LDS left unused by k
 constexpr int AMG_FUSE_MAX_POINTS = 1 << 24;   // levels up to this size take the fused cycle kernels
 constexpr size_t AMG_FUSE_LDS_MAX = 56 * 1024;  // their LDS arrays (36 bytes per listed point; 6 KiB static beside)
 
-// Coarse levels of at most this many points run their cycle steps in the fused
-// kernels when their block lists fit in LDS (UFE_AMG_FUSE=0 in the environment
-// at set-up turns them off, UFE_AMG_FUSE_MAX=n sets the size limit).
-static int amg_fuse_max() {
+// Coarse levels of at most this many points get the fused kernels' blocks and
+// halos when their block lists fit in LDS (UFE_AMG_FUSE_MAX=n sets the size
+// limit); their K-step dots follow those blocks whether the cycle steps run
+// fused or, with UFE_AMG_FUSE=0 in the environment at set-up, one kernel each.
+static bool amg_fuse_on() {
   const char *off = std::getenv("UFE_AMG_FUSE");
-  if (off && off[0] == '0') return 0;
+  return !(off && off[0] == '0');
+}
+static int amg_fuse_max() {
   const char *env = std::getenv("UFE_AMG_FUSE_MAX");
   if (env && env[0]) {
     const long v = std::strtol(env, nullptr, 10);
@@ -2345,6 +2349,7 @@ static int amg_setup(ufe_ctx *ctx) {
   const int L = (int)ops.size();
   const int kstep_min = amg_kstep_min();
   const int fuse_max = amg_fuse_max();
+  const bool fuse_on = amg_fuse_on();
   if ((r = amg_tail_lds_budget(ctx, &ctx->amg_tail_lds))) return r;
   ctx->amg.assign(L, AmgLevel{});
   ctx->amg[0].n = n0;
@@ -2404,7 +2409,8 @@ static int amg_setup(ufe_ctx *ctx) {
         V.F = AmgFuse{H.depth, d_ext_ptr, d_ext, d_cnt, d_rstart, d_rptr, d_lcol, d_lcf, V.S, nullptr,
                       d_gptr, d_gagg, d_glo, H.max_ext};
         V.fuse_blocks = H.nblocks;
-        V.fuse = true;
+        V.blocks = true;
+        V.fuse = fuse_on;
         if (H.transposed_ok) {   // the same blocks for the transposed hierarchy
           std::vector<float2> lcfT(std::max<size_t>(H.lsrcT.size(), 1));
           for (size_t k = 0; k < H.lsrcT.size(); k++) lcfT[k] = fT[H.lsrcT[k]];
@@ -2444,7 +2450,7 @@ static int amg_setup(ufe_ctx *ctx) {
     float2 **vecs[8] = {&V.b, &V.t, &V.e1, &V.e2, &V.w1, &V.w2, &V.rc2, &V.r};
     for (float2 **v : vecs)
       if ((r = dalloc(ctx, v, (size_t)C.n))) return r;
-    if ((r = dalloc(ctx, &V.part, (size_t)3 * AMG_DOT_BLOCKS))) return r;
+    if ((r = dalloc(ctx, &V.part, (size_t)3 * std::max(AMG_DOT_BLOCKS, V.fuse_blocks)))) return r;
     if ((r = dalloc(ctx, &V.sc, (size_t)8))) return r;
     HIPCHK(ctx, hipMemset(V.sc, 0, 8 * sizeof(double)));
   }
@@ -2522,9 +2528,25 @@ static bool amg_fused_cycle(const ufe_ctx *ctx, int l) {
   return l >= 1 && V.fuse && (!ctx->amg_T || V.fuseT) && std::max(1, ctx->amg_nu) <= V.F.depth;
 }
 
+// The K-step dots of a cycle's result (y = A x, and y.a, y.y, y.c by block),
+// which the fused post-smoothing computes itself when its halos reach one ring
+// beyond the smoothing; done reports whether it did.
+struct AmgDots {
+  const float2 *a;                  // null: the cycle's right-hand side
+  bool c_rhs;                       // y.c with c the right-hand side
+  float2 *y;
+  double *part;
+  bool done;
+};
+
+// K-step partial sums of level V: one per fused-kernel block when it has them
+static int amg_dot_count(const AmgLevel &V) {
+  return V.blocks ? V.fuse_blocks : std::min(AMG_DOT_BLOCKS, nblk(V.n, 256));
+}
+
 // x ~ A_l^-1 b: nu block-Jacobi sweeps, the coarse correction, nu sweeps.  The
 // 2 nu sweeps alternate between t and x, starting in t, so they end in x.
-static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
+static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x, AmgDots *dots = nullptr) {
   AmgLevel &V = ctx->amg[l];
   const AmgOp A = amg_op(ctx, V);
   const int nb = nblk(V.n), last = (int)ctx->amg.size() - 1;
@@ -2556,10 +2578,13 @@ static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x) {
     k_amg_fuse_pre<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(F, nu, b, V.t, C.b, AMG_BJ_OMEGA);
     amg_solve_level(ctx, l + 1);
     // a K-step level below leaves its step-2 partials to this kernel (amg_solve_level)
-    const int cnd = std::min(AMG_DOT_BLOCKS, nblk(C.n, 256));
+    const int cnd = amg_dot_count(C);
+    const bool dot = dots && V.F.depth >= nu + 1;
     k_amg_fuse_post<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(
         F, nu, b, V.t, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, C.kstep ? C.part : nullptr, cnd, x,
-        AMG_BJ_OMEGA);
+        AMG_BJ_OMEGA, dot ? dots->a : nullptr, dot && dots->c_rhs ? 1 : 0, dot ? dots->y : nullptr,
+        dot ? dots->part : nullptr);
+    if (dot) dots->done = true;
     return;
   }
   float2 *cur = V.t, *oth = x;
@@ -2633,19 +2658,31 @@ static void amg_solve_level(ufe_ctx *ctx, int l) {
     return;
   }
   AmgLevel &V = ctx->amg[l];
-  amg_cycle(ctx, l, V.b, V.e1);
+  AmgDots d1{nullptr, false, V.w1, V.part, false};
+  amg_cycle(ctx, l, V.b, V.e1, V.kstep ? &d1 : nullptr);
   if (!V.kstep) return;
   const AmgOp A = amg_op(ctx, V);
-  const int nd = std::min(AMG_DOT_BLOCKS, nblk(V.n, 256));
-  k_amg_matvec_dot<<<nd, 256, 0, ctx->stream>>>(A, V.e1, V.w1, V.b, nullptr, V.part);
+  const int nd = amg_dot_count(V);
+  if (!d1.done) {
+    if (V.blocks)
+      k_amg_dot_blocks<<<V.fuse_blocks, AMG_FUSE_BLOCK, 0, ctx->stream>>>(A, V.F, V.e1, V.w1, V.b, nullptr, V.part);
+    else
+      k_amg_matvec_dot<<<nd, 256, 0, ctx->stream>>>(A, V.e1, V.w1, V.b, nullptr, V.part);
+  }
   if (l >= 2) {   // the step-1 sums in every block of the update (no launch of their own)
     k_amg_kfin1_axpy<<<nblk(V.n, 256), 256, 0, ctx->stream>>>(nd, V.part, V.sc, V.n, V.b, V.w1, V.rc2);
   } else {
     k_amg_kfin<<<1, 256, 0, ctx->stream>>>(nd, V.part, V.sc, 1);
     k_amg_axpy<<<nblk(V.n), BLOCK, 0, ctx->stream>>>(V.n, V.b, V.w1, V.sc, V.rc2);
   }
-  amg_cycle(ctx, l, V.rc2, V.e2);
-  k_amg_matvec_dot<<<nd, 256, 0, ctx->stream>>>(A, V.e2, V.w2, V.w1, V.rc2, V.part);
+  AmgDots d2{V.w1, true, V.w2, V.part, false};
+  amg_cycle(ctx, l, V.rc2, V.e2, &d2);
+  if (!d2.done) {
+    if (V.blocks)
+      k_amg_dot_blocks<<<V.fuse_blocks, AMG_FUSE_BLOCK, 0, ctx->stream>>>(A, V.F, V.e2, V.w2, V.w1, V.rc2, V.part);
+    else
+      k_amg_matvec_dot<<<nd, 256, 0, ctx->stream>>>(A, V.e2, V.w2, V.w1, V.rc2, V.part);
+  }
   // the step-2 coefficients: computed by the fused post-smoothing of the level
   // above from these partials, or here for the other prolongations
   if (!amg_fused_cycle(ctx, l - 1)) k_amg_kfin<<<1, 256, 0, ctx->stream>>>(nd, V.part, V.sc, 2);
