@@ -336,3 +336,40 @@ def test_fused_cycle_kernels_are_bit_identical():
     for (ma, ua), (mb, ub) in zip(out["1"], out["0"]):
         assert ma == mb
         assert np.array_equal(ua, ub)
+
+
+@pytest.mark.parametrize("depth", [1, 2])
+def test_fused_cycle_kernels_with_shallow_halos(depth):
+    """Halos of depth 1 or 2 (UFE_AMG_FUSE_DEPTH, the fallback for lists too
+    long for LDS): the fused kernels run for up to that many sweeps, the K-step
+    dots go through k_amg_dot_blocks, and the results equal the per-step
+    kernels bit for bit (UFE_AMG_FUSE=0 with the same blocks)."""
+    import os
+    out = {}
+    try:
+        os.environ["UFE_AMG_FUSE_DEPTH"] = str(depth)
+        for flag in ("1", "0"):
+            os.environ["UFE_AMG_FUSE"] = flag
+            ctx = abi.Context(0)
+            try:
+                _linear_system(ctx)
+                res = []
+                for nu in (1, 2):
+                    st, uv = _solve(ctx, 1e-9, precision=abi.UFE_MIXED, gmres=20, precond_sweeps=nu,
+                                    precond_type=abi.UFE_PRECOND_MULTIGRID)
+                    assert st.status == 0
+                    res.append((st.matvecs, uv))
+                info = ctx.multigrid_info()
+                print(f"  depth {depth} UFE_AMG_FUSE={flag}: fused {info['fused']} matvecs {[m for m, _ in res]}",
+                      flush=True)
+                if flag == "1":
+                    assert max(info["fused"]) == depth
+                out[flag] = res
+            finally:
+                ctx.close()
+    finally:
+        os.environ.pop("UFE_AMG_FUSE", None)
+        os.environ.pop("UFE_AMG_FUSE_DEPTH", None)
+    for (ma, ua), (mb, ub) in zip(out["1"], out["0"]):
+        assert ma == mb
+        assert np.array_equal(ua, ub)

@@ -2181,6 +2181,16 @@ static bool amg_fuse_on() {
   const char *off = std::getenv("UFE_AMG_FUSE");
   return !(off && off[0] == '0');
 }
+// Deepest halo to build (UFE_AMG_FUSE_DEPTH=1..3 in the environment at set-up
+// lowers it; a level whose lists do not fit in LDS takes a shallower one).
+static int amg_fuse_depth() {
+  const char *env = std::getenv("UFE_AMG_FUSE_DEPTH");
+  if (env && env[0]) {
+    const long v = std::strtol(env, nullptr, 10);
+    if (v >= 1 && v <= AMG_FUSE_NU) return (int)v;
+  }
+  return AMG_FUSE_NU;
+}
 static int amg_fuse_max() {
   const char *env = std::getenv("UFE_AMG_FUSE_MAX");
   if (env && env[0]) {
@@ -2385,7 +2395,7 @@ static int amg_setup(ufe_ctx *ctx) {
       std::vector<int32_t> gptr, gagg;
       amg_blocks_by_aggregates(ops[l + 1].n, ops[l + 1].ptr, ops[l + 1].col, mptrs[l], AMG_FUSE_BLOCK, gptr, gagg);
       AmgHalos H;
-      for (int depth = AMG_FUSE_NU; depth >= 1; depth--) {
+      for (int depth = amg_fuse_depth(); depth >= 1; depth--) {
         amg_block_halos(C.n, C.ptr, C.col, gptr, gagg, mptrs[l], mems[l], depth, H, &pT, &cT);
         if ((size_t)H.max_ext * 36 <= AMG_FUSE_LDS_MAX) break;
       }
