@@ -2735,7 +2735,10 @@ static int amg_capture(ufe_ctx *ctx) {
 static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float2 *v, float2 *z) {
   const int n = ctx->nA, nu = std::max(1, ctx->amg_nu);
   k_fill<float2><<<nblk(n), BLOCK, 0, ctx->stream>>>(z, n, make_float2(0.0f, 0.0f));
-  for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, false, false, true, k == 0);
+  // the Neumann rows are brought up to date after the last sweep of each
+  // smoothing only: the residual and the output need them, the smoother's own
+  // sweeps converge as well with them one sweep behind (same iteration counts)
+  for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, false, false, k == nu - 1, k == 0);
   AmgLevel &C = ctx->amg[1];
   k_matvec_residual<<<nblk(n), BLOCK, 0, ctx->stream>>>(am, ctx->d_cU32, ctx->d_Sxy32, ctx->d_EV32,
                                                        active_ranges(ctx), z, v, ctx->amg_y);
@@ -2754,7 +2757,7 @@ static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float
   k_amg_prolong<<<nblk(n), BLOCK, 0, ctx->stream>>>(n, C.agg, C.e1, C.e2, C.kstep ? C.sc : nullptr, z,
                                                    ctx->gl_active ? ctx->d_skip : nullptr);
   neumann_f32(ctx, z);
-  for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, true, false, true);
+  for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, true, false, k == nu - 1);
 }
 
 // The same for an FP64 Krylov method: the cycle runs in FP32 on rounded copies
