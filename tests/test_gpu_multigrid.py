@@ -249,15 +249,19 @@ def test_multigrid_on_tiny_meshes(n):
         ctx.close()
 
 
-def test_small_level_kernel_is_bit_identical():
+@pytest.mark.parametrize("images", ["1", "0"])
+def test_small_level_kernel_is_bit_identical(images):
     """The levels below the last K-step level solved by one workgroup per visit
-    (k_amg_tail) give the same bits as one kernel per step (UFE_AMG_TAIL=0 at
-    set-up): the forward mixed solve and both directions of the multigrid
-    adjoint, whose backward cycles the transposed hierarchy."""
+    (k_amg_tail), with their constant arrays staged in LDS from the set-up's
+    images or read from global memory (UFE_AMG_TAIL_IMAGES=0), give the same
+    bits as one kernel per step (UFE_AMG_TAIL=0 at set-up): the forward mixed
+    solve and both directions of the multigrid adjoint, whose backward cycles
+    the transposed hierarchy."""
     import os
     import torch
     from ufemism_amd.adjoint import AdjointOptions, SSAProblem, ssa_velocities
     out = {}
+    os.environ["UFE_AMG_TAIL_IMAGES"] = images
     try:
         for flag in ("1", "0"):
             os.environ["UFE_AMG_TAIL"] = flag
@@ -285,6 +289,7 @@ def test_small_level_kernel_is_bit_identical():
                 ctx.close()
     finally:
         os.environ.pop("UFE_AMG_TAIL", None)
+        os.environ.pop("UFE_AMG_TAIL_IMAGES", None)
     a, b = out["1"], out["0"]
     assert a[0] == b[0]
     for k in (1, 2, 3):

@@ -2244,7 +2244,8 @@ static int amg_tail_images(ufe_ctx *ctx) {
   ctx->amg_tail_imgA = ctx->amg_tail_imgAT = ctx->amg_tail_imgB = nullptr;
   ctx->amg_tail_wA = ctx->amg_tail_wB = 0;
   const int l0 = ctx->amg_tail_l0, L = (int)ctx->amg.size();
-  if (l0 <= 0) return UFE_OK;
+  const char *env = std::getenv("UFE_AMG_TAIL_IMAGES");   // 0: read the arrays from global memory
+  if (l0 <= 0 || (env && env[0] == '0')) return UFE_OK;
   size_t vec = 0, bA = 0, bB = 0;
   for (int k = 0; k < L - l0; k++) {
     const AmgLevel &V = ctx->amg[l0 + k];
