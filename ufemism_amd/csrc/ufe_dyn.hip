@@ -2191,6 +2191,31 @@ static int amg_fuse_depth() {
   }
   return AMG_FUSE_NU;
 }
+// Most points a fused-kernel block owns on coarse level l (1-based) of n
+// points: the largest power of two up to n / 512, within 32..AMG_FUSE_BLOCK,
+// so that a level has about 512 blocks or more (two per CU) unless its blocks
+// are already small.  The coarse levels' kernels are bound by their dependent
+// steps, and a block's steps take fewer rounds of its threads when it lists
+// fewer points; the halo work this repeats is cheap on those levels.  The l-th
+// entry of UFE_AMG_FUSE_OWN (a comma list, its last entry repeated; each
+// 16..AMG_FUSE_BLOCK) overrides it.  Only the order of the K-step dots' partial
+// sums depends on the blocks.
+static int amg_fuse_own(int l, int n) {
+  int v = 32;
+  while (v < AMG_FUSE_BLOCK && 2 * v <= n / 512) v *= 2;
+  const char *env = std::getenv("UFE_AMG_FUSE_OWN");
+  if (env && env[0]) {
+    const char *s = env;
+    for (int k = 1; k <= l && *s; k++) {
+      char *end = nullptr;
+      const long x = std::strtol(s, &end, 10);
+      if (end == s) break;
+      if (x >= 16 && x <= AMG_FUSE_BLOCK) v = (int)x;
+      s = *end == ',' ? end + 1 : end;
+    }
+  }
+  return v;
+}
 static int amg_fuse_max() {
   const char *env = std::getenv("UFE_AMG_FUSE_MAX");
   if (env && env[0]) {
@@ -2394,7 +2419,7 @@ static int amg_setup(ufe_ctx *ctx) {
     if ((r = dupload(ctx, &V.S, S32.data(), S32.size()))) return r;
     if (l < L - 1 && C.n <= fuse_max) {   // the fused cycle kernels' blocks and halos, as deep as fit
       std::vector<int32_t> gptr, gagg;
-      amg_blocks_by_aggregates(ops[l + 1].n, ops[l + 1].ptr, ops[l + 1].col, mptrs[l], AMG_FUSE_BLOCK, gptr, gagg);
+      amg_blocks_by_aggregates(ops[l + 1].n, ops[l + 1].ptr, ops[l + 1].col, mptrs[l], amg_fuse_own(l, C.n), gptr, gagg);
       AmgHalos H;
       for (int depth = amg_fuse_depth(); depth >= 1; depth--) {
         amg_block_halos(C.n, C.ptr, C.col, gptr, gagg, mptrs[l], mems[l], depth, H, &pT, &cT);
