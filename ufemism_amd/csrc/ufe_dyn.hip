@@ -272,6 +272,7 @@ struct ufe_ctx {
   std::vector<AmgLevel> amg;        // [0] unused (the fine level), coarse levels from 1
   bool amg_ready = false, amg_direct = false, pc_amg = false;
   int amg_nu = 2;                   // smoothing sweeps before and after the coarse correction
+  int amg_kfold_max = 0;            // most K-step partials whose step-1 finish a fused cycle folds in
   double *amg_inv = nullptr;
   float2 *amg_y = nullptr;          // fine-level residual v - A z of the restriction
   struct SorGraph {                    // a batch of reference FP64 sweeps, captured
@@ -2216,6 +2217,21 @@ static int amg_fuse_own(int l, int n) {
   }
   return v;
 }
+// A fused level's second K-step cycle finishes step 1 in its pre-smoothing
+// kernel, every block summing the step-1 partials itself, when the level has
+// at most this many partials; larger levels keep the finisher kernels, where
+// the sums would cost each of the many blocks more than the launch they save
+// (UFE_AMG_KFOLD_MAX=n in the environment at set-up sets the limit, 0 turns
+// the folding off).
+constexpr int AMG_KFOLD_MAX_PARTS = 1024;
+static int amg_kfold_max() {
+  const char *env = std::getenv("UFE_AMG_KFOLD_MAX");
+  if (env && env[0]) {
+    const long v = std::strtol(env, nullptr, 10);
+    if (v >= 0 && v < (1L << 30)) return (int)v;
+  }
+  return AMG_KFOLD_MAX_PARTS;
+}
 static int amg_fuse_max() {
   const char *env = std::getenv("UFE_AMG_FUSE_MAX");
   if (env && env[0]) {
@@ -2386,6 +2402,7 @@ static int amg_setup(ufe_ctx *ctx) {
   const int kstep_min = amg_kstep_min();
   const int fuse_max = amg_fuse_max();
   const bool fuse_on = amg_fuse_on();
+  ctx->amg_kfold_max = amg_kfold_max();
   if ((r = amg_tail_lds_budget(ctx, &ctx->amg_tail_lds))) return r;
   ctx->amg.assign(L, AmgLevel{});
   ctx->amg[0].n = n0;
@@ -2580,9 +2597,22 @@ static int amg_dot_count(const AmgLevel &V) {
   return V.blocks ? V.fuse_blocks : std::min(AMG_DOT_BLOCKS, nblk(V.n, 256));
 }
 
+// The step-1 finish of a K-step (alpha1 from the partials, rc2 = b - alpha1 w1)
+// done by the second cycle's fused pre-smoothing kernel (amg_kfold_max).
+struct AmgFold {
+  const float2 *w1;
+  int knb;
+  const double *kpart;
+  double *sc;
+  float2 *rc2;
+};
+
 // x ~ A_l^-1 b: nu block-Jacobi sweeps, the coarse correction, nu sweeps.  The
-// 2 nu sweeps alternate between t and x, starting in t, so they end in x.
-static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x, AmgDots *dots = nullptr) {
+// 2 nu sweeps alternate between t and x, starting in t, so they end in x.  With
+// fold given (fused levels only) the right-hand side is fold->rc2, which the
+// pre-smoothing kernel forms from b.
+static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x, AmgDots *dots = nullptr,
+                      const AmgFold *fold = nullptr) {
   AmgLevel &V = ctx->amg[l];
   const AmgOp A = amg_op(ctx, V);
   const int nb = nblk(V.n), last = (int)ctx->amg.size() - 1;
@@ -2611,7 +2641,10 @@ static void amg_cycle(ufe_ctx *ctx, int l, const float2 *b, float2 *x, AmgDots *
     }
     // the blocks are whole aggregates of level l + 1: the pre-smoothing kernel
     // also sums their members' residuals into C.b (as k_amg_gather)
-    k_amg_fuse_pre<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(F, nu, b, V.t, C.b, AMG_BJ_OMEGA);
+    k_amg_fuse_pre<<<V.fuse_blocks, AMG_FUSE_BLOCK, lds, ctx->stream>>>(
+        F, nu, b, V.t, C.b, AMG_BJ_OMEGA, fold ? fold->w1 : nullptr, fold ? fold->knb : 0,
+        fold ? fold->kpart : nullptr, fold ? fold->sc : nullptr, fold ? fold->rc2 : nullptr);
+    if (fold) b = fold->rc2;
     amg_solve_level(ctx, l + 1);
     // a K-step level below leaves its step-2 partials to this kernel (amg_solve_level)
     const int cnd = amg_dot_count(C);
@@ -2705,14 +2738,21 @@ static void amg_solve_level(ufe_ctx *ctx, int l) {
     else
       k_amg_matvec_dot<<<nd, 256, 0, ctx->stream>>>(A, V.e1, V.w1, V.b, nullptr, V.part);
   }
-  if (l >= 2) {   // the step-1 sums in every block of the update (no launch of their own)
+  const bool fold = amg_fused_cycle(ctx, l) && nd <= ctx->amg_kfold_max;
+  const AmgFold F2{V.w1, nd, V.part, V.sc, V.rc2};
+  if (fold) {
+    // the second cycle's pre-smoothing kernel finishes step 1
+  } else if (l >= 2) {   // the step-1 sums in every block of the update (no launch of their own)
     k_amg_kfin1_axpy<<<nblk(V.n, 256), 256, 0, ctx->stream>>>(nd, V.part, V.sc, V.n, V.b, V.w1, V.rc2);
   } else {
     k_amg_kfin<<<1, 256, 0, ctx->stream>>>(nd, V.part, V.sc, 1);
     k_amg_axpy<<<nblk(V.n), BLOCK, 0, ctx->stream>>>(V.n, V.b, V.w1, V.sc, V.rc2);
   }
   AmgDots d2{V.w1, true, V.w2, V.part, false};
-  amg_cycle(ctx, l, V.rc2, V.e2, &d2);
+  if (fold)
+    amg_cycle(ctx, l, V.b, V.e2, &d2, &F2);
+  else
+    amg_cycle(ctx, l, V.rc2, V.e2, &d2);
   if (!d2.done) {
     if (V.blocks)
       k_amg_dot_blocks<<<V.fuse_blocks, AMG_FUSE_BLOCK, 0, ctx->stream>>>(A, V.F, V.e2, V.w2, V.w1, V.rc2, V.part);
