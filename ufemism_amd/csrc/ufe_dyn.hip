@@ -222,6 +222,8 @@ struct ufe_ctx {
   double *bic_S = nullptr;          // BiCGStab scalars on the device
   double *kry_part = nullptr, *kry_H1 = nullptr, *kry_H2 = nullptr, *kry_Hs = nullptr, *kry_y = nullptr,
          *kry_nrm = nullptr;
+  double *kry_g = nullptr;          // copies of d_scalar[0..2], after kry_Hs in the same block as kry_H1
+  double *h_kry = nullptr;          // pinned host mirror of that block, then the least-squares solution
   // algorithmic bytes per colour (SOR rows only)
   double bytes64[5] = {0, 0, 0, 0, 0}, bytes32[5] = {0, 0, 0, 0, 0};
   double degsum = 0.0;
@@ -370,7 +372,7 @@ static void free_mesh(ufe_ctx *ctx) {
   ctx->kry_V = ctx->kry_Z = ctx->kry_w = ctx->kry_x = nullptr;
   ctx->kry_R64 = nullptr;
   ctx->bic_S = nullptr;
-  ctx->kry_part = ctx->kry_H1 = ctx->kry_H2 = ctx->kry_Hs = ctx->kry_y = ctx->kry_nrm = nullptr;
+  ctx->kry_part = ctx->kry_H1 = ctx->kry_H2 = ctx->kry_Hs = ctx->kry_y = ctx->kry_nrm = ctx->kry_g = nullptr;
   ctx->adj_m = ctx->adj_K = ctx->adj_nck = 0;
   ctx->adj_ready = false;
   ctx->adj_V = ctx->adj_Z = ctx->adj_w = ctx->adj_r = ctx->adj_g = ctx->adj_lam = ctx->adj_ubar = ctx->adj_uK =
@@ -623,6 +625,8 @@ int ufe_ctx_destroy(ufe_ctx *ctx) {
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
   if (ctx->h_scalar) (void)hipHostFree(ctx->h_scalar);
+  if (ctx->h_kry) (void)hipHostFree(ctx->h_kry);
+  if (ctx->h_bk_rel) (void)hipHostFree(ctx->h_bk_rel);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return UFE_OK;
@@ -2117,8 +2121,9 @@ static void neumann_f32(ufe_ctx *ctx, float2 *D) {
 
 // skip_first: leave out the first colour of the pass (a preconditioner that
 // alternates direction would otherwise repeat the previous pass's last colour
-// with nothing changed in between).  from_zero: D holds zeros, so the first
-// colour's rows need neither neighbours nor coefficients (k_sor_zero_colour_f32).
+// with nothing changed in between).  from_zero: the sweep starts from D = 0
+// whatever D holds: one kernel zeroes D and updates the first colour's rows,
+// which need neither neighbours nor coefficients (k_sor_zero_colour_f32).
 static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, float2 *D = nullptr,
                                const float2 *R = nullptr, bool backward = false, bool skip_first = false,
                                bool neumann = true, bool from_zero = false) {
@@ -2129,14 +2134,15 @@ static void enqueue_sweep_fp32(ufe_ctx *ctx, const AaAcMesh &m, float omega, flo
   q.skip = ctx->gl_active ? ctx->d_skip : nullptr;
   if (!D) D = ctx->d_D32;
   if (!R) R = ctx->d_R32;
-  for (int k = skip_first ? 1 : 0; k < 5; k++) {
+  if (from_zero) {
+    const int c = backward ? 4 : 0;
+    k_sor_zero_colour_f32<<<nblk(ctx->nA), BLOCK, 0, ctx->stream>>>(
+        ctx->nA, q.EV, R, q.skip, D, ctx->colour_start[c], skip_first ? 0 : ctx->colour_active[c], omega);
+  }
+  for (int k = skip_first || from_zero ? 1 : 0; k < 5; k++) {
     const int c = backward ? 4 - k : k;
     const int n = ctx->colour_active[c];
     if (n == 0) continue;
-    if (from_zero && k == 0) {
-      k_sor_zero_colour_f32<<<nblk(n), BLOCK, 0, ctx->stream>>>(q.EV, R, q.skip, D, ctx->colour_start[c], n, omega);
-      continue;
-    }
     launch_sor_f32(ctx, ctx->tune_bs, ctx->tune_sched, ctx->tune_loads, m, q, D, R, c, omega);
   }
   if (!neumann) return;
@@ -2800,8 +2806,8 @@ static int amg_capture(ufe_ctx *ctx) {
 // the Neumann condition on the edge rows, like the SOR preconditioner's.
 static void amg_apply(ufe_ctx *ctx, const AaAcMesh &am, float omega, const float2 *v, float2 *z) {
   const int n = ctx->nA, nu = std::max(1, ctx->amg_nu);
-  k_fill<float2><<<nblk(n), BLOCK, 0, ctx->stream>>>(z, n, make_float2(0.0f, 0.0f));
-  // the Neumann rows are brought up to date after the last sweep of each
+  // the first sweep starts from z = 0 (its first colour pass sets all of z).
+  // The Neumann rows are brought up to date after the last sweep of each
   // smoothing only: the residual and the output need them, the smoother's own
   // sweeps converge as well with them one sweep behind (same iteration counts)
   for (int k = 0; k < nu; k++) enqueue_sweep_fp32(ctx, am, omega, z, v, false, false, k == nu - 1, k == 0);
@@ -3167,6 +3173,21 @@ static ActiveRanges active_ranges(ufe_ctx *ctx) {
   return a;
 }
 
+// The Krylov block read back to the host: H1 and H2 (65 x 64 each), Hs (65)
+// and copies of d_scalar[0..2] (kry_g), in one device allocation, so that one
+// copy into pinned memory brings all of it (four copies into pageable memory,
+// each waiting for the one before, had left the device idle for about 0.1 ms
+// per check).
+constexpr size_t KRY_HBLOCK = 2 * 65 * 64 + 65 + 4;
+static int kry_readback(ufe_ctx *ctx, const double **out) {
+  k_copy_f64<<<1, 64, 0, ctx->stream>>>(3, ctx->d_scalar, ctx->kry_g);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->h_kry, ctx->kry_H1, sizeof(double) * KRY_HBLOCK, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *out = ctx->h_kry;
+  return UFE_OK;
+}
+
 static int krylov_alloc(ufe_ctx *ctx, int m, bool f32) {
   if (ctx->kry_V && ctx->kry_m >= m && ctx->kry_f32 == f32) return UFE_OK;
   const size_t n = ctx->nA, es = f32 ? sizeof(float2) : sizeof(double2);
@@ -3183,10 +3204,13 @@ static int krylov_alloc(ufe_ctx *ctx, int m, bool f32) {
   HIPCHK(ctx, hipMemsetAsync(V, 0, es * n * (m + 1), ctx->stream));
   if (!ctx->kry_R64 && (r = dalloc(ctx, &ctx->kry_R64, n))) return r;
   if (!ctx->kry_part && (r = dalloc(ctx, &ctx->kry_part, (size_t)64 * DOT_BLOCKS))) return r;
-  if (!ctx->kry_H1) {
-    if ((r = dalloc(ctx, &ctx->kry_H1, 65 * 64))) return r;
-    if ((r = dalloc(ctx, &ctx->kry_H2, 65 * 64))) return r;
-    if ((r = dalloc(ctx, &ctx->kry_Hs, 65))) return r;
+  if (!ctx->kry_H1) {   // one block, read back by one copy: H1, H2, Hs, the scalars
+    if ((r = dalloc(ctx, &ctx->kry_H1, KRY_HBLOCK))) return r;
+    ctx->kry_H2 = ctx->kry_H1 + 65 * 64;
+    ctx->kry_Hs = ctx->kry_H2 + 65 * 64;
+    ctx->kry_g = ctx->kry_Hs + 65;
+    // the pinned mirror, and 64 more for the least-squares solution
+    if (!ctx->h_kry) HIPCHK(ctx, hipHostMalloc((void **)&ctx->h_kry, sizeof(double) * (KRY_HBLOCK + 64)));
     if ((r = dalloc(ctx, &ctx->kry_y, 65))) return r;
     if ((r = dalloc(ctx, &ctx->kry_nrm, 4))) return r;
   }
@@ -3304,15 +3328,12 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
     k_normalize<T><<<nb, BLOCK, 0, ctx->stream>>>(n, w, ctx->kry_nrm, V + (int64_t)(j + 1) * n, ctx->kry_Hs + j);
     if (target > 0.0 && j + 1 < m && (j + 1) % GMRES_CHECK == 0) {
       const int k = j + 1;
-      std::vector<double> G1(65 * k), G2(65 * k), Gs(k), g(4), yy;
-      HIPCHK(ctx, hipMemcpyAsync(G1.data(), ctx->kry_H1, sizeof(double) * 65 * k, hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHK(ctx, hipMemcpyAsync(G2.data(), ctx->kry_H2, sizeof(double) * 65 * k, hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHK(ctx, hipMemcpyAsync(Gs.data(), ctx->kry_Hs, sizeof(double) * k, hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHK(ctx, hipMemcpyAsync(g.data(), ctx->d_scalar, sizeof(double) * 3, hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      std::vector<double> yy;
+      const double *hb = nullptr;
+      if (const int e = kry_readback(ctx, &hb)) return e;
       double res = HUGE_VAL;
-      const double beta = std::sqrt(g[1]);
-      hessenberg_lsq(k, G1.data(), G2.data(), Gs.data(), beta, yy, &res);
+      const double beta = std::sqrt(hb[KRY_HBLOCK - 4 + 1]);
+      hessenberg_lsq(k, hb, hb + 65 * 64, hb + 2 * 65 * 64, beta, yy, &res);
       if (res <= target * beta) {
         m = k;
         break;
@@ -3321,17 +3342,15 @@ static int gmres_cycle(ufe_ctx *ctx, int m, int ksweeps, double omega, int passe
   }
   LAUNCHCHK(ctx);
   // least squares min |beta e1 - H y| on the host (Givens rotations)
-  std::vector<double> H1(65 * 64), H2(65 * 64), Hs(65), sc(4);
-  HIPCHK(ctx, hipMemcpyAsync(H1.data(), ctx->kry_H1, sizeof(double) * 65 * m, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(H2.data(), ctx->kry_H2, sizeof(double) * 65 * m, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(Hs.data(), ctx->kry_Hs, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(sc.data(), ctx->d_scalar, sizeof(double) * 3, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const double *hb = nullptr;
+  if (const int e = kry_readback(ctx, &hb)) return e;
   prof_collect(ctx);
   std::vector<double> y;
-  const int kk = hessenberg_lsq(m, H1.data(), H2.data(), Hs.data(), std::sqrt(sc[1]), y);
+  const int kk = hessenberg_lsq(m, hb, hb + 65 * 64, hb + 2 * 65 * 64, std::sqrt(hb[KRY_HBLOCK - 4 + 1]), y);
   if (kk == 0) return UFE_OK;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->kry_y, y.data(), sizeof(double) * kk, hipMemcpyHostToDevice, ctx->stream));
+  double *hy = ctx->h_kry + KRY_HBLOCK;   // pinned: the copy reads it after this call returns
+  std::copy(y.begin(), y.begin() + kk, hy);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->kry_y, hy, sizeof(double) * kk, hipMemcpyHostToDevice, ctx->stream));
   k_lincomb<T><<<nb, BLOCK, 0, ctx->stream>>>(n, kk, Z, ctx->kry_y, x);
   k_add_to_w<T><<<nb, BLOCK, 0, ctx->stream>>>(n, x, ctx->d_pair[0]);
   neumann_w(ctx);
