@@ -140,7 +140,8 @@ struct AmgLevel {
   AmgFuse F{};
   int fuse_blocks = 0;
   bool fuseT = false;               // and for the transposed operator: its local rows
-  int32_t *fuse_rptrT = nullptr, *fuse_lcolT = nullptr;
+  int32_t *fuse_rptrT = nullptr;
+  uint16_t *fuse_lcolT = nullptr;
   float2 *fuse_lcfT = nullptr;
   float2 *b = nullptr, *t = nullptr, *e1 = nullptr, *e2 = nullptr, *w1 = nullptr, *w2 = nullptr,
          *rc2 = nullptr, *r = nullptr;
@@ -2452,14 +2453,18 @@ static int amg_setup(ufe_ctx *ctx) {
         std::vector<float2> lcf(std::max<size_t>(H.lsrc.size(), 1));
         for (size_t k = 0; k < H.lsrc.size(); k++) lcf[k] = cf32[H.lsrc[k]];
         if (H.lcol.empty()) H.lcol.push_back(0);
-        int32_t *d_ext_ptr, *d_ext, *d_cnt, *d_rstart, *d_rptr, *d_lcol, *d_gptr, *d_gagg, *d_glo;
+        // local columns index lists of at most max_ext points: 16 bits hold them
+        static_assert(AMG_FUSE_LDS_MAX / 36 < 65536, "local columns must fit in 16 bits");
+        const std::vector<uint16_t> lcol16(H.lcol.begin(), H.lcol.end());
+        int32_t *d_ext_ptr, *d_ext, *d_cnt, *d_rstart, *d_rptr, *d_gptr, *d_gagg, *d_glo;
+        uint16_t *d_lcol;
         float2 *d_lcf;
         if ((r = dupload(ctx, &d_ext_ptr, H.ext_ptr.data(), H.ext_ptr.size())) ||
             (r = dupload(ctx, &d_ext, H.ext.data(), H.ext.size())) ||
             (r = dupload(ctx, &d_cnt, H.cnt.data(), H.cnt.size())) ||
             (r = dupload(ctx, &d_rstart, H.rstart.data(), H.rstart.size())) ||
             (r = dupload(ctx, &d_rptr, H.rptr.data(), H.rptr.size())) ||
-            (r = dupload(ctx, &d_lcol, H.lcol.data(), H.lcol.size())) ||
+            (r = dupload(ctx, &d_lcol, lcol16.data(), lcol16.size())) ||
             (r = dupload(ctx, &d_lcf, lcf.data(), lcf.size())) ||
             (r = dupload(ctx, &d_gptr, H.gptr.data(), H.gptr.size())) ||
             (r = dupload(ctx, &d_gagg, H.gagg.data(), H.gagg.size())) ||
@@ -2474,8 +2479,9 @@ static int amg_setup(ufe_ctx *ctx) {
           std::vector<float2> lcfT(std::max<size_t>(H.lsrcT.size(), 1));
           for (size_t k = 0; k < H.lsrcT.size(); k++) lcfT[k] = fT[H.lsrcT[k]];
           if (H.lcolT.empty()) H.lcolT.push_back(0);
+          const std::vector<uint16_t> lcolT16(H.lcolT.begin(), H.lcolT.end());
           if ((r = dupload(ctx, &V.fuse_rptrT, H.rptrT.data(), H.rptrT.size())) ||
-              (r = dupload(ctx, &V.fuse_lcolT, H.lcolT.data(), H.lcolT.size())) ||
+              (r = dupload(ctx, &V.fuse_lcolT, lcolT16.data(), lcolT16.size())) ||
               (r = dupload(ctx, &V.fuse_lcfT, lcfT.data(), lcfT.size())))
             return r;
           V.fuseT = true;
