@@ -247,6 +247,22 @@ def test_colouring_identify_path_aborts():
     assert status == 4
 
 
+def test_colouring_rotation_path_gets_stuck():
+    """The Q5 branch looks for two non-adjacent neighbours x, y of v with
+    deg < C%nconmax; without such a pair it returns v to the rear of Q5
+    (src/mesh_five_colour_module.f90:222-228).  On the icosahedron every vertex
+    has degree 5, so with nC_mem = 5 no pair qualifies: Q5 rotates until the
+    reduction gives up after 2 nVAaAc iterations (:167-170, the 'reduce got
+    stuck' abort, status 5).  With nC_mem = 6 the first vertex has a pair and
+    identify() aborts (status 4); a threshold of deg <= k, or k taken from
+    anything but nC_mem, would change one of the two."""
+    ico = [[2, 3, 4, 5, 6], [1, 3, 6, 7, 8], [1, 2, 4, 8, 9], [1, 3, 5, 9, 10], [1, 4, 6, 10, 11],
+           [1, 2, 5, 7, 11], [2, 6, 8, 11, 12], [2, 3, 7, 9, 12], [3, 4, 8, 10, 12],
+           [4, 5, 9, 11, 12], [5, 6, 7, 10, 12], [7, 8, 9, 10, 11]]
+    assert po.colour_graph(ico, nC_mem=5)[0] == 5
+    assert po.colour_graph(ico, nC_mem=6)[0] == 4
+
+
 def test_colouring_small_known_graph():
     """A wheel with five spokes: the hub has degree 5, the rim degree 3; the
     LIFO reduction deletes rim vertices from the top of Q4 until five remain."""
