@@ -3186,6 +3186,10 @@ static ActiveRanges active_ranges(ufe_ctx *ctx) {
 // per check).
 constexpr size_t KRY_HBLOCK = 2 * 65 * 64 + 65 + 4;
 static int kry_readback(ufe_ctx *ctx, const double **out) {
+  if (!ctx->h_kry) {
+    ctx->err = "the Krylov read-back buffer was not allocated";
+    return UFE_HIP_ERROR;
+  }
   k_copy_f64<<<1, 64, 0, ctx->stream>>>(3, ctx->d_scalar, ctx->kry_g);
   HIPCHK(ctx, hipMemcpyAsync(ctx->h_kry, ctx->kry_H1, sizeof(double) * KRY_HBLOCK, hipMemcpyDeviceToHost,
                              ctx->stream));
@@ -3215,11 +3219,12 @@ static int krylov_alloc(ufe_ctx *ctx, int m, bool f32) {
     ctx->kry_H2 = ctx->kry_H1 + 65 * 64;
     ctx->kry_Hs = ctx->kry_H2 + 65 * 64;
     ctx->kry_g = ctx->kry_Hs + 65;
-    // the pinned mirror, and 64 more for the least-squares solution
-    if (!ctx->h_kry) HIPCHK(ctx, hipHostMalloc((void **)&ctx->h_kry, sizeof(double) * (KRY_HBLOCK + 64)));
     if ((r = dalloc(ctx, &ctx->kry_y, 65))) return r;
     if ((r = dalloc(ctx, &ctx->kry_nrm, 4))) return r;
   }
+  // the block's pinned mirror, and 64 more for the least-squares solution
+  // (kept with the context, across meshes)
+  if (!ctx->h_kry) HIPCHK(ctx, hipHostMalloc((void **)&ctx->h_kry, sizeof(double) * (KRY_HBLOCK + 64)));
   ctx->kry_m = m;
   ctx->kry_f32 = f32;
   return UFE_OK;
